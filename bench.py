@@ -1,0 +1,175 @@
+#!/usr/bin/env python3
+"""Retarded BEM wavelength-sweep benchmark on one MI355X.
+
+Workload: gold-Drude capped rod (10 nm diameter x 40 nm, curved triangles, 2,496 faces) in
+eps = 1.77, plane wave polarised along the rod axis, 201 wavelengths over 500-1000 nm. One
+step = the full sweep (assembly, factorisation, solve, far field, extinction for every
+wavelength), batched `--batch` wavelengths per factorisation, inputs resident on the GPU.
+
+value = wavelength solves/s of the mixed-precision path (complex64 LU chain on the matrix
+cores + FP64 iterative refinement on the exact operator); the complex128 LU path is timed
+in the same run as the reference point. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s @2.5k faces mixed vs FP64 LU; 25k-face GMRES time-to-1e-6"
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+DRUDE = (10.0, 9.065, 0.0708)
+ROD_N = [25, 11, 34]
+
+
+def build_workload():
+    import mnpbem_amd as M
+    rod = M.trirod(10.0, 40.0, ROD_N, triangles=True)
+    cp = M.ComParticle([M.EpsConst(1.77), M.EpsDrude(*DRUDE)], [rod], [[2, 1]], 1, interp="curv")
+    exc = M.PlaneWaveRet([0.0, 0.0, 1.0], [1.0, 0.0, 0.0])
+    return M, cp, exc
+
+
+def cpu_baseline(cp, nsample):
+    """The oracle (FP64 numpy/LAPACK restatement of the reference path) on the host cores:
+    per-wavelength assembly + LU chain + solve + extinction, timed over `nsample`
+    wavelengths after the wavelength-independent set-up."""
+    from oracle import bem_oracle as O
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+    fp = np.zeros(s.n, int)
+    green = O.Green(s, fp, [[2, 1]], [(np.arange(s.n), np.arange(s.n), 1.0)])
+    lams = np.linspace(500, 1000, 201)[:: max(1, 201 // max(nsample, 1))][:nsample]
+    pol, dr = np.array([0.0, 0, 1.0]), np.array([1.0, 0, 0])
+    t0 = time.perf_counter()
+    for lam in lams:
+        eps_k = [O.eps_const(lam, 1.77), O.eps_drude(lam, *DRUDE)]
+        ks = [e[1] for e in eps_k]
+        G1, G2, H1, H2 = O.matrices(green, ks)
+        e1 = np.full(s.n, eps_k[1][0])
+        e2 = np.full(s.n, eps_k[0][0])
+        k0 = 2 * np.pi / lam
+        bem = O.BEMRetOracle(G1, G2, H1, H2, s.nvec, e1, e2, k0)
+        a2, a2p = O.planewave(s.pos, s.nvec, pol, dr, lam, 1.77, np.arange(s.n))
+        z = np.zeros_like(a2)
+        sol = bem.solve(*O.excitation(s.nvec, k0, e1, e2, z, z, a2, a2p))
+        O.extinction(pol, dr, lam, ks[0], 1.77, s.pos, s.area, sol, np.zeros(0, int), np.arange(s.n))
+    dt = time.perf_counter() - t0
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([int(i.get("num_threads", 1)) for i in threadpool_info()] or [1])
+    except Exception:
+        threads = len(os.sched_getaffinity(0))
+    return dict(value=len(lams) / dt, unit="solves/s", cores=threads, kind="port",
+                sample="{} of the 201 wavelengths at n={} (assembly+LU chain+solve+extinction per wavelength, "
+                       "{:.1f} s; geometry set-up excluded)".format(len(lams), s.n, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--nb", type=int, default=32)
+    ap.add_argument("--cpu-sample", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this pool has one MI355X and the wavelength sweep "
+              "is sharded across GPUs only by design (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from mnpbem_amd import _lib
+    M, cp, exc = build_workload()
+    n = cp.nfaces
+    lams = np.linspace(500, 1000, 201)
+    torch.cuda.set_device(0)
+    g = M.CompGreenRet(cp, cp)
+
+    def sweep(prec):
+        bem = solvers[prec]
+        return bem.solve_spectrum(exc, lams, precision=prec)
+
+    solvers = {"mixed": M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g)}
+    if not args.skip_fp64:
+        solvers["double"] = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="double", green=g)
+
+    results = {}
+    for prec in solvers:
+        for _ in range(args.warmup):
+            sweep(prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = sweep(prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        results[prec] = (dt, out)
+        solvers[prec]._solver = None if prec == "double" else solvers[prec]._solver
+        torch.cuda.empty_cache()
+
+    # GEMM (dominant kernel) timing with HIP events over one extra mixed sweep
+    _lib.call("mnp_prof_enable", 1)
+    sweep("mixed")
+    torch.cuda.synchronize()
+    prof = np.zeros(3)
+    _lib.call("mnp_prof_collect", prof.ctypes.data)
+    _lib.call("mnp_prof_enable", 0)
+    gemm_ms, gemm_flops, gemm_launches = prof
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+
+    dt_mixed, out_mixed = results["mixed"]
+    value = args.steps * len(lams) / dt_mixed
+    extra = {}
+    if "double" in results:
+        dt_d, out_d = results["double"]
+        v_d = args.steps * len(lams) / dt_d
+        rel = np.max(np.abs(out_mixed["ext"] - out_d["ext"]) / np.abs(out_d["ext"]))
+        extra = dict(fp64_solves_per_s=round(v_d, 3), speedup_vs_fp64=round(value / v_d, 3),
+                     ext_rel_err_vs_fp64=float(rel))
+    flops_per_solve = 34.67 * n ** 3
+    extra["path_tflops_mixed"] = round(value * flops_per_solve / 1e12, 3)
+    extra["refine_iterations_max"] = int(max(out_mixed.get("refine_iterations", [0]) or [0]))
+    extra["fp64_fallbacks"] = len(out_mixed.get("fallback", []))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cp, args.cpu_sample)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt_mixed / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+c128-refine",
+        "data": "synthetic",
+        "config": {"workload": "BEMRet gold-Drude capped rod 10x40 nm, n={} curved triangles, eps_out=1.77, "
+                               "axial plane wave, 201 wavelengths 500-1000 nm, batch {}".format(n, args.batch),
+                   "faces": n, "wavelengths": len(lams), "batch": args.batch, "nb": args.nb},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_sub<float>", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": None, "gemm_ms_per_sweep": round(gemm_ms, 3),
+                     "gemm_launches_per_sweep": int(gemm_launches)},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

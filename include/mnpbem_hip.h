@@ -1,0 +1,146 @@
+/*
+ * libmnpbem_hip.so — C-ABI of the MI355X (gfx950) retarded BEM solve path.
+ *
+ * The reference (Yoo-JK/MNPBEM) is pure Python; its drop-in boundary for this path is the
+ * Python class API (ComParticle, CompGreenRet, BEMRet.solve, PlaneWaveRet.extinction) plus
+ * the dense linear-algebra dispatch of mnpbem/utils/gpu.py. Each entry point below replaces
+ * a named piece of that path; the Python package mnpbem_amd binds them with ctypes.
+ *
+ * Conventions
+ *   - Array arguments are device pointers (hipMalloc / torch tensors) unless noted "host".
+ *   - Complex numbers are interleaved (re, im). Matrices are column-major with leading
+ *     dimension ld; a batch holds one matrix per wavelength at `stride` elements.
+ *   - prec: 0 = complex double, 1 = complex float.
+ *   - Work is enqueued on `stream` (hipStream_t; NULL = default stream) and is asynchronous.
+ *   - Return: 0 ok; -k argument k (1-based) rejected before any work was enqueued;
+ *     1000 + hipError_t when a launch failed.
+ *   - Pivots are 0-based row indices. LU info: 0 ok, j > 0 = zero pivot in column j-1
+ *     (LAPACK zgetrf convention; the reference does not detect it, utils/gpu.py:189-191).
+ */
+#ifndef MNPBEM_HIP_H
+#define MNPBEM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI revision of this header (1). */
+int mnp_abi_version(void);
+
+/* Test hook: one wave runs v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 on known operands
+ * and writes the 64 x 4 result registers of each to out[0..255] and out[256..511] (device
+ * doubles). No counterpart in the reference. */
+int mnp_probe_mfma_layout(double* out, void* stream);
+
+/* GEMM timing hooks for roofline reports (no counterpart in the reference): while enabled,
+ * each matrix-core GEMM launch is bracketed by HIP events on its stream; collect returns
+ * out[0] = GEMM kernel ms, out[1] = algorithmic flops (8 m n k per complex GEMM),
+ * out[2] = launches since the previous collect (host array; waits for the events). */
+int mnp_prof_enable(int on);
+int mnp_prof_collect(double* out);
+
+/* G1, G2, H1, H2 of a composite particle for nlam wavelengths.
+ * Replaces CompGreenRet.eval(i, i, 'G'|'H1'|'H2', enei) (greenfun/compgreen_ret.py:507-553,
+ * :739-858) with GreenRetRefined.eval (greenfun/greenret_refined.py:654-810) per block.
+ * pos, nvec: n x 3; area: n; fpart: n particle index per face; con_in, con_out: npart x npart
+ * material index (1-based, 0 = not connected) for the inside / outside region pairs;
+ * kmat: nlam x nmat complex wavenumbers; refined pairs: ref_row/ref_col (nref), ref_g
+ * (nref x (order+1)), ref_f (nref x 3 x (order+1)) Taylor coefficients in (ik).
+ * Outputs: column-major n x n matrices at G1 + l*stride etc. for wavelength l. */
+int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const double* pos, const double* nvec,
+                       const double* area, const int* fpart, const int* con_in, const int* con_out,
+                       const double* kmat, int nref, const int* ref_row, const int* ref_col, const double* ref_g,
+                       const double* ref_f, int prec, void* G1, void* G2, void* H1, void* H2, long ld, long stride,
+                       void* stream);
+
+/* C -= A * B (m x k times k x n), batched. Matrix-core kernel behind the LU and solves.
+ * Replaces the cupy/numpy GEMMs of bem_ret.py:472-495 / :844-886. */
+int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb,
+                         long sB, void* C, long ldc, long sC, int batch, void* stream);
+
+/* In-place LU with partial pivoting of `batch` n x n matrices (panel width nb, 1..64).
+ * Replaces lu_factor_dispatch (utils/gpu.py:145-191: scipy zgetrf / cupyx getrf). */
+int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
+                      void* stream);
+
+/* Row permutation pf (P b = b[pf]) and its inverse from the pivots of mnp_getrf_batched
+ * (n <= 16384). Part of lu_solve_dispatch (utils/gpu.py:194-251). */
+int mnp_perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, void* stream);
+
+/* X = A^{-1} B for r <= 8 columns from an LU factor (B != X).
+ * Replaces lu_solve_dispatch / lu_solve_native (utils/gpu.py:194-301). */
+int mnp_getrs_batched(int prec, int n, int r, int batch, const void* LU, long lda, long sA, const int* pf, long sQ,
+                      const void* B, long ldb, long sB, void* X, long ldx, long sX, void* stream);
+
+/* X = B A^{-1} (m x n, B overwritten, X separate) from an LU factor of A.
+ * Replaces Sigma_i = H_i G_i^{-1} (bem_ret.py:472-473; GPU form :818-819 getrs trans) and the
+ * explicit Delta^{-1} (bem_ret.py:476-479, :838). */
+int mnp_rsolve_batched(int prec, int m, int n, int batch, const void* LU, long lda, long sA, const int* qinv, long sQ,
+                       void* B, long ldb, long sB, void* X, long ldx, long sX, int nb, void* stream);
+
+/* Y = A X for r <= 8 columns. Matrix-vector products of the solve (bem_ret.py:1716-1752). */
+int mnp_gemv_batched(int prec, int n, int r, int batch, const void* A, long lda, long sA, const void* X, long ldx,
+                     long sX, void* Y, long ldy, long sY, void* stream);
+
+/* M_b = s A + diag(d_b) for b < batch (A: one n x n matrix; d: batch x n complex double).
+ * Builds the quasistatic BEM matrix -(F + diag Lambda) of BEMStat._init_matrices
+ * (bem/bem_stat.py:241-404) from the k = 0 assembly. */
+int mnp_scale_diag_batched(int prec, int n, int batch, double s, const void* A, long lda, const double* d, void* M,
+                           long ld, long stride, void* stream);
+
+/* BEMRet.init for a batch of wavelengths (bem_ret.py:293-525): LU(G1), LU(G2),
+ * Sigma1 = H1 G1^{-1}, Sigma2 = H2 G2^{-1}, Delta^{-1}, Sigma = Sigma1 L1 - Sigma2 L2 +
+ * k^2 L (Delta^{-1} o n n^T) L, LU(Sigma). In: G1, G2, H1, H2; out: G1, G2 (LU), S1 (Sigma1),
+ * S2 (LU of Sigma), D (Delta^{-1}); H1, H2 destroyed. ipiv: 4 x batch x n; perm: 8 x batch x n;
+ * info: 4 x batch; nvec: n x 3; L1, L2: batch x n complex (inside / outside eps per face);
+ * kvac: batch (2 pi / lambda). */
+int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, void* H2, void* S1, void* S2, void* D,
+                      long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, const double* L1,
+                      const double* L2, const double* kvac, int nb, void* stream);
+
+/* BEMRet.solve (bem_ret.py:1569-1831, Eqs. 19-20): in = [phi a De alpha] (n x 8 per
+ * wavelength, precision prec) -> out = [sig1 h1 sig2 h2] (n x 8); work: 24 n complex per
+ * wavelength. Factors from mnp_bemret_factor. */
+int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2, const void* S1, const void* D,
+                     const void* S2, long ld, long stride, const int* perm, const double* nvec, const double* L1,
+                     const double* L2, const double* kvac, const void* in, long sIn, void* out, long sOut, void* work,
+                     long sW, void* stream);
+
+/* Exact complex-double BEM operator y = A x, or residual y = rhs - A x when rhs != NULL
+ * (BEMRetIter._afun, bem/bem_ret_iter.py:1362-1501); norms (device, 2 x batch, may be NULL):
+ * |y|^2 and |rhs|^2 per wavelength. x != y; work: 32 n complex double per wavelength. */
+int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const void* H1, const void* H2, long ld,
+                     long stride, const double* nvec, const double* L1, const double* L2, const double* kvac,
+                     const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work, long sW,
+                     double* norms, void* stream);
+
+/* Plane-wave right-hand side [phi a De alpha] (PlaneWaveRet.potential,
+ * simulation/planewave_ret.py:257-361, with BEMRet._excitation, bem_ret.py:1360-1429).
+ * side: n flags (bit 0 inside medium = excitation medium, bit 1 outside); kmed: batch complex;
+ * pol, dir: host double[3]. */
+int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, const double* kmed,
+                      const double* kvac, const double* L1, const double* L2, const double* pol, const double* dir,
+                      void* out, long so, void* stream);
+
+/* Far fields e, h in ndir directions from x = [sig1 h1 sig2 h2] (SpectrumRet._farfield,
+ * spectrum/spectrum_ret.py:231-392); out: batch x ndir x 6 complex. */
+int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, const double* kmed,
+                 const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out, void* stream);
+
+/* Arnoldi kernels of the device GMRES (BEMRetIter; the reference calls scipy.sparse.linalg.gmres,
+ * bem/bem_iter.py:45-101): h[c] = <V_c, w> and w += sum_c V_c s[c] over nv <= 128 complex double
+ * vectors of length len stored ldv apart; h, s: device arrays of 2 nv doubles. */
+int mnp_mdot(long len, int nv, const void* V, long ldv, const void* w, double* h, void* stream);
+int mnp_mupdate(long len, int nv, const void* V, long ldv, const double* s, void* w, void* stream);
+
+/* dst = cast(src) from complex double (refinement plumbing; no counterpart in the reference). */
+int mnp_cast_batched(int prec_dst, long count, int batch, const void* src, long ss, void* dst, long sd, void* stream);
+
+/* dst (complex double) += src where active[b] (refinement update; no counterpart). */
+int mnp_accum_batched(int prec_src, long count, int batch, const void* src, long ss, void* dst, long sd,
+                      const int* active, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MNPBEM_HIP_H */

@@ -1,0 +1,68 @@
+"""ctypes binding of libmnpbem_hip.so (declarations: include/mnpbem_hip.h).
+
+The library is mandatory: importing this module raises when it is missing, and every call
+raises ``RuntimeError`` on a non-zero status. There is no host fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmnpbem_hip.so")
+
+P = C.c_void_p
+I = C.c_int
+L = C.c_long
+
+# name -> argument types (all return int status)
+_SIGS = {
+    "mnp_abi_version": [],
+    "mnp_probe_mfma_layout": [P, P],
+    "mnp_prof_enable": [I],
+    "mnp_prof_collect": [P],
+    "mnp_green_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, I, P, P, P, P, L, L, P],
+    "mnp_gemm_sub_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
+    "mnp_getrf_batched": [I, I, I, P, L, L, P, L, P, I, P],
+    "mnp_perm_batched": [I, I, P, L, P, P, L, P],
+    "mnp_getrs_batched": [I, I, I, I, P, L, L, P, L, P, L, L, P, L, L, P],
+    "mnp_rsolve_batched": [I, I, I, I, P, L, L, P, L, P, L, L, P, L, L, I, P],
+    "mnp_gemv_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, P],
+    "mnp_scale_diag_batched": [I, I, I, C.c_double, P, L, P, P, L, L, P],
+    "mnp_bemret_factor": [I, I, I, P, P, P, P, P, P, P, L, L, P, P, P, P, P, P, P, I, P],
+    "mnp_bemret_solve": [I, I, I, P, P, P, P, P, L, L, P, P, P, P, P, P, L, P, L, P, L, P],
+    "mnp_bem_operator": [I, I, P, P, P, P, L, L, P, P, P, P, P, L, P, L, P, L, P, L, P, P],
+    "mnp_planewave_rhs": [I, I, P, P, P, P, P, P, P, P, P, P, L, P],
+    "mnp_farfield": [I, I, P, P, P, P, P, I, P, P, L, P, P],
+    "mnp_mdot": [L, I, P, L, P, P, P],
+    "mnp_mupdate": [L, I, P, L, P, P, P],
+    "mnp_cast_batched": [I, L, I, P, L, P, L, P],
+    "mnp_accum_batched": [I, L, I, P, L, P, L, P, P],
+}
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libmnpbem_hip.so not found at {}: build it with "
+                          "`python -m mnpbem_amd._build` (hipcc, gfx950)".format(LIB_PATH))
+    lib = C.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = I
+    return lib
+
+
+lib = _load()
+
+
+def call(name, *args):
+    """Call ``name`` and raise on a non-zero status."""
+    st = getattr(lib, name)(*args)
+    if st != 0:
+        if st < 0:
+            raise ValueError("{}: argument {} rejected".format(name, -st))
+        raise RuntimeError("{}: HIP launch failure (status {})".format(name, st))
+    return st
+
+
+def exported_symbols():
+    return list(_SIGS)

@@ -1,0 +1,396 @@
+// C-ABI of libmnpbem_hip.so: plain pointers and sizes, no framework types.
+//
+// Conventions (declared in include/mnpbem_hip.h):
+//   * every array argument is a device pointer unless stated otherwise;
+//   * complex values are interleaved (re, im); matrices are column-major with leading dimension
+//     `ld`, one matrix per wavelength at `stride` elements;
+//   * `prec` selects the arithmetic of matrices: 0 = complex double, 1 = complex float;
+//   * work is enqueued on `stream` (a hipStream_t, 0 = default) and returns immediately;
+//   * return value 0 = ok, -k = argument k invalid, 1000 + hipError_t = launch failure.
+#include "common.hpp"
+#include "args.hpp"
+
+namespace mnp {
+template <typename OT>
+void launch_assemble(const AsmArgs&, int, const int*, const int*, const double*, const double*, void*, void*, void*,
+                     void*, hipStream_t);
+template <typename T>
+void gemm_sub(int, int, int, const void*, long, long, const void*, long, long, void*, long, long, int, hipStream_t);
+template <typename T>
+void getrf_batched(int, int, void*, long, long, int*, long, int*, int, hipStream_t);
+void perm_batched(int, int, const int*, long, int*, int*, long, hipStream_t);
+template <typename T>
+void rsolve_batched(int, int, int, const void*, long, long, const int*, long, void*, long, long, void*, long, long, int,
+                    hipStream_t);
+template <typename T>
+void getrs_batched(int, int, int, const void*, long, long, const int*, long, const void*, long, long, void*, long,
+                   long, hipStream_t);
+template <typename T>
+void gemv_batched(int, int, int, const void*, long, long, const void*, long, long, void*, long, long, hipStream_t);
+template <typename T>
+void sub_batched(int, int, const void*, const void*, void*, long, long, hipStream_t);
+template <typename T>
+void sigma_batched(int, int, const void*, void*, const void*, long, long, const double*, const double*, const double*,
+                   const double*, hipStream_t);
+template <typename T>
+void identity_batched(int, int, void*, long, long, hipStream_t);
+template <typename T>
+void scale_diag_batched(int, int, double, const void*, long, const double*, void*, long, long, hipStream_t);
+template <typename T>
+void solve_chain(const Face&, int, const void*, const void*, const void*, const void*, const void*, long, long,
+                 const int*, const int*, const int*, long, const void*, long, void*, long, void*, long, hipStream_t);
+void bem_operator(const Face&, int, const void*, const void*, const void*, const void*, long, long, const void*, long,
+                  const void*, long, void*, long, void*, long, double*, hipStream_t);
+void planewave_rhs(const PW&, int, void*, long, hipStream_t);
+void farfield(const PW&, const double*, int, const double*, int, const void*, long, void*, hipStream_t);
+template <typename T>
+void cast_batched(long, int, const void*, long, void*, long, hipStream_t);
+template <typename T>
+void accum_batched(long, int, const void*, long, void*, long, const int*, hipStream_t);
+void probe_layout(double*, hipStream_t);
+void prof_enable(bool);
+void mdot(long, int, const void*, long, const void*, double*, hipStream_t);
+void mupdate(long, int, const void*, long, const double*, void*, hipStream_t);
+void prof_collect(double*, double*, double*);
+}  // namespace mnp
+
+using namespace mnp;
+
+static int launch_status() {
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : 1000 + (int)e;
+}
+#define REQUIRE(cond, k) \
+    do {                 \
+        if (!(cond)) return -(k); \
+    } while (0)
+
+extern "C" {
+
+int mnp_abi_version(void) { return 1; }
+
+int mnp_probe_mfma_layout(double* out, void* stream) {
+    REQUIRE(out, 1);
+    probe_layout(out, (hipStream_t)stream);
+    return launch_status();
+}
+
+// GEMM timing for roofline reports: while enabled, every matrix-core GEMM launch is
+// bracketed by HIP events on its stream. Not thread safe; meant for benchmarks.
+int mnp_prof_enable(int on) {
+    prof_enable(on != 0);
+    return 0;
+}
+
+// out[0] = summed GEMM kernel time (ms), out[1] = algorithmic flops (8 m n k per complex
+// GEMM), out[2] = launches, since the previous collect. Host array; waits for the events.
+int mnp_prof_collect(double* out) {
+    REQUIRE(out, 1);
+    prof_collect(out, out + 1, out + 2);
+    return 0;
+}
+
+int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const double* pos, const double* nvec,
+                       const double* area, const int* fpart, const int* con_in, const int* con_out,
+                       const double* kmat, int nref, const int* ref_row, const int* ref_col, const double* ref_g,
+                       const double* ref_f, int prec, void* G1, void* G2, void* H1, void* H2, long ld, long stride,
+                       void* stream) {
+    REQUIRE(n > 0, 1);
+    REQUIRE(npart > 0, 2);
+    REQUIRE(nmat > 0, 3);
+    REQUIRE(nlam > 0, 4);
+    REQUIRE(order >= 0 && order <= 7, 5);
+    REQUIRE(pos && nvec && area && fpart && con_in && con_out && kmat, 6);
+    REQUIRE(nref >= 0, 13);
+    REQUIRE(nref == 0 || (ref_row && ref_col && ref_g && ref_f), 14);
+    REQUIRE(prec == 0 || prec == 1, 18);
+    REQUIRE(G1 && G2 && H1 && H2, 19);
+    REQUIRE(ld >= n, 23);
+    REQUIRE(stride >= ld * (long)n || nlam == 1, 24);
+    AsmArgs a{n, npart, nmat, nlam, order, pos, nvec, area, fpart, con_in, con_out, kmat, ld, stride};
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 0)
+        launch_assemble<double>(a, nref, ref_row, ref_col, ref_g, ref_f, G1, G2, H1, H2, s);
+    else
+        launch_assemble<float>(a, nref, ref_row, ref_col, ref_g, ref_f, G1, G2, H1, H2, s);
+    return launch_status();
+}
+
+int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb,
+                         long sB, void* C, long ldc, long sC, int batch, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(m >= 0 && n >= 0 && k >= 0, 2);
+    REQUIRE(A && B && C, 5);
+    REQUIRE(lda >= (m > 0 ? m : 1) && ldb >= (k > 0 ? k : 1) && ldc >= (m > 0 ? m : 1), 6);
+    REQUIRE(batch >= 1, 14);
+    if (prec == 0)
+        gemm_sub<double>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, (hipStream_t)stream);
+    else
+        gemm_sub<float>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
+                      void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0, 2);
+    REQUIRE(batch >= 1, 3);
+    REQUIRE(A, 4);
+    REQUIRE(lda >= n, 5);
+    REQUIRE(ipiv && info, 7);
+    REQUIRE(sP >= n, 8);
+    REQUIRE(nb >= 1 && nb <= 64, 10);
+    hipStream_t s = (hipStream_t)stream;
+    hipMemsetAsync(info, 0, sizeof(int) * batch, s);
+    if (prec == 0)
+        getrf_batched<double>(n, batch, A, lda, sA, ipiv, sP, info, nb, s);
+    else
+        getrf_batched<float>(n, batch, A, lda, sA, ipiv, sP, info, nb, s);
+    return launch_status();
+}
+
+int mnp_perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, void* stream) {
+    REQUIRE(n > 0 && n <= 16384, 1);
+    REQUIRE(batch >= 1, 2);
+    REQUIRE(ipiv && pf && qinv, 3);
+    perm_batched(n, batch, ipiv, sP, pf, qinv, sQ, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_getrs_batched(int prec, int n, int r, int batch, const void* LU, long lda, long sA, const int* pf, long sQ,
+                      const void* B, long ldb, long sB, void* X, long ldx, long sX, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0, 2);
+    REQUIRE(r >= 1 && r <= 8, 3);
+    REQUIRE(batch >= 1, 4);
+    REQUIRE(LU && pf && B && X, 5);
+    REQUIRE(B != X, 11);
+    if (prec == 0)
+        getrs_batched<double>(n, r, batch, LU, lda, sA, pf, sQ, B, ldb, sB, X, ldx, sX, (hipStream_t)stream);
+    else
+        getrs_batched<float>(n, r, batch, LU, lda, sA, pf, sQ, B, ldb, sB, X, ldx, sX, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_rsolve_batched(int prec, int m, int n, int batch, const void* LU, long lda, long sA, const int* qinv, long sQ,
+                       void* B, long ldb, long sB, void* X, long ldx, long sX, int nb, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(m > 0, 2);
+    REQUIRE(n > 0, 3);
+    REQUIRE(batch >= 1, 4);
+    REQUIRE(LU && qinv && B && X, 5);
+    REQUIRE(B != X, 13);
+    REQUIRE(nb >= 1 && nb <= 64, 16);
+    if (prec == 0)
+        rsolve_batched<double>(m, n, batch, LU, lda, sA, qinv, sQ, B, ldb, sB, X, ldx, sX, nb, (hipStream_t)stream);
+    else
+        rsolve_batched<float>(m, n, batch, LU, lda, sA, qinv, sQ, B, ldb, sB, X, ldx, sX, nb, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_gemv_batched(int prec, int n, int r, int batch, const void* A, long lda, long sA, const void* X, long ldx,
+                     long sX, void* Y, long ldy, long sY, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0, 2);
+    REQUIRE(r >= 1 && r <= 8, 3);
+    REQUIRE(batch >= 1, 4);
+    REQUIRE(A && X && Y, 5);
+    if (prec == 0)
+        gemv_batched<double>(n, r, batch, A, lda, sA, X, ldx, sX, Y, ldy, sY, (hipStream_t)stream);
+    else
+        gemv_batched<float>(n, r, batch, A, lda, sA, X, ldx, sX, Y, ldy, sY, (hipStream_t)stream);
+    return launch_status();
+}
+
+// M_b = s A + diag(d_b), b < batch: A one n x n matrix (lda), d batch x n complex double,
+// M batch matrices (ld, stride).
+int mnp_scale_diag_batched(int prec, int n, int batch, double s, const void* A, long lda, const double* d, void* M,
+                           long ld, long stride, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0, 2);
+    REQUIRE(batch >= 1, 3);
+    REQUIRE(A && d && M, 5);
+    REQUIRE(lda >= n && ld >= n, 6);
+    if (prec == 0)
+        scale_diag_batched<double>(n, batch, s, A, lda, d, M, ld, stride, (hipStream_t)stream);
+    else
+        scale_diag_batched<float>(n, batch, s, A, lda, d, M, ld, stride, (hipStream_t)stream);
+    return launch_status();
+}
+
+// Factor the BEMRet system for a batch of wavelengths. Buffers (each `batch` matrices of n x n):
+//   G1, G2 (in: Green matrices; out: LU factors), H1, H2 (in: H matrices; destroyed),
+//   S1 (out: Sigma1), S2 (out: LU of Sigma), D (out: Delta^{-1}).
+// ipiv: 4 x batch x n (G1, G2, Delta, Sigma); perm: 4 x 2 x batch x n (pf, qinv per factor);
+// info: 4 x batch. L1, L2: batch x n complex double; kvac: batch.
+int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, void* H2, void* S1, void* S2, void* D,
+                      long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, const double* L1,
+                      const double* L2, const double* kvac, int nb, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0 && n <= 16384, 2);
+    REQUIRE(batch >= 1, 3);
+    REQUIRE(G1 && G2 && H1 && H2 && S1 && S2 && D, 4);
+    REQUIRE(ld >= n, 11);
+    REQUIRE(stride >= ld * (long)n, 12);
+    REQUIRE(ipiv && perm && info, 13);
+    REQUIRE(nvec && L1 && L2 && kvac, 16);
+    REQUIRE(nb >= 1 && nb <= 64, 20);
+    hipStream_t s = (hipStream_t)stream;
+    const long sP = n, sQ = n;
+    const long PB = (long)batch * n;  // ints per factor
+    int *ip1 = ipiv, *ip2 = ipiv + PB, *ipD = ipiv + 2 * PB, *ipS = ipiv + 3 * PB;
+    int *pf1 = perm, *q1 = perm + PB, *pf2 = perm + 2 * PB, *q2 = perm + 3 * PB;
+    int *pfD = perm + 4 * PB, *qD = perm + 5 * PB, *pfS = perm + 6 * PB, *qS = perm + 7 * PB;
+    hipMemsetAsync(info, 0, sizeof(int) * 4 * batch, s);
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        getrf_batched<T>(n, batch, G1, ld, stride, ip1, sP, info, nb, s);
+        getrf_batched<T>(n, batch, G2, ld, stride, ip2, sP, info + batch, nb, s);
+        perm_batched(n, batch, ip1, sP, pf1, q1, sQ, s);
+        perm_batched(n, batch, ip2, sP, pf2, q2, sQ, s);
+        rsolve_batched<T>(n, n, batch, G1, ld, stride, q1, sQ, H1, ld, stride, S1, ld, stride, nb, s);
+        rsolve_batched<T>(n, n, batch, G2, ld, stride, q2, sQ, H2, ld, stride, S2, ld, stride, nb, s);
+        sub_batched<T>(n, batch, S1, S2, H1, ld, stride, s);  // Delta into H1
+        getrf_batched<T>(n, batch, H1, ld, stride, ipD, sP, info + 2 * batch, nb, s);
+        perm_batched(n, batch, ipD, sP, pfD, qD, sQ, s);
+        identity_batched<T>(n, batch, H2, ld, stride, s);
+        rsolve_batched<T>(n, n, batch, H1, ld, stride, qD, sQ, H2, ld, stride, D, ld, stride, nb, s);
+        sigma_batched<T>(n, batch, S1, S2, D, ld, stride, nvec, L1, L2, kvac, s);
+        getrf_batched<T>(n, batch, S2, ld, stride, ipS, sP, info + 3 * batch, nb, s);
+        perm_batched(n, batch, ipS, sP, pfS, qS, sQ, s);
+    };
+    if (prec == 0)
+        run(double(0));
+    else
+        run(float(0));
+    return launch_status();
+}
+
+// Solve for packed right-hand sides IN (n x 8 per wavelength: phi a De alpha) -> OUT (n x 8:
+// sig1 h1 sig2 h2), with the factors of mnp_bemret_factor. IN/OUT/work are of the factor's
+// precision; work holds 24 n complex values per wavelength.
+int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2, const void* S1, const void* D,
+                     const void* S2, long ld, long stride, const int* perm, const double* nvec, const double* L1,
+                     const double* L2, const double* kvac, const void* in, long sIn, void* out, long sOut, void* work,
+                     long sW, void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0, 2);
+    REQUIRE(batch >= 1, 3);
+    REQUIRE(G1 && G2 && S1 && D && S2, 4);
+    REQUIRE(perm, 11);
+    REQUIRE(nvec && L1 && L2 && kvac, 12);
+    REQUIRE(in && out && work, 16);
+    REQUIRE(sIn >= 8L * n && sOut >= 8L * n, 17);
+    REQUIRE(sW >= 24L * n, 21);
+    const long PB = (long)batch * n;
+    Face f{n, nvec, L1, L2, kvac};
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == 0)
+        solve_chain<double>(f, batch, G1, G2, S1, D, S2, ld, stride, perm, perm + 2 * PB, perm + 6 * PB, n, in, sIn,
+                            out, sOut, work, sW, s);
+    else
+        solve_chain<float>(f, batch, G1, G2, S1, D, S2, ld, stride, perm, perm + 2 * PB, perm + 6 * PB, n, in, sIn,
+                           out, sOut, work, sW, s);
+    return launch_status();
+}
+
+// y = A x (rhs == NULL) or y = rhs - A x, with A the exact complex-double BEM operator built
+// from G1, G2, H1, H2. x, rhs, y: n x 8 per wavelength; work: 32 n per wavelength. When
+// `norms` is given, norms[2b] = |y_b|^2 and norms[2b+1] = |rhs_b|^2 (device, double).
+int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const void* H1, const void* H2, long ld,
+                     long stride, const double* nvec, const double* L1, const double* L2, const double* kvac,
+                     const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work, long sW,
+                     double* norms, void* stream) {
+    REQUIRE(n > 0, 1);
+    REQUIRE(batch >= 1, 2);
+    REQUIRE(G1 && G2 && H1 && H2, 3);
+    REQUIRE(nvec && L1 && L2 && kvac, 9);
+    REQUIRE(x && y && work, 13);
+    REQUIRE(x != y, 17);
+    REQUIRE(sW >= 32L * n, 20);
+    Face f{n, nvec, L1, L2, kvac};
+    bem_operator(f, batch, G1, G2, H1, H2, ld, stride, x, sx, rhs, sr, y, sy, work, sW, norms, (hipStream_t)stream);
+    return launch_status();
+}
+
+// Packed plane-wave right-hand side (n x 8 complex double per wavelength). side[i]: bit 0 =
+// inside medium of face i is the excitation medium, bit 1 = outside medium is. kmed: complex
+// wavenumber in that medium per wavelength; pol, dir: host arrays of 3 doubles.
+int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, const double* kmed,
+                      const double* kvac, const double* L1, const double* L2, const double* pol, const double* dir,
+                      void* out, long so, void* stream) {
+    REQUIRE(n > 0, 1);
+    REQUIRE(batch >= 1, 2);
+    REQUIRE(pos && nvec && side && kmed && kvac && L1 && L2, 3);
+    REQUIRE(pol && dir, 10);
+    REQUIRE(out, 12);
+    REQUIRE(so >= 8L * n, 13);
+    PW p{n, pos, nvec, side, kmed, kvac, L1, L2, {pol[0], pol[1], pol[2]}, {dir[0], dir[1], dir[2]}};
+    planewave_rhs(p, batch, out, so, (hipStream_t)stream);
+    return launch_status();
+}
+
+// Far fields for ndir directions: out[(b * ndir + d) * 6 + c] = e_c (c < 3) and h_{c-3}.
+int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, const double* kmed,
+                 const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out, void* stream) {
+    REQUIRE(n > 0, 1);
+    REQUIRE(batch >= 1 && batch <= 65535, 2);
+    REQUIRE(pos && area && side && kmed && kvac, 3);
+    REQUIRE(ndir >= 1, 8);
+    REQUIRE(dirs && x && out, 9);
+    PW p{n, pos, nullptr, side, kmed, kvac, nullptr, nullptr, {0, 0, 0}, {0, 0, 0}};
+    farfield(p, area, ndir, dirs, batch, x, sx, out, (hipStream_t)stream);
+    return launch_status();
+}
+
+// h[c] = sum_e conj(V_c[e]) w[e] for c < nv (complex double vectors of length len, basis
+// vectors ldv apart); h: device, 2 nv doubles (re, im), overwritten. nv <= 128.
+int mnp_mdot(long len, int nv, const void* V, long ldv, const void* w, double* h, void* stream) {
+    REQUIRE(len > 0, 1);
+    REQUIRE(nv >= 1 && nv <= 128, 2);
+    REQUIRE(V && w && h, 3);
+    REQUIRE(ldv >= len || nv == 1, 4);
+    mdot(len, nv, V, ldv, w, h, (hipStream_t)stream);
+    return launch_status();
+}
+
+// w[e] += sum_c V_c[e] s[c] for c < nv; s: device, 2 nv doubles (complex). nv <= 128.
+int mnp_mupdate(long len, int nv, const void* V, long ldv, const double* s, void* w, void* stream) {
+    REQUIRE(len > 0, 1);
+    REQUIRE(nv >= 1 && nv <= 128, 2);
+    REQUIRE(V && s && w, 3);
+    REQUIRE(ldv >= len || nv == 1, 4);
+    mupdate(len, nv, V, ldv, s, w, (hipStream_t)stream);
+    return launch_status();
+}
+
+// dst = (float or double) src for complex double src (count complex values per wavelength).
+int mnp_cast_batched(int prec_dst, long count, int batch, const void* src, long ss, void* dst, long sd, void* stream) {
+    REQUIRE(prec_dst == 0 || prec_dst == 1, 1);
+    REQUIRE(count >= 0, 2);
+    REQUIRE(batch >= 1, 3);
+    REQUIRE(src && dst, 4);
+    if (count == 0) return 0;
+    if (prec_dst == 1)
+        cast_batched<float>(count, batch, src, ss, dst, sd, (hipStream_t)stream);
+    else
+        cast_batched<double>(count, batch, src, ss, dst, sd, (hipStream_t)stream);
+    return launch_status();
+}
+
+// dst (complex double) += src (prec_src) for wavelengths with active[b] != 0 (active may be NULL).
+int mnp_accum_batched(int prec_src, long count, int batch, const void* src, long ss, void* dst, long sd,
+                      const int* active, void* stream) {
+    REQUIRE(prec_src == 0 || prec_src == 1, 1);
+    REQUIRE(count >= 0, 2);
+    REQUIRE(batch >= 1, 3);
+    REQUIRE(src && dst, 4);
+    if (count == 0) return 0;
+    if (prec_src == 1)
+        accum_batched<float>(count, batch, src, ss, dst, sd, active, (hipStream_t)stream);
+    else
+        accum_batched<double>(count, batch, src, ss, dst, sd, active, (hipStream_t)stream);
+    return launch_status();
+}
+
+}  // extern "C"

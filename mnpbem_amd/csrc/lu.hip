@@ -1,0 +1,432 @@
+// Batched blocked LU with partial pivoting, permutations and triangular solves.
+//
+// Right-looking blocked getrf (column-major, interleaved complex):
+//   for each panel of nb columns starting at j0:
+//     k_panel      : one workgroup per matrix factors the (n-j0) x nb panel (pivot = max |re|+|im|,
+//                    first index on ties, as LAPACK's izamax), swapping rows inside the panel;
+//     k_laswp      : applies the panel's row interchanges to the columns left and right of it;
+//     k_trsm_lln   : U12 = L11^{-1} A12 (unit lower nb x nb block held in LDS);
+//     gemm_sub     : A22 -= L21 * U12 on the matrix cores.
+// Pivots are 0-based global row indices. `info` gets 1 + the first column whose pivot is zero.
+//
+// After the factorisation k_perm turns the interchange list into a row permutation pf
+// (P b = b[pf]) and its inverse, so the solves gather instead of swapping sequentially.
+#include "common.hpp"
+
+namespace mnp {
+
+template <typename T>
+void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb, long sB, void* C,
+              long ldc, long sC, int batch, hipStream_t s);
+
+constexpr int PANEL_THREADS = 512;
+
+template <typename T>
+__global__ __launch_bounds__(PANEL_THREADS) void k_panel(int n, int j0, int nb, cx<T>* A, long lda, long sA,
+                                                         int* ipiv, long sP, int* info) {
+    __shared__ T red_v[PANEL_THREADS / 64];
+    __shared__ int red_i[PANEL_THREADS / 64];
+    __shared__ cx<T> urow[64];
+    __shared__ int piv_s;
+    const int b = blockIdx.x, t = threadIdx.x;
+    A += (long)b * sA;
+    ipiv += (long)b * sP;
+    for (int c = 0; c < nb; ++c) {
+        const int col = j0 + c;
+        // pivot search in column col, rows col..n-1
+        T best = -1;
+        int bi = n;
+        for (int r = col + t; r < n; r += PANEL_THREADS) {
+            T v = cabs1(A[r + (long)col * lda]);
+            if (v > best) { best = v; bi = r; }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            T ov = __shfl_down(best, off, 64);
+            int oi = __shfl_down(bi, off, 64);
+            if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        if ((t & 63) == 0) { red_v[t >> 6] = best; red_i[t >> 6] = bi; }
+        __syncthreads();
+        if (t == 0) {
+            T bv = red_v[0];
+            int bx = red_i[0];
+            for (int w = 1; w < PANEL_THREADS / 64; ++w)
+                if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bx)) { bv = red_v[w]; bx = red_i[w]; }
+            if (bx >= n) bx = col;
+            piv_s = bx;
+            ipiv[col] = bx;
+        }
+        __syncthreads();
+        const int p = piv_s;
+        // swap rows col and p within the panel, and stage the pivot row
+        if (t < nb) {
+            cx<T>* x = A + col + (long)(j0 + t) * lda;
+            cx<T>* y = A + p + (long)(j0 + t) * lda;
+            cx<T> u = *y;
+            if (p != col) { *y = *x; *x = u; }
+            urow[t] = u;
+        }
+        __syncthreads();
+        cx<T> piv = urow[c];
+        if (piv.re == 0 && piv.im == 0) {
+            if (t == 0 && info[b] == 0) info[b] = col + 1;
+            __syncthreads();
+            continue;
+        }
+        cx<T> inv = crecip(piv);
+        for (int r = col + 1 + t; r < n; r += PANEL_THREADS) {
+            cx<T>* Ar = A + r + (long)j0 * lda;
+            cx<T> l = Ar[(long)c * lda] * inv;
+            Ar[(long)c * lda] = l;
+            // trailing columns in chunks of 8: the chunk's loads are issued before its updates
+            for (int c0 = c + 1; c0 < nb; c0 += 8) {
+                cx<T> v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (c0 + u < nb) v[u] = Ar[(long)(c0 + u) * lda];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (c0 + u < nb) Ar[(long)(c0 + u) * lda] = fms(v[u], l, urow[c0 + u]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// apply interchanges ipiv[j0 .. j0+nb) to every column outside [j0, j0+nb)
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp(int n, int j0, int nb, cx<T>* A, long lda, long sA,
+                                               const int* ipiv, long sP) {
+    const int b = blockIdx.y;
+    int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n - nb) return;
+    if (c >= j0) c += nb;
+    A += (long)b * sA + (long)c * lda;
+    ipiv += (long)b * sP;
+    for (int r = j0; r < j0 + nb; ++r) {
+        int p = ipiv[r];
+        if (p != r) { cx<T> x = A[r]; A[r] = A[p]; A[p] = x; }
+    }
+}
+
+// X = L^{-1} X for the unit-lower nb x nb block at (j0, j0), X = rows j0.. j0+nb of columns
+// j0+nb .. n-1; one lane per column.
+template <typename T, int NB>
+__global__ __launch_bounds__(128) void k_trsm_lln(int n, int j0, int nb, cx<T>* A, long lda, long sA) {
+    __shared__ cx<T> L[NB][NB + 1];
+    const int b = blockIdx.y;
+    A += (long)b * sA;
+    for (int e = threadIdx.x; e < nb * nb; e += 128) {
+        int r = e % nb, c = e / nb;
+        L[r][c] = A[(j0 + r) + (long)(j0 + c) * lda];
+    }
+    __syncthreads();
+    int col = j0 + nb + blockIdx.x * 128 + threadIdx.x;
+    if (col >= n) return;
+    cx<T>* x = A + j0 + (long)col * lda;
+    cx<T> v[NB];
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+        if (r < nb) v[r] = x[r];
+#pragma unroll
+    for (int r = 1; r < NB; ++r) {
+        if (r < nb) {
+            cx<T> s = v[r];
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (q < r) s = fms(s, L[r][q], v[q]);
+            v[r] = s;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NB; ++r)
+        if (r < nb) x[r] = v[r];
+}
+
+// pf: P b = b[pf] for the interchange list; qinv = pf^{-1}. One workgroup per matrix; the
+// sequential interchange pass runs in LDS on one lane.
+__global__ __launch_bounds__(64) void k_perm(int n, const int* ipiv, long sP, int* pf, int* qinv, long sQ) {
+    extern __shared__ int sp[];
+    const int b = blockIdx.x, t = threadIdx.x;
+    ipiv += (long)b * sP;
+    pf += (long)b * sQ;
+    qinv += (long)b * sQ;
+    for (int i = t; i < n; i += 64) sp[i] = i;
+    __syncthreads();
+    if (t == 0) {
+        for (int i = 0; i < n; ++i) {
+            int p = ipiv[i];
+            int x = sp[i];
+            sp[i] = sp[p];
+            sp[p] = x;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += 64) {
+        int v = sp[i];
+        pf[i] = v;
+        qinv[v] = i;
+    }
+}
+
+template <typename T>
+void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
+                   hipStream_t s) {
+    cx<T>* a = (cx<T>*)A;
+    for (int j0 = 0; j0 < n; j0 += nb) {
+        int w = (n - j0 < nb) ? (n - j0) : nb;
+        hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv, sP, info);
+        if (n - w > 0)
+            hipLaunchKernelGGL(k_laswp<T>, dim3((n - w + 255) / 256, batch), dim3(256), 0, s, n, j0, w, a, lda, sA,
+                               ipiv, sP);
+        int rest = n - j0 - w;
+        if (rest > 0) {
+            if (nb <= 32)
+                hipLaunchKernelGGL((k_trsm_lln<T, 32>), dim3((rest + 127) / 128, batch), dim3(128), 0, s, n, j0, w, a,
+                                   lda, sA);
+            else
+                hipLaunchKernelGGL((k_trsm_lln<T, 64>), dim3((rest + 127) / 128, batch), dim3(128), 0, s, n, j0, w, a,
+                                   lda, sA);
+            gemm_sub<T>(rest, rest, w, a + (j0 + w) + (long)j0 * lda, lda, sA, a + j0 + (long)(j0 + w) * lda, lda, sA,
+                        a + (j0 + w) + (long)(j0 + w) * lda, lda, sA, batch, s);
+        }
+    }
+}
+
+// n <= 16384 (64 KiB of LDS holds the permutation)
+void perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, hipStream_t s) {
+    hipLaunchKernelGGL(k_perm, dim3(batch), dim3(64), (size_t)n * sizeof(int), s, n, ipiv, sP, pf, qinv, sQ);
+}
+
+template void getrf_batched<double>(int, int, void*, long, long, int*, long, int*, int, hipStream_t);
+template void getrf_batched<float>(int, int, void*, long, long, int*, long, int*, int, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
+// Right-side solve X = B A^{-1} with A = P^T L U, i.e. X = B U^{-1} L^{-1} P.
+//   phase 1 (left to right):  B_j <- B_j U_jj^{-1};  B_{>j} -= B_j U_{j,>j}
+//   phase 2 (right to left):  B_j <- B_j L_jj^{-1};  B_{<j} -= B_j L_{j,<j}
+//   phase 3: X[:, c] = B[:, qinv[c]]  (out of place)
+// One lane per row of B for the diagonal blocks.
+template <typename T, int NB>
+__global__ __launch_bounds__(128) void k_rtrsm_upper(int m, int j0, int w, const cx<T>* A, long lda, long sA,
+                                                     cx<T>* B, long ldb, long sB) {
+    __shared__ cx<T> U[NB][NB + 1];
+    const int b = blockIdx.y;
+    A += (long)b * sA;
+    B += (long)b * sB;
+    for (int e = threadIdx.x; e < w * w; e += 128) {
+        int r = e % w, c = e / w;
+        U[r][c] = A[(j0 + r) + (long)(j0 + c) * lda];
+    }
+    __syncthreads();
+    int i = blockIdx.x * 128 + threadIdx.x;
+    if (i >= m) return;
+    cx<T> v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < w) v[c] = B[i + (long)(j0 + c) * ldb];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c < w) {
+            cx<T> s = v[c];
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (q < c) s = fms(s, v[q], U[q][c]);
+            v[c] = cdiv(s, U[c][c]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < w) B[i + (long)(j0 + c) * ldb] = v[c];
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(128) void k_rtrsm_lower_unit(int m, int j0, int w, const cx<T>* A, long lda, long sA,
+                                                          cx<T>* B, long ldb, long sB) {
+    __shared__ cx<T> L[NB][NB + 1];
+    const int b = blockIdx.y;
+    A += (long)b * sA;
+    B += (long)b * sB;
+    for (int e = threadIdx.x; e < w * w; e += 128) {
+        int r = e % w, c = e / w;
+        L[r][c] = A[(j0 + r) + (long)(j0 + c) * lda];
+    }
+    __syncthreads();
+    int i = blockIdx.x * 128 + threadIdx.x;
+    if (i >= m) return;
+    cx<T> v[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < w) v[c] = B[i + (long)(j0 + c) * ldb];
+#pragma unroll
+    for (int c = NB - 1; c >= 0; --c) {
+        if (c < w) {
+            cx<T> s = v[c];
+#pragma unroll
+            for (int q = 0; q < NB; ++q)
+                if (q > c && q < w) s = fms(s, v[q], L[q][c]);
+            v[c] = s;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < w) B[i + (long)(j0 + c) * ldb] = v[c];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_cols(int m, int n, const cx<T>* B, long ldb, long sB, cx<T>* X,
+                                                     long ldx, long sX, const int* q, long sQ) {
+    const int b = blockIdx.z;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= m) return;
+    int src = q[(long)b * sQ + c];
+    X[(long)b * sX + i + (long)c * ldx] = B[(long)b * sB + i + (long)src * ldb];
+}
+
+template <typename T>
+void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, const int* qinv, long sQ, void* B,
+                    long ldb, long sB, void* X, long ldx, long sX, int nb, hipStream_t s) {
+    const cx<T>* a = (const cx<T>*)LU;
+    cx<T>* bm = (cx<T>*)B;
+    dim3 g((m + 127) / 128, batch);
+    for (int j0 = 0; j0 < n; j0 += nb) {
+        int w = (n - j0 < nb) ? (n - j0) : nb;
+        if (nb <= 32)
+            hipLaunchKernelGGL((k_rtrsm_upper<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
+        else
+            hipLaunchKernelGGL((k_rtrsm_upper<T, 64>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
+        int rest = n - j0 - w;
+        if (rest > 0)
+            gemm_sub<T>(m, rest, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)(j0 + w) * lda, lda, sA,
+                        bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
+    }
+    int last = ((n - 1) / nb) * nb;
+    for (int j0 = last; j0 >= 0; j0 -= nb) {
+        int w = (n - j0 < nb) ? (n - j0) : nb;
+        if (nb <= 32)
+            hipLaunchKernelGGL((k_rtrsm_lower_unit<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
+        else
+            hipLaunchKernelGGL((k_rtrsm_lower_unit<T, 64>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
+        if (j0 > 0)
+            gemm_sub<T>(m, j0, w, bm + (long)j0 * ldb, ldb, sB, a + j0, lda, sA, bm, ldb, sB, batch, s);
+    }
+    hipLaunchKernelGGL(k_gather_cols<T>, dim3((m + 255) / 256, n, batch), dim3(256), 0, s, m, n, bm, ldb, sB,
+                       (cx<T>*)X, ldx, sX, qinv, sQ);
+}
+
+template void rsolve_batched<double>(int, int, int, const void*, long, long, const int*, long, void*, long, long,
+                                     void*, long, long, int, hipStream_t);
+template void rsolve_batched<float>(int, int, int, const void*, long, long, const int*, long, void*, long, long,
+                                    void*, long, long, int, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
+// Left solve with a few right-hand sides: X = A^{-1} B = U^{-1} L^{-1} (B[pf]).
+// One workgroup (512 lanes) per matrix. Rows are processed in 64-row blocks: the diagonal block
+// is staged in LDS by all lanes, wave 0 solves it with lane shuffles (lane = row), then every
+// lane updates the rows below (above) with the block's solution held in LDS; the update loop
+// issues its 64 column loads independently of the accumulation.
+constexpr int TS_MAXR = 8;
+constexpr int TS_THREADS = 512;
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(TS_THREADS) void k_trsv(int n, int r, const cx<T>* A, long lda, long sA, cx<T>* X,
+                                                     long ldx, long sX) {
+    __shared__ cx<T> D[64][65];
+    __shared__ cx<T> xs[64][TS_MAXR];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
+    A += (long)b * sA;
+    X += (long)b * sX;
+    const int nblk = (n + 63) / 64;
+    for (int s = 0; s < nblk; ++s) {
+        const int blk = LOWER ? s : nblk - 1 - s;
+        const int i0 = blk * 64;
+        const int bs = (n - i0 < 64) ? (n - i0) : 64;
+        for (int e = t; e < 64 * 64; e += TS_THREADS) {
+            int rr = e & 63, cc = e >> 6;
+            D[rr][cc] = (rr < bs && cc < bs) ? A[(i0 + rr) + (long)(i0 + cc) * lda] : mk<T>(0, 0);
+        }
+        __syncthreads();
+        if (t < 64) {
+            for (int q = 0; q < r; ++q) {
+                cx<T> v = lane < bs ? X[i0 + lane + (long)q * ldx] : mk<T>(0, 0);
+                if (LOWER) {
+                    for (int c = 0; c < bs; ++c) {
+                        T xr = __shfl(v.re, c, 64), xi = __shfl(v.im, c, 64);
+                        if (lane > c) v = fms(v, D[lane][c], mk(xr, xi));
+                    }
+                } else {
+                    for (int c = bs - 1; c >= 0; --c) {
+                        if (lane == c) v = cdiv(v, D[c][c]);
+                        T xr = __shfl(v.re, c, 64), xi = __shfl(v.im, c, 64);
+                        if (lane < c) v = fms(v, D[lane][c], mk(xr, xi));
+                    }
+                }
+                if (lane < bs) {
+                    xs[lane][q] = v;
+                    X[i0 + lane + (long)q * ldx] = v;
+                }
+            }
+        }
+        __syncthreads();
+        const int lo = LOWER ? i0 + bs : 0;
+        const int hi = LOWER ? n : i0;
+        for (int i = lo + t; i < hi; i += TS_THREADS) {
+            cx<T> acc[TS_MAXR];
+#pragma unroll
+            for (int q = 0; q < TS_MAXR; ++q)
+                if (q < r) acc[q] = X[i + (long)q * ldx];
+            const cx<T>* Ai = A + i + (long)i0 * lda;
+            int c = 0;
+            for (; c + 8 <= bs; c += 8) {
+                cx<T> a8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a8[u] = Ai[(long)(c + u) * lda];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int q = 0; q < TS_MAXR; ++q)
+                        if (q < r) acc[q] = fms(acc[q], a8[u], xs[c + u][q]);
+            }
+            for (; c < bs; ++c) {
+                cx<T> a = Ai[(long)c * lda];
+#pragma unroll
+                for (int q = 0; q < TS_MAXR; ++q)
+                    if (q < r) acc[q] = fms(acc[q], a, xs[c][q]);
+            }
+#pragma unroll
+            for (int q = 0; q < TS_MAXR; ++q)
+                if (q < r) X[i + (long)q * ldx] = acc[q];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_rows(int n, int r, const cx<T>* B, long ldb, long sB, cx<T>* X,
+                                                     long ldx, long sX, const int* pf, long sQ) {
+    const int b = blockIdx.y;
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    int src = pf[(long)b * sQ + i];
+    for (int q = 0; q < r; ++q) X[(long)b * sX + i + (long)q * ldx] = B[(long)b * sB + src + (long)q * ldb];
+}
+
+// X = A^{-1} B for r <= 8 columns; B and X must not alias.
+template <typename T>
+void getrs_batched(int n, int r, int batch, const void* LU, long lda, long sA, const int* pf, long sQ, const void* B,
+                   long ldb, long sB, void* X, long ldx, long sX, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, batch), dim3(256), 0, s, n, r, (const cx<T>*)B, ldb,
+                       sB, (cx<T>*)X, ldx, sX, pf, sQ);
+    hipLaunchKernelGGL((k_trsv<T, true>), dim3(batch), dim3(TS_THREADS), 0, s, n, r, (const cx<T>*)LU, lda, sA,
+                       (cx<T>*)X, ldx, sX);
+    hipLaunchKernelGGL((k_trsv<T, false>), dim3(batch), dim3(TS_THREADS), 0, s, n, r, (const cx<T>*)LU, lda, sA,
+                       (cx<T>*)X, ldx, sX);
+}
+
+template void getrs_batched<double>(int, int, int, const void*, long, long, const int*, long, const void*, long,
+                                    long, void*, long, long, hipStream_t);
+template void getrs_batched<float>(int, int, int, const void*, long, long, const int*, long, const void*, long,
+                                   long, void*, long, long, hipStream_t);
+
+}  // namespace mnp

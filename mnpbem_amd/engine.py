@@ -1,0 +1,262 @@
+"""Device-resident retarded BEM pipeline for batches of wavelengths.
+
+Buffers are torch tensors on the GPU (torch provides memory and the stream); all arithmetic
+is done by libmnpbem_hip.so. A matrix of the batch is stored column-major: a tensor of shape
+(B, n, ld) holds matrix b with element (i, j) at [b, j, i]. Packed vectors of eight columns
+(phi a De alpha, or sig1 h1 sig2 h2) are tensors of shape (B, 8, n).
+
+Per wavelength batch:
+  assemble G1 G2 H1 H2 -> factor (LU chain) -> plane-wave right-hand side -> solve
+  (+ FP64 iterative refinement in mixed precision) -> far field -> cross sections.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call
+
+C128 = torch.complex128
+C64 = torch.complex64
+
+
+def _p(t):
+    return t.data_ptr() if t is not None else None
+
+
+def stream_handle():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("mnpbem_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def connectivity(inout):
+    """Region connectivity of the composite particle (greenfun/compgreen_ret.py:435-505):
+    con[r1][r2][a, b] = material joining region r1 of particle a and region r2 of particle b."""
+    io = np.asarray(inout)
+    con = [[None, None], [None, None]]
+    for r1 in range(2):
+        for r2 in range(2):
+            a = io[:, r1][:, None]
+            b = io[:, r2][None, :]
+            con[r1][r2] = np.where(a == b, a, 0).astype(np.int32)
+    return con
+
+
+class DeviceGeometry:
+    """Per-face geometry, refinement data and media of a ComParticle, uploaded once."""
+
+    def __init__(self, cp, nearfield, dev=None):
+        self.dev = dev or device()
+        self.cp = cp
+        self.n = cp.nfaces
+        d = self.dev
+        f64 = torch.float64
+        self.pos = torch.tensor(np.ascontiguousarray(cp.pos), dtype=f64, device=d)
+        self.nvec = torch.tensor(np.ascontiguousarray(cp.nvec), dtype=f64, device=d)
+        self.area = torch.tensor(np.ascontiguousarray(cp.area), dtype=f64, device=d)
+        self.fpart = torch.tensor(cp.face_particle.astype(np.int32), device=d)
+        con = connectivity(cp.inout)
+        if np.any(con[1][0]) or np.any(con[0][1]):
+            raise NotImplementedError("particles whose inside and outside media connect (nested or touching "
+                                      "regions) are not supported by the device BEM path")
+        self.npart = cp.inout.shape[0]
+        self.con_in = torch.tensor(con[0][0].ravel(), device=d)
+        self.con_out = torch.tensor(con[1][1].ravel(), device=d)
+        self.nmat = len(cp.eps)
+        nf = nearfield
+        if nf is not None:   # None: far-field use only (no matrix assembly)
+            self.order = nf.order
+            self.nref = nf.n_ref
+            self.ref_row = torch.tensor(nf.rows.astype(np.int32), device=d)
+            self.ref_col = torch.tensor(nf.cols.astype(np.int32), device=d)
+            self.ref_g = torch.tensor(np.ascontiguousarray(nf.g), dtype=f64, device=d)
+            self.ref_f = torch.tensor(np.ascontiguousarray(nf.f), dtype=f64, device=d)
+        self._side = {}
+
+    def side(self, medium):
+        """Per face: bit 0 = inside medium is ``medium``, bit 1 = outside medium is."""
+        if medium not in self._side:
+            io = self.cp.inout_faces
+            s = (io[:, 0] == medium).astype(np.int32) + 2 * (io[:, 1] == medium).astype(np.int32)
+            self._side[medium] = torch.tensor(s, device=self.dev)
+        return self._side[medium]
+
+    def media(self, enei_list):
+        """k per (wavelength, material), eps inside/outside per (wavelength, face), k vacuum."""
+        enei_list = np.atleast_1d(np.asarray(enei_list, float))
+        ek = [f(enei_list) for f in self.cp.eps]
+        kmat = np.stack([np.broadcast_to(np.asarray(e[1], complex), enei_list.shape) for e in ek], axis=1)
+        epsm = np.stack([np.broadcast_to(np.asarray(e[0], complex), enei_list.shape) for e in ek], axis=1)
+        io = self.cp.inout_faces
+        L1 = epsm[:, io[:, 0] - 1]
+        L2 = epsm[:, io[:, 1] - 1]
+        d = self.dev
+        return (torch.tensor(np.ascontiguousarray(kmat), dtype=C128, device=d),
+                torch.tensor(np.ascontiguousarray(L1), dtype=C128, device=d),
+                torch.tensor(np.ascontiguousarray(L2), dtype=C128, device=d),
+                torch.tensor(2 * np.pi / enei_list, dtype=torch.float64, device=d),
+                kmat, epsm)
+
+
+class GreenBatch:
+    """G1, G2, H1, H2 for a batch of wavelengths in one precision."""
+
+    def __init__(self, geo, batch, dtype=C128):
+        self.geo, self.batch, self.dtype = geo, batch, dtype
+        n = geo.n
+        self.ld = n
+        self.stride = n * self.ld
+        mk = lambda: torch.empty((batch, n, self.ld), dtype=dtype, device=geo.dev)
+        self.G1, self.G2, self.H1, self.H2 = mk(), mk(), mk(), mk()
+
+    @property
+    def prec(self):
+        return 0 if self.dtype == C128 else 1
+
+    def assemble(self, kmat, nlam):
+        g = self.geo
+        call("mnp_green_assemble", g.n, g.npart, g.nmat, nlam, g.order, _p(g.pos), _p(g.nvec), _p(g.area),
+             _p(g.fpart), _p(g.con_in), _p(g.con_out), _p(kmat), g.nref, _p(g.ref_row), _p(g.ref_col),
+             _p(g.ref_g), _p(g.ref_f), self.prec, _p(self.G1), _p(self.G2), _p(self.H1), _p(self.H2),
+             self.ld, self.stride, stream_handle())
+
+    @staticmethod
+    def to_numpy(t, b):
+        """Column-major device matrix b -> (n, n) numpy array."""
+        return t[b].transpose(0, 1).cpu().numpy()
+
+
+class RetSolverBatch:
+    """Dense BEMRet factors for up to ``batch`` wavelengths.
+
+    precision: 'double' (complex128 chain), 'single' (complex64 chain, no refinement) or
+    'mixed' (complex64 chain as approximate inverse, FP64 iterative refinement against the
+    exact complex128 operator; wavelengths whose refinement stalls are redone in 'double').
+    """
+
+    def __init__(self, geo, batch, precision="double", nb=32, refine_tol=1e-12, max_refine=12):
+        if precision not in ("double", "single", "mixed"):
+            raise ValueError("precision must be 'double', 'single' or 'mixed'")
+        self.geo, self.batch, self.precision, self.nb = geo, batch, precision, nb
+        self.refine_tol, self.max_refine = refine_tol, max_refine
+        n, d = geo.n, geo.dev
+        fdt = C128 if precision == "double" else C64
+        self.fac = GreenBatch(geo, batch, fdt)   # G1, G2 -> LU; H1, H2 scratch
+        mk = lambda: torch.empty((batch, n, n), dtype=fdt, device=d)
+        self.S1, self.S2, self.D = mk(), mk(), mk()
+        self.exact = GreenBatch(geo, batch, C128) if precision == "mixed" else None
+        self.ipiv = torch.empty((4, batch, n), dtype=torch.int32, device=d)
+        self.perm = torch.empty((8, batch, n), dtype=torch.int32, device=d)
+        self.info = torch.empty((4, batch), dtype=torch.int32, device=d)
+        self.work = torch.empty((batch, 24, n), dtype=fdt, device=d)
+        self.opwork = torch.empty((batch, 32, n), dtype=C128, device=d) if precision == "mixed" else None
+        self.nlam = 0
+        self.stats = {}
+
+    @property
+    def prec(self):
+        return self.fac.prec
+
+    def factor(self, enei_list):
+        enei_list = np.atleast_1d(np.asarray(enei_list, float))
+        nl = len(enei_list)
+        if nl > self.batch:
+            raise ValueError("batch holds {} wavelengths, got {}".format(self.batch, nl))
+        g = self.geo
+        self.kmat, self.L1, self.L2, self.kvac, self.kmat_np, self.eps_np = g.media(enei_list)
+        self.enei = enei_list
+        self.nlam = nl
+        if self.exact is not None:
+            self.exact.assemble(self.kmat, nl)
+        self.fac.assemble(self.kmat, nl)
+        f = self.fac
+        call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
+             _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info), _p(g.nvec),
+             _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
+        return self
+
+    def check_info(self):
+        """(4, nlam) LU info of G1, G2, Delta, Sigma: 0 = ok, j = zero pivot in column j-1.
+        The factor call lays pivots and info out for the wavelengths it was given."""
+        nl = self.nlam
+        return self.info.view(-1)[: 4 * nl].reshape(4, nl).cpu().numpy()
+
+    def _chain(self, rhs, out):
+        """out = M rhs with the factor precision (rhs, out: (B, 8, n) of that dtype)."""
+        g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
+        call("mnp_bemret_solve", self.prec, n, nl, _p(f.G1), _p(f.G2), _p(self.S1), _p(self.D), _p(self.S2),
+             f.ld, f.stride, _p(self.perm), _p(g.nvec), _p(self.L1), _p(self.L2), _p(self.kvac), _p(rhs), 8 * n,
+             _p(out), 8 * n, _p(self.work), 24 * n, stream_handle())
+
+    def residual(self, x, rhs, out, norms=None):
+        """out = rhs - A x with the exact complex128 operator; norms (B, 2) = |r|^2, |rhs|^2."""
+        e, g, n, nl = self.exact, self.geo, self.geo.n, self.nlam
+        call("mnp_bem_operator", n, nl, _p(e.G1), _p(e.G2), _p(e.H1), _p(e.H2), e.ld, e.stride, _p(g.nvec),
+             _p(self.L1), _p(self.L2), _p(self.kvac), _p(x), 8 * n, _p(rhs), 8 * n, _p(out), 8 * n,
+             _p(self.opwork), 32 * n, _p(norms), stream_handle())
+
+    def solve(self, rhs):
+        """Solve for packed right-hand sides rhs (B, 8, n) complex128 -> (B, 8, n) complex128."""
+        n, nl = self.geo.n, self.nlam
+        if self.precision == "double":
+            out = torch.empty_like(rhs)
+            self._chain(rhs, out)
+            return out
+        r32 = torch.empty((self.batch, 8, n), dtype=C64, device=rhs.device)
+        dx = torch.empty_like(r32)
+        call("mnp_cast_batched", 1, 8 * n, nl, _p(rhs), 8 * n, _p(r32), 8 * n, stream_handle())
+        self._chain(r32, dx)
+        x = torch.zeros_like(rhs)
+        call("mnp_accum_batched", 1, 8 * n, nl, _p(dx), 8 * n, _p(x), 8 * n, None, stream_handle())
+        if self.precision == "single":
+            return x
+        r = torch.empty_like(rhs)
+        norms = torch.zeros((self.batch, 2), dtype=torch.float64, device=rhs.device)
+        active = torch.ones(self.batch, dtype=torch.int32, device=rhs.device)
+        hist = []
+        prev = None
+        stalled = np.zeros(nl, dtype=bool)
+        for it in range(self.max_refine + 1):
+            self.residual(x, rhs, r, norms)
+            nr = norms[:nl].cpu().numpy()
+            rel = np.sqrt(nr[:, 0] / np.maximum(nr[:, 1], 1e-300))
+            hist.append(rel)
+            done = rel <= self.refine_tol
+            if prev is not None:
+                stalled |= (~done) & (rel > 0.5 * prev)
+            if np.all(done | stalled) or it == self.max_refine:
+                break
+            prev = rel
+            act = (~done) & (~stalled)
+            active[:nl] = torch.tensor(act.astype(np.int32), device=rhs.device)
+            call("mnp_cast_batched", 1, 8 * n, nl, _p(r), 8 * n, _p(r32), 8 * n, stream_handle())
+            self._chain(r32, dx)
+            call("mnp_accum_batched", 1, 8 * n, nl, _p(dx), 8 * n, _p(x), 8 * n, _p(active), stream_handle())
+        self.stats = dict(relres=hist[-1], history=np.array(hist), iterations=len(hist) - 1,
+                          fallback=np.nonzero(~(hist[-1] <= self.refine_tol))[0])
+        return x
+
+
+def planewave_rhs(geo, kvac, kmed, L1, L2, pol, direc, medium, nlam, batch):
+    n = geo.n
+    out = torch.empty((batch, 8, n), dtype=C128, device=geo.dev)
+    pol = np.ascontiguousarray(np.asarray(pol, float).ravel())
+    direc = np.ascontiguousarray(np.asarray(direc, float).ravel())
+    call("mnp_planewave_rhs", n, nlam, _p(geo.pos), _p(geo.nvec), _p(geo.side(medium)), _p(kmed), _p(kvac),
+         _p(L1), _p(L2), pol.ctypes.data, direc.ctypes.data, _p(out), 8 * n, stream_handle())
+    return out
+
+
+def farfield(geo, x, kvac, kmed, dirs, medium, nlam):
+    """(nlam, ndir, 6) complex: e (3), h (3) per direction."""
+    n = geo.n
+    dirs_t = torch.tensor(np.ascontiguousarray(np.atleast_2d(dirs), dtype=float), device=geo.dev)
+    ndir = dirs_t.shape[0]
+    out = torch.empty((nlam, ndir, 6), dtype=C128, device=geo.dev)
+    call("mnp_farfield", n, nlam, _p(geo.pos), _p(geo.area), _p(geo.side(medium)), _p(kmed), _p(kvac), ndir,
+         _p(dirs_t), _p(x), 8 * n, _p(out), stream_handle())
+    return out

@@ -1,0 +1,153 @@
+"""Composite Green functions and the field container that crosses the API.
+
+``CompGreenRet(p, p)`` mirrors greenfun/compgreen_ret.py:14-858: wavelength-independent
+set-up (refinement pairs, Taylor coefficients, closed-surface correction) runs once on the
+host; ``eval(i, j, key, enei)`` assembles the requested region-pair matrix on the GPU.
+"""
+import numpy as np
+import torch
+
+from .engine import DeviceGeometry, GreenBatch, connectivity, device
+from .nearfield import NearField
+
+
+class CompStruct:
+    """Attribute container (greenfun/compgreen_stat.py:1154-1420): particle ``p``,
+    wavelength ``enei`` and named fields."""
+
+    def __init__(self, p, enei, **fields):
+        self.p = p
+        self.enei = enei
+        self._fields = {}
+        for k, v in fields.items():
+            self.set_field(k, v)
+
+    def set_field(self, key, val):
+        self._fields[key] = val
+        object.__setattr__(self, key, val)
+
+    def set(self, **fields):
+        for k, v in fields.items():
+            self.set_field(k, v)
+        return self
+
+    def get(self, key, default=None):
+        if key == "enei":
+            return self.enei
+        return self._fields.get(key, default)
+
+    def __getitem__(self, key):
+        if key == "enei":
+            return self.enei
+        return self._fields[key]
+
+    def __contains__(self, key):
+        return key in self._fields
+
+    def keys(self):
+        return self._fields.keys()
+
+    def __repr__(self):
+        return "CompStruct(enei={}, fields={})".format(self.enei, list(self._fields))
+
+
+class CompGreenRet:
+    """Retarded Green function of a composite particle with itself.
+
+    Options (reference defaults): ``RelCutoff=3``, ``AbsCutoff=0``, ``order=5``,
+    ``deriv='cart'``."""
+
+    def __init__(self, p1, p2, **op):
+        if p1 is not p2:
+            raise NotImplementedError("CompGreenRet on the device covers p1 is p2 (the BEM solve path)")
+        if op.get("deriv", "cart") != "cart":
+            raise NotImplementedError("only the Cartesian surface derivative (deriv='cart') is provided")
+        self.p1 = self.p2 = p1
+        self.deriv = "cart"
+        self.nearfield = NearField(p1, rel_cutoff=op.get("RelCutoff", 3.0), abs_cutoff=op.get("AbsCutoff", 0.0),
+                                   order=op.get("order", 5))
+        self.con = [[c for c in row] for row in connectivity(p1.inout)]
+        self._geo = None
+
+    @property
+    def geo(self):
+        if self._geo is None:
+            self._geo = DeviceGeometry(self.p1, self.nearfield, device())
+        return self._geo
+
+    def eval(self, i, j, key, enei):
+        """Region-pair matrix (n, n) complex128 (host numpy), or 0 when the regions do not
+        connect (compgreen_ret.py:507-553). Keys: 'G', 'F', 'H1', 'H2'."""
+        if key not in ("G", "F", "H1", "H2"):
+            raise ValueError("Unknown key: {}".format(key))
+        if not np.any(self.con[i][j]):
+            return 0
+        if i != j:
+            raise NotImplementedError("cross-region Green functions are not provided")
+        gb = GreenBatch(self.geo, 1, torch.complex128)
+        kmat = self.geo.media([enei])[0]
+        gb.assemble(kmat, 1)
+        if key == "G":
+            t = gb.G1 if i == 0 else gb.G2
+            return GreenBatch.to_numpy(t, 0)
+        t = gb.H1 if i == 0 else gb.H2
+        m = GreenBatch.to_numpy(t, 0)
+        if key == "F":
+            # H = F +- 2 pi on the diagonal of same-particle blocks
+            m = m - (2 * np.pi if i == 0 else -2 * np.pi) * np.eye(m.shape[0])
+        elif (key == "H1") != (i == 0):
+            m = m + (4 * np.pi if key == "H1" else -4 * np.pi) * np.eye(m.shape[0])
+        return m
+
+
+class CompGreenStat:
+    """Quasistatic Green function G = a_j / d and surface derivative
+    F = -n_i.(r_i - r_j) a_j / d^3 of a composite particle with itself, with refined self
+    and near entries and the closed-surface correction (greenfun/compgreen_stat.py:70-528).
+
+    They are the k -> 0 limit of the retarded assembly (phase 1, only the n = 0 Taylor
+    coefficients), so the device assembly runs at k = 0 with every particle pair connected.
+    ``G`` and ``F`` are host numpy (n, n); ``device_F()`` keeps F + 2 pi I on the GPU.
+
+    As in the reference, the closed-surface correction is applied only when the particle
+    was given closed arguments (compgreen_stat.py:154-163); ``diag_shift`` is then 0,
+    otherwise it removes the correction that the retarded set-up folds in by default."""
+
+    def __init__(self, p1, p2, **op):
+        if p1 is not p2:
+            raise NotImplementedError("CompGreenStat on the device covers p1 is p2 (the BEM solve path)")
+        self.p1 = self.p2 = p1
+        self.green = op.get("green") or CompGreenRet(p1, p1, **{k: v for k, v in op.items()
+                                                                if k in ("RelCutoff", "AbsCutoff", "order")})
+        self._dev = None
+        corr = self.green.nearfield.closed_correction
+        given = any(c is not None for c in p1.closed)
+        self.diag_shift = np.zeros_like(corr) if given else -corr
+
+    def _assemble(self):
+        if self._dev is None:
+            geo = self.green.geo
+            gb = GreenBatch(geo, 1, torch.complex128)
+            kmat = torch.zeros((1, geo.nmat), dtype=torch.complex128, device=geo.dev)
+            con_in, con_out = geo.con_in, geo.con_out
+            geo.con_in = geo.con_out = torch.full_like(con_in, 1)
+            try:
+                gb.assemble(kmat, 1)
+            finally:
+                geo.con_in, geo.con_out = con_in, con_out
+            self._dev = gb
+        return self._dev
+
+    def device_F(self):
+        """(G1 tensor, H1 tensor): G and F + 2 pi I (column-major, batch of one)."""
+        gb = self._assemble()
+        return gb.G1, gb.H1
+
+    @property
+    def G(self):
+        return GreenBatch.to_numpy(self._assemble().G1, 0).real.copy()
+
+    @property
+    def F(self):
+        m = GreenBatch.to_numpy(self._assemble().H1, 0).real.copy()
+        return m + np.diag(self.diag_shift - 2 * np.pi)

@@ -17,6 +17,7 @@ the reference algorithm of MNPBEM's retarded direct solver:
     closed surfaces       greenfun/compgreen_ret.py:192-400, greenfun/compgreen_stat.py:219-464
     Green / H matrices    greenfun/greenret_refined.py:654-810, compgreen_ret.py:435-505, :739-858
     BEMRet init/solve     bem/bem_ret.py:407-525, :1360-1429, :1708-1752
+    BEMStat               bem/bem_stat.py:241-404, :752-807; simulation/planewave_stat.py:152-355
     plane wave            simulation/planewave_ret.py:257-361, :391-504
     far field             spectrum/spectrum_ret.py:231-459
     8N operator           bem/bem_ret_iter.py:1196-1232, :1362-1501
@@ -480,6 +481,41 @@ def scattering(enei, k, eps_medium, pos, area, sol, ind1, ind2, pinf_nvec, pinf_
     e, h = farfield(pos, area, pinf_nvec, enei, k, *sol, ind1, ind2)
     dsca = 0.5 * np.real(np.sum(pinf_nvec * np.cross(e, np.conj(h)), axis=1))
     return float(np.dot(pinf_area, dsca) / (0.5 * np.real(np.sqrt(complex(eps_medium)))))
+
+
+# ------------------------------------------------------------------ BEMStat
+def stat_G(s, kind):
+    """Refined quasistatic G = a_j / d (compgreen_stat.py:219-464): self and near entries
+    integrate 1/r over the source element."""
+    r = s.pos[:, None, :] - s.pos[None, :, :]
+    d = np.maximum(np.sqrt(np.sum(r * r, axis=2)), EPSF)
+    G = s.area[None, :] / d
+    np.fill_diagonal(G, 0.0)
+    for j in range(s.n):
+        obs = np.nonzero(kind[:, j])[0]
+        if len(obs) == 0:
+            continue
+        qp, qw = s.quad_points(j)
+        pp, pw = s.polar_points(j)
+        for i in obs:
+            pts, w = (pp, pw) if kind[i, j] == 2 else (qp, qw)
+            G[i, j] = np.sum(w / np.maximum(np.linalg.norm(s.pos[i] - pts, axis=1), EPSF))
+    return G
+
+
+def bemstat_solve(F, eps1, eps2, phip):
+    """sigma = -(diag Lambda + F)^{-1} phi' (bem_stat.py:241-404, :752-807)."""
+    lam = 2 * np.pi * (eps1 + eps2) / (eps1 - eps2)
+    M = -(np.diag(lam) + F)
+    return sla.lu_solve(sla.lu_factor(M, check_finite=False), phip, check_finite=False)
+
+
+def stat_cross_sections(pos, area, sig, pol, k):
+    """Dipole absorption / scattering / extinction (planewave_stat.py:225-355)."""
+    dip = (area[:, None] * pos).T @ sig
+    ab = float(np.real(4 * np.pi * k * np.imag(np.dot(pol, dip))))
+    sc = float(np.real(8 * np.pi / 3 * np.real(k) ** 4 * np.sum(np.abs(dip) ** 2)))
+    return sc + ab, sc, ab
 
 
 # ------------------------------------------------------------------ 8N operator

@@ -1,0 +1,157 @@
+"""Kernel-level parity on the GPU: matrix-core GEMM, LU, permutations, solves, GEMV.
+Reference arithmetic: numpy / LAPACK (scipy) on the host."""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from conftest import banner
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _dev():
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    return torch.device("cuda", 0)
+
+
+def _lib():
+    from mnpbem_amd import _lib
+    return _lib
+
+
+def _sh():
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _cm(mats, dtype):
+    """list of (m, n) numpy matrices -> device tensor (B, n, m) holding them column-major."""
+    arr = np.stack([np.asarray(M).T for M in mats])
+    return torch.tensor(np.ascontiguousarray(arr), dtype=dtype, device=_dev())
+
+
+def _np(t, b):
+    return t[b].transpose(0, 1).cpu().numpy()
+
+
+def _rand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def test_mfma_layout_probe():
+    banner("mfma layout probe")
+    out = torch.zeros(512, dtype=torch.float64, device=_dev())
+    _lib().call("mnp_probe_mfma_layout", out.data_ptr(), _sh())
+    got = out.cpu().numpy().reshape(2, 64, 4)
+    l = np.arange(64)[:, None]
+    r = np.arange(4)[None, :]
+    # D[i][j] = (i + 16)(j + 1); register r of lane l holds row l/16 + 4r (f64) and
+    # 4(l/16) + r (f32), column l % 16 -- the mapping the GEMM epilogue is written for
+    assert np.array_equal(got[0], (l // 16 + 4 * r + 16) * ((l % 16) + 1))
+    assert np.array_equal(got[1], (4 * (l // 16) + r + 16) * ((l % 16) + 1))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gemm_sub(prec):
+    banner("gemm_sub prec={}".format(prec))
+    rng = np.random.default_rng(1)
+    m, n, k, B = 150, 97, 45, 3
+    dt = torch.complex128 if prec == 0 else torch.complex64
+    A = [_rand(rng, m, k) for _ in range(B)]
+    Bm = [_rand(rng, k, n) for _ in range(B)]
+    Cm = [_rand(rng, m, n) for _ in range(B)]
+    tA, tB, tC = _cm(A, dt), _cm(Bm, dt), _cm(Cm, dt)
+    _lib().call("mnp_gemm_sub_batched", prec, m, n, k, tA.data_ptr(), m, m * k, tB.data_ptr(), k, k * n,
+                tC.data_ptr(), m, m * n, B, _sh())
+    tol = 1e-13 if prec == 0 else 2e-5
+    for b in range(B):
+        ref = Cm[b] - A[b] @ Bm[b]
+        assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < tol
+
+
+@pytest.mark.parametrize("n,nb", [(200, 32), (97, 32), (64, 16)])
+def test_getrf_matches_lapack(n, nb):
+    banner("getrf n={} nb={}".format(n, nb))
+    rng = np.random.default_rng(n)
+    B = 3
+    A = [_rand(rng, n, n) for _ in range(B)]
+    t = _cm(A, torch.complex128)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    _lib().call("mnp_getrf_batched", 0, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), nb, _sh())
+    assert np.all(info.cpu().numpy() == 0)
+    for b in range(B):
+        lu, piv = sla.lu_factor(A[b])
+        assert np.array_equal(ipiv[b].cpu().numpy(), piv)
+        assert np.max(np.abs(_np(t, b) - lu)) < 1e-11 * np.max(np.abs(lu))
+
+
+def test_getrf_single_backward_error():
+    banner("getrf complex64")
+    rng = np.random.default_rng(5)
+    n, B = 180, 2
+    A = [_rand(rng, n, n) for _ in range(B)]
+    t = _cm(A, torch.complex64)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    _lib().call("mnp_getrf_batched", 1, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    for b in range(B):
+        LU = _np(t, b).astype(complex)
+        L = np.tril(LU, -1) + np.eye(n)
+        U = np.triu(LU)
+        PA = A[b].copy()
+        for i, p in enumerate(ipiv[b].cpu().numpy()):
+            PA[[i, p]] = PA[[p, i]]
+        assert np.linalg.norm(PA - L @ U) / np.linalg.norm(A[b]) < 1e-5
+
+
+def test_singular_matrix_reports_info():
+    banner("getrf singular")
+    n = 40
+    A = np.zeros((n, n), complex)
+    A[:, :10] = np.random.default_rng(2).standard_normal((n, 10))
+    t = _cm([A], torch.complex128)
+    ipiv = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(1, dtype=torch.int32, device=_dev())
+    _lib().call("mnp_getrf_batched", 0, n, 1, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    assert info.cpu().numpy()[0] == 11
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_solves(prec):
+    banner("getrs/rsolve/gemv prec={}".format(prec))
+    rng = np.random.default_rng(7)
+    n, B, r = 133, 2, 5
+    dt = torch.complex128 if prec == 0 else torch.complex64
+    A = [_rand(rng, n, n) + n * np.eye(n) * 0.1 for _ in range(B)]
+    Rhs = [_rand(rng, n, r) for _ in range(B)]
+    Bm = [_rand(rng, n, n) for _ in range(B)]
+    L = _lib()
+    t = _cm(A, dt)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    L.call("mnp_getrf_batched", prec, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    pf = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    q = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    L.call("mnp_perm_batched", n, B, ipiv.data_ptr(), n, pf.data_ptr(), q.data_ptr(), n, _sh())
+    tr = _cm(Rhs, dt)
+    tx = torch.zeros_like(tr)
+    L.call("mnp_getrs_batched", prec, n, r, B, t.data_ptr(), n, n * n, pf.data_ptr(), n, tr.data_ptr(), n, n * r,
+           tx.data_ptr(), n, n * r, _sh())
+    tb = _cm(Bm, dt)
+    tX = torch.zeros_like(tb)
+    L.call("mnp_rsolve_batched", prec, n, n, B, t.data_ptr(), n, n * n, q.data_ptr(), n, tb.data_ptr(), n, n * n,
+           tX.data_ptr(), n, n * n, 32, _sh())
+    tA = _cm(A, dt)
+    ty = torch.zeros_like(tr)
+    L.call("mnp_gemv_batched", prec, n, r, B, tA.data_ptr(), n, n * n, tr.data_ptr(), n, n * r, ty.data_ptr(), n,
+           n * r, _sh())
+    tol = 1e-11 if prec == 0 else 5e-4
+    for b in range(B):
+        x = np.linalg.solve(A[b], Rhs[b])
+        assert np.max(np.abs(_np(tx, b) - x)) / np.max(np.abs(x)) < tol
+        X = Bm[b] @ np.linalg.inv(A[b])
+        assert np.max(np.abs(_np(tX, b) - X)) / np.max(np.abs(X)) < tol
+        y = A[b] @ Rhs[b]
+        assert np.max(np.abs(_np(ty, b) - y)) / np.max(np.abs(y)) < (1e-13 if prec == 0 else 1e-5)

@@ -1,0 +1,102 @@
+"""Host-side product code against the oracle, and the C-ABI library surface. CPU only."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import bem_oracle as O
+from mnpbem_amd.mesh import trisphere, trirod
+from mnpbem_amd.comparticle import ComParticle
+from mnpbem_amd.materials import EpsConst, EpsDrude, EpsTable
+from mnpbem_amd.nearfield import NearField
+from conftest import ROOT
+
+
+def _cp(part, eps=None):
+    eps = eps or [EpsConst(1.0), EpsDrude(10.0, 9.065, 0.0708)]
+    return ComParticle(eps, [part], [[2, 1]], 1, interp="curv")
+
+
+@pytest.mark.parametrize("make", [lambda: trisphere(144, 20), lambda: trirod(10.0, 40.0, [15, 15, 15]),
+                                  lambda: trirod(10.0, 40.0, [12, 6, 10], triangles=True)])
+def test_geometry_and_nearfield_match_oracle(make):
+    cp = _cp(make())
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+    assert np.allclose(p.pos, s.pos, atol=1e-13)
+    assert np.allclose(p.nvec, s.nvec, atol=1e-13)
+    assert np.allclose(p.area, s.area, rtol=1e-13)
+    assert np.allclose(p.bradius(), s.rad, rtol=1e-13)
+    nf = NearField(cp)
+    g = O.Green(s, np.zeros(s.n, int), [[2, 1]], [(np.arange(s.n), np.arange(s.n), 1.0)])
+    keys = sorted(g.coef)
+    assert np.array_equal(nf.rows, [k[0] for k in keys]) and np.array_equal(nf.cols, [k[1] for k in keys])
+    G = np.array([g.coef[k][0] for k in keys])
+    F = np.array([g.coef[k][1] for k in keys])
+    # self terms carry ~1e-10 conditioning noise (points 1e-7 nm from the centroid)
+    assert np.max(np.abs(G - nf.g)) / np.max(np.abs(G)) < 1e-9
+    assert np.max(np.abs(F - nf.f)) / np.max(np.abs(F)) < 1e-9
+
+
+def test_dimer_pairs_cross_particle():
+    a = trisphere(60, 10)
+    b = trisphere(60, 10).shift([0, 0, 11.5])
+    cp = ComParticle([EpsConst(1.0), EpsDrude(10.0, 9.065, 0.0708)], [a, b], [[2, 1], [2, 1]], 1, 2, interp="curv")
+    nf = NearField(cp)
+    fp = cp.face_particle
+    cross = fp[nf.rows] != fp[nf.cols]
+    assert cross.any()                      # gap pairs are refined
+    assert np.all(nf.kind[cross] == 1)      # never as self terms
+
+
+def test_materials():
+    d = EpsDrude(10.0, 9.065, 0.0708)
+    e, k = d(600.0)
+    w = (1 / 8.0655477e-4) / 600.0
+    assert np.isclose(e, 10 - 9.065 ** 2 / (w * (w + 0.0708j)))
+    assert np.isclose(k, 2 * np.pi / 600 * np.sqrt(e))
+    gt = EpsTable("gold.dat")
+    with pytest.raises(ValueError):
+        gt(100.0)
+    tab = np.loadtxt(os.path.join(ROOT, "mnpbem_amd", "data", "gold.dat"), comments="%")
+    lam = np.linspace(450, 900, 7)
+    assert np.allclose(gt(lam)[0], O.eps_table(lam, tab)[0], rtol=1e-14)
+
+
+def test_comparticle_validation():
+    p = trisphere(32, 5)
+    with pytest.raises(ValueError):
+        ComParticle([EpsConst(1.0)], [p], [[2, 1]])
+    with pytest.raises(ValueError):
+        ComParticle([EpsConst(1.0), EpsConst(2.0)], [p], [[0, 1]])
+    with pytest.raises(ValueError):
+        ComParticle(None, [p], [[2, 1]])
+
+
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "mnpbem_hip.h")).read()
+    return sorted(set(re.findall(r"^int (mnp_\w+)\(", txt, re.M)))
+
+
+def test_library_exports_header():
+    lib = ctypes.CDLL(os.path.join(ROOT, "mnpbem_amd", "libmnpbem_hip.so"))
+    syms = _header_symbols()
+    assert len(syms) >= 15
+    for s in syms:
+        assert hasattr(lib, s), s
+    from mnpbem_amd import _lib
+    assert sorted(_lib.exported_symbols()) == syms
+    assert lib.mnp_abi_version() == 1
+
+
+def test_cabi_rejects_bad_arguments_without_gpu_work():
+    from mnpbem_amd import _lib
+    with pytest.raises(ValueError):
+        _lib.call("mnp_getrf_batched", 7, 10, 1, 1, 10, 100, 1, 10, 1, 32, None)     # bad prec
+    with pytest.raises(ValueError):
+        _lib.call("mnp_getrs_batched", 0, 10, 9, 1, 1, 10, 100, 1, 10, 2, 10, 0, 3, 10, 0, None)  # r > 8
+    with pytest.raises(ValueError):
+        _lib.call("mnp_bemret_factor", 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 32, None)
+    assert _lib.lib.mnp_rsolve_batched(0, 4, 4, 1, 1, 4, 16, 1, 4, 5, 4, 16, 5, 4, 16, 32, None) == -13  # B == X

@@ -1,0 +1,185 @@
+"""Pins of the CPU oracle (oracle/bem_oracle.py) against the reference's own validation
+tables and against mathematics. CPU only."""
+import os
+from math import factorial
+
+import numpy as np
+import pytest
+
+from oracle import bem_oracle as O
+from mnpbem_amd.mesh import trisphere, trirod
+from mnpbem_amd.comparticle import ComParticle
+from mnpbem_amd.materials import EpsConst
+from conftest import GOLDEN, ROOT
+
+GOLD = np.loadtxt(os.path.join(ROOT, "mnpbem_amd", "data", "gold.dat"), comments="%")
+PINF = np.load(os.path.join(ROOT, "mnpbem_amd", "data", "pinfty256.npz"))
+
+
+def _surface(part):
+    cp = ComParticle([EpsConst(1.0), EpsConst(2.0)], [part], [[2, 1]], 1, interp="curv")
+    p = cp.pc
+    return O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+
+
+def test_lgl_and_rule18_exactness():
+    x, w = O.lgl(7)
+    assert abs(w.sum() - 2) < 1e-14
+    # LGL with n+1 points integrates degree 2n-1 exactly
+    for d in range(14):
+        assert abs(np.sum(w * x ** d) - (0 if d % 2 else 2.0 / (d + 1))) < 1e-13
+    xs, ys, ws = O.rule18()
+    for a in range(12):
+        for b in range(12 - a):
+            exact = 2.0 * factorial(a) * factorial(b) / factorial(a + b + 2)
+            assert abs(np.sum(ws * xs ** a * ys ** b) - exact) < 1e-15
+
+
+def test_polar_rules_normalised():
+    (x3, y3, w3), (x4, y4, w4) = O.polar_rules((7, 5))
+    assert len(w3) == 144 and len(w4) == 192
+    assert abs(w3.sum() - 1) < 1e-15 and abs(w4.sum() - 4) < 1e-14
+    # radial nodes reach rho = 1 + 5e-7 (the rule's offset that keeps rho > 0 at the centre)
+    assert np.all(x3 >= -1e-6) and np.all(y3 >= -1e-6) and np.all(x3 + y3 <= 1 + 1e-6)
+    assert np.all(np.abs(x4) <= 1 + 1e-6) and np.all(np.abs(y4) <= 1 + 1e-6)
+
+
+def test_sphere_geometry_invariants():
+    s = _surface(trisphere(144, 20))
+    assert s.n == 284
+    # curved elements reproduce the sphere area to quadrature/interpolation accuracy
+    assert abs(s.area.sum() - 4 * np.pi * 100) / (4 * np.pi * 100) < 2e-4
+    assert np.allclose(np.linalg.norm(s.nvec, axis=1), 1)
+    assert np.all(np.sum(s.nvec * s.pos, axis=1) > 0)          # outward normals
+    assert np.allclose(np.sum(s.nvec * s.t1, axis=1), 0, atol=1e-14)
+
+
+def test_closed_surface_identity():
+    """Fuchs-Liu: sum_i a_i F_ij / a_j = -2 pi on a closed surface once the self terms carry
+    the correction (compgreen_ret.py:192-272)."""
+    s = _surface(trisphere(60, 10))
+    kind = O.refinement_kind(s)
+    corr, F = O.closed_correction(s, kind, [(np.arange(s.n), np.arange(s.n), 1.0)])
+    Fc = F + np.diag(corr)
+    colsum = np.sum(s.area[:, None] * Fc / s.area[None, :], axis=0)
+    assert np.allclose(colsum, -2 * np.pi, atol=1e-12)
+    # and the correction is small relative to 2 pi (the integration is already accurate)
+    assert np.max(np.abs(corr)) < 0.05 * 2 * np.pi
+
+
+def _brute(s, i, j, sub=12):
+    """g_0 and f_0 of pair (i, j) by a subdivided 28-point rule on the source element."""
+    tri = s._tri_nodes(j)
+    N, Nx, Ny = O._tri6(*O.rule18()[:2])
+    # subdivide the reference triangle into sub^2 triangles and map the rule
+    xs, ys, ws = O.rule18()
+    P, W = [], []
+    h = 1.0 / sub
+    for a in range(sub):
+        for b in range(sub - a):
+            for flip in (0, 1):
+                if flip and b == sub - a - 1:
+                    continue
+                if not flip:
+                    u = (a + xs) * h
+                    v = (b + ys) * h
+                else:
+                    u = (a + 1 - xs) * h
+                    v = (b + 1 - ys) * h
+                Nn, Nxx, Nyy = O._tri6(u, v)
+                for nodes in tri:
+                    p = Nn.T @ nodes
+                    J = np.cross(Nxx.T @ nodes, Nyy.T @ nodes)
+                    P.append(p)
+                    W.append(0.5 * ws * h * h * np.linalg.norm(J, axis=1))
+    P, W = np.vstack(P), np.concatenate(W)
+    v = s.pos[i] - P
+    r = np.linalg.norm(v, axis=1)
+    return np.sum(W / r), -(W / r ** 3) @ v
+
+
+def test_near_coefficients_match_fine_integration():
+    """Near-pair n = 0 coefficients are the integrals of 1/r and -r/r^3 over the source
+    element: compare with a 12x12-subdivided rule (converged to ~1e-7)."""
+    s = _surface(trisphere(60, 10))
+    kind = O.refinement_kind(s)
+    coef = O.refined_coefficients(s, kind)
+    pairs = [(i, j) for (i, j) in coef if kind[i, j] == 1][:12]
+    for i, j in pairs:
+        g, f = coef[(i, j)]
+        gb, fb = _brute(s, i, j)
+        assert abs(g[0] - gb) / abs(gb) < 2e-6
+        assert np.linalg.norm(f[:, 0] - fb) / np.linalg.norm(fb) < 2e-5
+
+
+def test_lu_chain_backward_error():
+    rng = np.random.default_rng(0)
+    n = 40
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    import scipy.linalg as sla
+    lu = sla.lu_factor(A)
+    x = sla.lu_solve(lu, np.eye(n))
+    assert np.linalg.norm(A @ x - np.eye(n)) < 1e-12 * np.linalg.norm(A) * np.linalg.norm(x)
+
+
+@pytest.fixture(scope="module")
+def gold_sphere():
+    part = trisphere(144, 20)
+    s = _surface(part)
+    ml = np.loadtxt(os.path.join(GOLDEN, "bemret_sphere_matlab.csv"), delimiter=",", skiprows=1)
+    eps = [lambda e: O.eps_const(e, 1.0), lambda e: O.eps_table(e, GOLD)]
+    out = O.solve_spectrum(s, np.zeros(s.n, int), [[2, 1]], eps, ml[:, 0], np.array([1.0, 0, 0]),
+                           np.array([0, 0, 1.0]), PINF["nvec"], PINF["area"], return_fields=True)
+    return ml, out
+
+
+def test_oracle_matches_matlab_sphere(gold_sphere):
+    """validation/03_bemret/sphere: trisphere(144, 20), gold, 41 wavelengths."""
+    ml, out = gold_sphere
+    assert np.max(np.abs(out["ext"] - ml[:, 1]) / ml[:, 1]) < 1e-9
+    assert np.max(np.abs(out["sca"] - ml[:, 2]) / ml[:, 2]) < 1e-9
+    assert np.max(np.abs(out["ext"] - out["sca"] - ml[:, 3]) / ml[:, 3]) < 1e-9
+
+
+def test_energy_balance_and_operator(gold_sphere):
+    ml, out = gold_sphere
+    assert np.all(out["ext"] - out["sca"] > 0)     # absorption >= 0
+    # the solution satisfies the exact 8N operator of bem_ret_iter (afun) to LU accuracy
+    f = out["fields"][5]
+    x = O.pack(*[np.asarray(v) for v in (f["sol"][0], f["sol"][1], f["sol"][2], f["sol"][3])])
+    phi, a, alpha, De = f["exc"]
+    b = O.pack(phi, a, De, alpha)
+    y = O.afun(f["G1"], f["G2"], f["H1"], f["H2"], out["green"].s.nvec,
+               f["eps1"], f["eps2"], f["k"], x)
+    assert np.linalg.norm(y - b) / np.linalg.norm(b) < 1e-11
+
+
+def test_oracle_matches_matlab_rod():
+    """validation/03_bemret/rod: trirod(10, 40, [15, 15, 15]) (curved quads), gold, both
+    polarisations, 3 of the 41 wavelengths."""
+    s = _surface(trirod(10.0, 40.0, [15, 15, 15]))
+    ml = np.loadtxt(os.path.join(GOLDEN, "bemret_rod_matlab.csv"), delimiter=",", skiprows=1)
+    sel = [0, 20, 40]
+    eps = [lambda e: O.eps_const(e, 1.0), lambda e: O.eps_table(e, GOLD)]
+    for pol, dr, ce, cs in (([1.0, 0, 0], [0, 0, 1.0], 1, 2), ([0, 0, 1.0], [1.0, 0, 0], 3, 4)):
+        out = O.solve_spectrum(s, np.zeros(s.n, int), [[2, 1]], eps, ml[sel, 0], np.array(pol), np.array(dr),
+                               PINF["nvec"], PINF["area"])
+        assert np.max(np.abs(out["ext"] - ml[sel, ce]) / ml[sel, ce]) < 1e-9
+        assert np.max(np.abs(out["sca"] - ml[sel, cs]) / ml[sel, cs]) < 1e-9
+
+
+def test_oracle_bemstat_matches_matlab():
+    """validation/02_bemstat/sphere: quasistatic trisphere(144, 20) in gold (x-polarised)."""
+    s = _surface(trisphere(144, 20))
+    ml = np.loadtxt(os.path.join(GOLDEN, "bemstat_sphere_matlab.csv"), delimiter=",", skiprows=1)
+    kind = O.refinement_kind(s)
+    corr, F = O.closed_correction(s, kind, [(np.arange(s.n), np.arange(s.n), 1.0)])
+    F = F + np.diag(corr)
+    pol = np.array([1.0, 0, 0])
+    for row in ml[::8]:
+        lam = row[0]
+        e_in = O.eps_table(lam, GOLD)[0]
+        sig = O.bemstat_solve(F, np.full(s.n, e_in), np.ones(s.n), -s.nvec @ pol)
+        ext, sca, ab = O.stat_cross_sections(s.pos, s.area, sig, pol, 2 * np.pi / lam)
+        assert abs(ext - row[1]) / row[1] < 1e-9
+        assert abs(sca - row[2]) / row[2] < 1e-9
