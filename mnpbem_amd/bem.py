@@ -108,7 +108,8 @@ class BEMRet:
         n = p.nfaces
         out = np.zeros((npol, 8, n), complex)
         for ip in range(npol):
-            rhs = torch.tensor(_pack_rhs(p, exc, k0, eps1, eps2, ip)[None], device=s.geo.dev)
+            # explicit (8 n, n, 1) strides: a numpy view with a new leading axis would carry stride 0
+            rhs = torch.tensor(_pack_rhs(p, exc, k0, eps1, eps2, ip), device=s.geo.dev).reshape(1, 8, n).contiguous()
             out[ip] = s.solve(rhs)[0].cpu().numpy()
         if npol == 1:
             o = out[0]

@@ -92,13 +92,14 @@ __global__ __launch_bounds__(256) void k_identity(int n, cx<T>* A, long ld, long
 // four at a time, X staged through LDS in 256-column chunks; partial sums of the waves are
 // combined in LDS at the end.
 constexpr int GV_MAXR = 8;
-template <typename T>
-__global__ __launch_bounds__(256) void k_gemv(int n, int r, const cx<T>* A, long lda, long sA, const cx<T>* X,
+// SUB = false: Y = A X (m x k times k x r); SUB = true: Y -= A X.
+template <typename T, bool SUB>
+__global__ __launch_bounds__(256) void k_gemv(int m, int n, int r, const cx<T>* A, long lda, long sA, const cx<T>* X,
                                               long ldx, long sX, cx<T>* Y, long ldy, long sY) {
     __shared__ cx<T> xs[256][GV_MAXR];
     const int b = blockIdx.y, t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int i = blockIdx.x * 64 + lane;
-    const bool row_ok = i < n;
+    const bool row_ok = i < m;
     A += (long)b * sA;
     X += (long)b * sX;
     Y += (long)b * sY;
@@ -158,6 +159,7 @@ __global__ __launch_bounds__(256) void k_gemv(int n, int r, const cx<T>* A, long
                 v = v + red[64 + lane][q];
                 v = v + red[128 + lane][q];
                 v = v + red[192 + lane][q];
+                if (SUB) v = Y[i + (long)q * ldy] - v;
                 Y[i + (long)q * ldy] = v;
             }
     }
@@ -166,9 +168,22 @@ __global__ __launch_bounds__(256) void k_gemv(int n, int r, const cx<T>* A, long
 template <typename T>
 void gemv_batched(int n, int r, int batch, const void* A, long lda, long sA, const void* X, long ldx, long sX,
                   void* Y, long ldy, long sY, hipStream_t s) {
-    hipLaunchKernelGGL(k_gemv<T>, dim3((n + 63) / 64, batch), dim3(256), 0, s, n, r, (const cx<T>*)A, lda, sA,
-                       (const cx<T>*)X, ldx, sX, (cx<T>*)Y, ldy, sY);
+    hipLaunchKernelGGL((k_gemv<T, false>), dim3((n + 63) / 64, batch), dim3(256), 0, s, n, n, r, (const cx<T>*)A, lda,
+                       sA, (const cx<T>*)X, ldx, sX, (cx<T>*)Y, ldy, sY);
 }
+
+// Y -= A X with A m x k (the off-diagonal updates of the blocked triangular solves).
+template <typename T>
+void gemv_sub_batched(int m, int k, int r, int batch, const void* A, long lda, long sA, const void* X, long ldx,
+                      long sX, void* Y, long ldy, long sY, hipStream_t s) {
+    if (m <= 0 || k <= 0) return;
+    hipLaunchKernelGGL((k_gemv<T, true>), dim3((m + 63) / 64, batch), dim3(256), 0, s, m, k, r, (const cx<T>*)A, lda,
+                       sA, (const cx<T>*)X, ldx, sX, (cx<T>*)Y, ldy, sY);
+}
+template void gemv_sub_batched<double>(int, int, int, int, const void*, long, long, const void*, long, long, void*,
+                                       long, long, hipStream_t);
+template void gemv_sub_batched<float>(int, int, int, int, const void*, long, long, const void*, long, long, void*,
+                                      long, long, hipStream_t);
 
 // ----------------------------------------------------------------------------- solve glue
 // Every "vector block" is N x c complex, column-major with leading dimension n, stride per

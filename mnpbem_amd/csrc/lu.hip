@@ -93,26 +93,27 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(int n, int j0, int nb, 
     }
 }
 
-// apply interchanges ipiv[j0 .. j0+nb) to every column outside [j0, j0+nb)
+// apply interchanges ipiv[r0 .. r1) to the columns [a0, a1) and [b0, b1)
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp(int n, int j0, int nb, cx<T>* A, long lda, long sA,
-                                               const int* ipiv, long sP) {
+__global__ __launch_bounds__(256) void k_laswp(int r0, int r1, int a0, int a1, int b0, int b1, cx<T>* A, long lda,
+                                               long sA, const int* ipiv, long sP) {
     const int b = blockIdx.y;
     int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= n - nb) return;
-    if (c >= j0) c += nb;
+    const int na = a1 - a0;
+    if (c >= na + (b1 - b0)) return;
+    c = (c < na) ? a0 + c : b0 + (c - na);
     A += (long)b * sA + (long)c * lda;
     ipiv += (long)b * sP;
-    for (int r = j0; r < j0 + nb; ++r) {
+    for (int r = r0; r < r1; ++r) {
         int p = ipiv[r];
         if (p != r) { cx<T> x = A[r]; A[r] = A[p]; A[p] = x; }
     }
 }
 
-// X = L^{-1} X for the unit-lower nb x nb block at (j0, j0), X = rows j0.. j0+nb of columns
-// j0+nb .. n-1; one lane per column.
+// X = L^{-1} X for the unit-lower nb x nb block at (j0, j0), X = rows j0 .. j0+nb of the
+// columns [c0, c1); one lane per column.
 template <typename T, int NB>
-__global__ __launch_bounds__(128) void k_trsm_lln(int n, int j0, int nb, cx<T>* A, long lda, long sA) {
+__global__ __launch_bounds__(128) void k_trsm_lln(int c0, int c1, int j0, int nb, cx<T>* A, long lda, long sA) {
     __shared__ cx<T> L[NB][NB + 1];
     const int b = blockIdx.y;
     A += (long)b * sA;
@@ -121,8 +122,8 @@ __global__ __launch_bounds__(128) void k_trsm_lln(int n, int j0, int nb, cx<T>* 
         L[r][c] = A[(j0 + r) + (long)(j0 + c) * lda];
     }
     __syncthreads();
-    int col = j0 + nb + blockIdx.x * 128 + threadIdx.x;
-    if (col >= n) return;
+    int col = c0 + blockIdx.x * 128 + threadIdx.x;
+    if (col >= c1) return;
     cx<T>* x = A + j0 + (long)col * lda;
     cx<T> v[NB];
 #pragma unroll
@@ -170,25 +171,64 @@ __global__ __launch_bounds__(64) void k_perm(int n, const int* ipiv, long sP, in
 }
 
 template <typename T>
+static void laswp(int r0, int r1, int a0, int a1, int b0, int b1, cx<T>* a, long lda, long sA, int* ipiv, long sP,
+                  int batch, hipStream_t s) {
+    int cols = (a1 - a0) + (b1 - b0);
+    if (cols <= 0 || r1 <= r0) return;
+    hipLaunchKernelGGL(k_laswp<T>, dim3((cols + 255) / 256, batch), dim3(256), 0, s, r0, r1, a0, a1, b0, b1, a, lda,
+                       sA, ipiv, sP);
+}
+
+template <typename T>
+static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, long sA, int batch, hipStream_t s) {
+    if (c1 <= c0) return;
+    dim3 g((c1 - c0 + 127) / 128, batch);
+    if (nb <= 32)
+        hipLaunchKernelGGL((k_trsm_lln<T, 32>), g, dim3(128), 0, s, c0, c1, j0, w, a, lda, sA);
+    else
+        hipLaunchKernelGGL((k_trsm_lln<T, 64>), g, dim3(128), 0, s, c0, c1, j0, w, a, lda, sA);
+}
+
+// Two-level blocking: outer blocks of NBO columns carry the large trailing update (a GEMM with
+// k = NBO, so C is read and written once per NBO columns), inner panels of nb columns are
+// factored by k_panel inside the outer block.
+constexpr int NBO = 128;
+
+template <typename T>
 void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                    hipStream_t s) {
     cx<T>* a = (cx<T>*)A;
-    for (int j0 = 0; j0 < n; j0 += nb) {
-        int w = (n - j0 < nb) ? (n - j0) : nb;
-        hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv, sP, info);
-        if (n - w > 0)
-            hipLaunchKernelGGL(k_laswp<T>, dim3((n - w + 255) / 256, batch), dim3(256), 0, s, n, j0, w, a, lda, sA,
-                               ipiv, sP);
-        int rest = n - j0 - w;
-        if (rest > 0) {
-            if (nb <= 32)
-                hipLaunchKernelGGL((k_trsm_lln<T, 32>), dim3((rest + 127) / 128, batch), dim3(128), 0, s, n, j0, w, a,
-                                   lda, sA);
-            else
-                hipLaunchKernelGGL((k_trsm_lln<T, 64>), dim3((rest + 127) / 128, batch), dim3(128), 0, s, n, j0, w, a,
-                                   lda, sA);
-            gemm_sub<T>(rest, rest, w, a + (j0 + w) + (long)j0 * lda, lda, sA, a + j0 + (long)(j0 + w) * lda, lda, sA,
-                        a + (j0 + w) + (long)(j0 + w) * lda, lda, sA, batch, s);
+    const int NB = (NBO > nb) ? (NBO / nb) * nb : nb;
+    for (int J0 = 0; J0 < n; J0 += NB) {
+        const int W = (n - J0 < NB) ? (n - J0) : NB;
+        const int JE = J0 + W;
+        // factor the tall outer panel A[J0:n, J0:JE]
+        for (int j0 = J0; j0 < JE; j0 += nb) {
+            int w = (JE - j0 < nb) ? (JE - j0) : nb;
+            hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv, sP,
+                               info);
+            laswp<T>(j0, j0 + w, J0, j0, j0 + w, JE, a, lda, sA, ipiv, sP, batch, s);
+            int rest = JE - j0 - w;
+            if (rest > 0) {
+                trsm_lln<T>(j0 + w, JE, j0, w, nb, a, lda, sA, batch, s);
+                gemm_sub<T>(n - j0 - w, rest, w, a + (j0 + w) + (long)j0 * lda, lda, sA, a + j0 + (long)(j0 + w) * lda,
+                            lda, sA, a + (j0 + w) + (long)(j0 + w) * lda, lda, sA, batch, s);
+            }
+        }
+        // interchanges of the outer panel on the columns outside it
+        laswp<T>(J0, JE, 0, J0, JE, n, a, lda, sA, ipiv, sP, batch, s);
+        if (JE < n) {
+            // U12 = L11^{-1} A12 for the W x (n - JE) block row, nb rows at a time
+            for (int j0 = J0; j0 < JE; j0 += nb) {
+                int w = (JE - j0 < nb) ? (JE - j0) : nb;
+                trsm_lln<T>(JE, n, j0, w, nb, a, lda, sA, batch, s);
+                if (j0 + w < JE)
+                    gemm_sub<T>(JE - j0 - w, n - JE, w, a + (j0 + w) + (long)j0 * lda, lda, sA,
+                                a + j0 + (long)JE * lda, lda, sA, a + (j0 + w) + (long)JE * lda, lda, sA, batch, s);
+            }
+            // trailing update with k = W
+            gemm_sub<T>(n - JE, n - JE, W, a + JE + (long)J0 * lda, lda, sA, a + J0 + (long)JE * lda, lda, sA,
+                        a + JE + (long)JE * lda, lda, sA, batch, s);
         }
     }
 }
@@ -290,26 +330,47 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
     const cx<T>* a = (const cx<T>*)LU;
     cx<T>* bm = (cx<T>*)B;
     dim3 g((m + 127) / 128, batch);
-    for (int j0 = 0; j0 < n; j0 += nb) {
-        int w = (n - j0 < nb) ? (n - j0) : nb;
+    const int NB = (NBO > nb) ? (NBO / nb) * nb : nb;
+    auto upper = [&](int j0, int w) {
         if (nb <= 32)
             hipLaunchKernelGGL((k_rtrsm_upper<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
         else
             hipLaunchKernelGGL((k_rtrsm_upper<T, 64>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
-        int rest = n - j0 - w;
-        if (rest > 0)
-            gemm_sub<T>(m, rest, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)(j0 + w) * lda, lda, sA,
-                        bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
-    }
-    int last = ((n - 1) / nb) * nb;
-    for (int j0 = last; j0 >= 0; j0 -= nb) {
-        int w = (n - j0 < nb) ? (n - j0) : nb;
+    };
+    auto lower = [&](int j0, int w) {
         if (nb <= 32)
             hipLaunchKernelGGL((k_rtrsm_lower_unit<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
         else
             hipLaunchKernelGGL((k_rtrsm_lower_unit<T, 64>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
-        if (j0 > 0)
-            gemm_sub<T>(m, j0, w, bm + (long)j0 * ldb, ldb, sB, a + j0, lda, sA, bm, ldb, sB, batch, s);
+    };
+    // phase 1: Y U = B, outer blocks left to right
+    for (int J0 = 0; J0 < n; J0 += NB) {
+        const int JE = (n - J0 < NB) ? n : J0 + NB;
+        for (int j0 = J0; j0 < JE; j0 += nb) {
+            int w = (JE - j0 < nb) ? (JE - j0) : nb;
+            upper(j0, w);
+            if (j0 + w < JE)
+                gemm_sub<T>(m, JE - j0 - w, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)(j0 + w) * lda, lda, sA,
+                            bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
+        }
+        if (JE < n)
+            gemm_sub<T>(m, n - JE, JE - J0, bm + (long)J0 * ldb, ldb, sB, a + J0 + (long)JE * lda, lda, sA,
+                        bm + (long)JE * ldb, ldb, sB, batch, s);
+    }
+    // phase 2: Z L = Y, outer blocks right to left
+    const int lastJ = ((n - 1) / NB) * NB;
+    for (int J0 = lastJ; J0 >= 0; J0 -= NB) {
+        const int JE = (n - J0 < NB) ? n : J0 + NB;
+        const int lastj = J0 + ((JE - J0 - 1) / nb) * nb;
+        for (int j0 = lastj; j0 >= J0; j0 -= nb) {
+            int w = (JE - j0 < nb) ? (JE - j0) : nb;
+            lower(j0, w);
+            if (j0 > J0)
+                gemm_sub<T>(m, j0 - J0, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)J0 * lda, lda, sA,
+                            bm + (long)J0 * ldb, ldb, sB, batch, s);
+        }
+        if (J0 > 0)
+            gemm_sub<T>(m, J0, JE - J0, bm + (long)J0 * ldb, ldb, sB, a + J0, lda, sA, bm, ldb, sB, batch, s);
     }
     hipLaunchKernelGGL(k_gather_cols<T>, dim3((m + 255) / 256, n, batch), dim3(256), 0, s, m, n, bm, ldb, sB,
                        (cx<T>*)X, ldx, sX, qinv, sQ);
@@ -322,10 +383,12 @@ template void rsolve_batched<float>(int, int, int, const void*, long, long, cons
 
 // ---------------------------------------------------------------------------------------------
 // Left solve with a few right-hand sides: X = A^{-1} B = U^{-1} L^{-1} (B[pf]).
-// One workgroup (512 lanes) per matrix. Rows are processed in 64-row blocks: the diagonal block
-// is staged in LDS by all lanes, wave 0 solves it with lane shuffles (lane = row), then every
-// lane updates the rows below (above) with the block's solution held in LDS; the update loop
-// issues its 64 column loads independently of the accumulation.
+// k_trsv: one workgroup (512 lanes) per matrix solves an n <= 256 triangle in 64-row blocks:
+// the diagonal block is staged in LDS by all lanes, wave q solves right-hand side q with lane
+// shuffles (lane = row), then every lane updates the rows below (above) with the block's
+// solution held in LDS, issuing the block's column loads independently of the accumulation.
+// getrs_batched walks the full triangle in 256-row blocks: k_trsv on the diagonal block, then
+// the rows below (above) are updated by the multi-workgroup GEMV (gemv_sub_batched).
 constexpr int TS_MAXR = 8;
 constexpr int TS_THREADS = 512;
 
@@ -347,8 +410,10 @@ __global__ __launch_bounds__(TS_THREADS) void k_trsv(int n, int r, const cx<T>* 
             D[rr][cc] = (rr < bs && cc < bs) ? A[(i0 + rr) + (long)(i0 + cc) * lda] : mk<T>(0, 0);
         }
         __syncthreads();
-        if (t < 64) {
-            for (int q = 0; q < r; ++q) {
+        // one wave per right-hand side
+        const int wave = t >> 6;
+        for (int q = wave; q < r; q += TS_THREADS / 64) {
+            {
                 cx<T> v = lane < bs ? X[i0 + lane + (long)q * ldx] : mk<T>(0, 0);
                 if (LOWER) {
                     for (int c = 0; c < bs; ++c) {
@@ -403,6 +468,12 @@ __global__ __launch_bounds__(TS_THREADS) void k_trsv(int n, int r, const cx<T>* 
 }
 
 template <typename T>
+void gemv_sub_batched(int m, int k, int r, int batch, const void* A, long lda, long sA, const void* X, long ldx,
+                      long sX, void* Y, long ldy, long sY, hipStream_t s);
+
+constexpr int TS_BLOCK = 256;
+
+template <typename T>
 __global__ __launch_bounds__(256) void k_gather_rows(int n, int r, const cx<T>* B, long ldb, long sB, cx<T>* X,
                                                      long ldx, long sX, const int* pf, long sQ) {
     const int b = blockIdx.y;
@@ -418,10 +489,25 @@ void getrs_batched(int n, int r, int batch, const void* LU, long lda, long sA, c
                    long ldb, long sB, void* X, long ldx, long sX, hipStream_t s) {
     hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, batch), dim3(256), 0, s, n, r, (const cx<T>*)B, ldb,
                        sB, (cx<T>*)X, ldx, sX, pf, sQ);
-    hipLaunchKernelGGL((k_trsv<T, true>), dim3(batch), dim3(TS_THREADS), 0, s, n, r, (const cx<T>*)LU, lda, sA,
-                       (cx<T>*)X, ldx, sX);
-    hipLaunchKernelGGL((k_trsv<T, false>), dim3(batch), dim3(TS_THREADS), 0, s, n, r, (const cx<T>*)LU, lda, sA,
-                       (cx<T>*)X, ldx, sX);
+    const cx<T>* a = (const cx<T>*)LU;
+    cx<T>* x = (cx<T>*)X;
+    for (int i0 = 0; i0 < n; i0 += TS_BLOCK) {
+        int bs = (n - i0 < TS_BLOCK) ? (n - i0) : TS_BLOCK;
+        hipLaunchKernelGGL((k_trsv<T, true>), dim3(batch), dim3(TS_THREADS), 0, s, bs, r, a + i0 + (long)i0 * lda, lda,
+                           sA, x + i0, ldx, sX);
+        int rest = n - i0 - bs;
+        if (rest > 0)
+            gemv_sub_batched<T>(rest, bs, r, batch, a + (i0 + bs) + (long)i0 * lda, lda, sA, x + i0, ldx, sX,
+                                x + i0 + bs, ldx, sX, s);
+    }
+    int last = ((n - 1) / TS_BLOCK) * TS_BLOCK;
+    for (int i0 = last; i0 >= 0; i0 -= TS_BLOCK) {
+        int bs = (n - i0 < TS_BLOCK) ? (n - i0) : TS_BLOCK;
+        hipLaunchKernelGGL((k_trsv<T, false>), dim3(batch), dim3(TS_THREADS), 0, s, bs, r, a + i0 + (long)i0 * lda,
+                           lda, sA, x + i0, ldx, sX);
+        if (i0 > 0)
+            gemv_sub_batched<T>(i0, bs, r, batch, a + (long)i0 * lda, lda, sA, x + i0, ldx, sX, x, ldx, sX, s);
+    }
 }
 
 template void getrs_batched<double>(int, int, int, const void*, long, long, const int*, long, const void*, long,

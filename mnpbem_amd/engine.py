@@ -202,19 +202,21 @@ class RetSolverBatch:
     def solve(self, rhs):
         """Solve for packed right-hand sides rhs (B, 8, n) complex128 -> (B, 8, n) complex128."""
         n, nl = self.geo.n, self.nlam
+        if rhs.dim() != 3 or rhs.shape[1:] != (8, n) or rhs.stride() != (8 * n, n, 1):
+            raise ValueError("rhs must be a contiguous (batch, 8, n) tensor")
         if self.precision == "double":
-            out = torch.empty_like(rhs)
+            out = torch.empty(rhs.shape, dtype=rhs.dtype, device=rhs.device)
             self._chain(rhs, out)
             return out
         r32 = torch.empty((self.batch, 8, n), dtype=C64, device=rhs.device)
         dx = torch.empty_like(r32)
         call("mnp_cast_batched", 1, 8 * n, nl, _p(rhs), 8 * n, _p(r32), 8 * n, stream_handle())
         self._chain(r32, dx)
-        x = torch.zeros_like(rhs)
+        x = torch.zeros(rhs.shape, dtype=rhs.dtype, device=rhs.device)
         call("mnp_accum_batched", 1, 8 * n, nl, _p(dx), 8 * n, _p(x), 8 * n, None, stream_handle())
         if self.precision == "single":
             return x
-        r = torch.empty_like(rhs)
+        r = torch.empty(rhs.shape, dtype=rhs.dtype, device=rhs.device)
         norms = torch.zeros((self.batch, 2), dtype=torch.float64, device=rhs.device)
         active = torch.ones(self.batch, dtype=torch.int32, device=rhs.device)
         hist = []
