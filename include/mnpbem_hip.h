@@ -64,7 +64,7 @@ int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, in
                       void* stream);
 
 /* Row permutation pf (P b = b[pf]) and its inverse from the pivots of mnp_getrf_batched
- * (n <= 16384). Part of lu_solve_dispatch (utils/gpu.py:194-251). */
+ * (any n). Part of lu_solve_dispatch (utils/gpu.py:194-251). */
 int mnp_perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, void* stream);
 
 /* X = A^{-1} B for r <= 8 columns from an LU factor (B != X).
@@ -139,6 +139,82 @@ int mnp_cast_batched(int prec_dst, long count, int batch, const void* src, long 
 /* dst (complex double) += src where active[b] (refinement update; no counterpart). */
 int mnp_accum_batched(int prec_src, long count, int batch, const void* src, long ss, void* dst, long sd,
                       const int* active, void* stream);
+
+
+/* ---------------------------------------------------------------------------------------------
+ * H-matrices (BEMRetIter with hmatrix=True: greenfun/aca_compgreen_ret.py:20-208,
+ * greenfun/hmatrix.py:94-541, greenfun/clustertree.py:13-380). The cluster tree and the block
+ * partition are built on the host; entries, ACA and products run here.
+ *
+ * Kinds: 0 = G1, 1 = G2, 2 = H1, 3 = H2. Blocks are (r0, c0, m, n) in cluster order. */
+
+/* Geometry and refined-pair table for on-the-fly Green-function entries at one wavelength
+ * (same values as mnp_green_assemble). rowptr/rcol: CSR of the refined pairs with ascending
+ * columns per row; rg/rf: their Taylor coefficients in that order; kmat: nmat complex. */
+typedef struct mnp_green_geom {
+    int n, npart, order;
+    const double* pos;
+    const double* nvec;
+    const double* area;
+    const int* fpart;
+    const int* con_in;
+    const int* con_out;
+    const double* kmat;
+    const int* rowptr;
+    const int* rcol;
+    const double* rg;
+    const double* rf;
+} mnp_green_geom;
+
+/* A built set of four H-matrices on one tree (all device pointers). D: complex double dense
+ * blocks, kind q at D + q dstride, block b at doff[b]; U, V: complex float low-rank factors
+ * (m x rank, n x rank) of kind q and block b at uoff[q nlr + b], voff[q nlr + b];
+ * rank: nlr x 4; tmp/toff: complex double scratch of 8 x (largest rank of block b) per block. */
+typedef struct mnp_hmat {
+    int n, nd, nlr;
+    const int* c2p;
+    const int* dblk;
+    const long* doff;
+    const void* D;
+    long dstride;
+    const int* lblk;
+    const long* uoff;
+    const long* voff;
+    const void* U;
+    const void* V;
+    const int* rank;
+    void* tmp;
+    const long* toff;
+} mnp_hmat;
+
+/* Dense (inadmissible) blocks of all four kinds, entries evaluated on the fly
+ * (HMatrix.aca dense part, hmatrix.py:172-180; entries of CompGreenRet.eval). */
+int mnp_hmat_fill_dense(const mnp_green_geom* g, const int* c2p, int nd, const int* dblk, const long* doff, void* D,
+                        long dstride, void* stream);
+
+/* Partially pivoted ACA of the admissible blocks of one kind (HMatrix._aca_block,
+ * hmatrix.py:195-336), one workgroup per block, complex double factors U (m x kr) and
+ * V (n x kr), kr = min(m, n, kmax), at uws[b], vws[b]; rank[4 b + kind] receives the rank.
+ * Blocks must satisfy m, n <= 8192. */
+int mnp_hmat_aca(const mnp_green_geom* g, const int* c2p, int kind, int nlr, const int* lblk, const long* uws,
+                 const long* vws, void* U, void* V, int* rank, double htol, int kmax, void* stream);
+
+/* Copy the first rank[4 b + kind] columns of the ACA factors to complex float storage at
+ * uoff[b], voff[b] (leading dimensions m and n). No counterpart (the reference keeps complex128). */
+int mnp_hmat_compact(int nlr, const int* lblk, int kind, const int* rank, const long* uws, const long* vws,
+                     const void* Uws, const void* Vws, const long* uoff, const long* voff, void* U, void* V,
+                     void* stream);
+
+/* y += H_kind x for r <= 8 complex double columns in cluster order (HMatrix.mtimes_vec,
+ * hmatrix.py:450-541, without its permutations). */
+int mnp_hmat_matvec(const mnp_hmat* h, int kind, int r, const void* x, long ldx, void* y, long ldy, void* stream);
+
+/* The packed 8N operator of BEMRetIter._afun (bem_ret_iter.py:1362-1501) on the four
+ * H-matrices, one wavelength: y = A x, or y = rhs - A x with norms = (|y|^2, |rhs|^2) when rhs
+ * is given. x, rhs, y: 8 n complex double (y != x); work: 64 n complex double. */
+int mnp_bem_operator_hmat(const mnp_hmat* h, const double* nvec, const double* L1, const double* L2,
+                          const double* kvac, const void* x, const void* rhs, void* y, void* work, long sW,
+                          double* norms, void* stream);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,24 @@ _SIGS = {
     "mnp_mupdate": [L, I, P, L, P, P, P],
     "mnp_cast_batched": [I, L, I, P, L, P, L, P],
     "mnp_accum_batched": [I, L, I, P, L, P, L, P, P],
+    "mnp_hmat_fill_dense": [P, P, I, P, P, P, L, P],
+    "mnp_hmat_aca": [P, P, I, I, P, P, P, P, P, P, C.c_double, I, P],
+    "mnp_hmat_compact": [I, P, I, P, P, P, P, P, P, P, P, P, P],
+    "mnp_hmat_matvec": [P, I, I, P, L, P, L, P],
+    "mnp_bem_operator_hmat": [P, P, P, P, P, P, P, P, P, L, P, P],
 }
+
+
+class GreenGeomC(C.Structure):
+    """mnp_green_geom of include/mnpbem_hip.h (device pointers)."""
+    _fields_ = [("n", I), ("npart", I), ("order", I)] + [
+        (f, P) for f in ("pos", "nvec", "area", "fpart", "con_in", "con_out", "kmat", "rowptr", "rcol", "rg", "rf")]
+
+
+class HMatC(C.Structure):
+    """mnp_hmat of include/mnpbem_hip.h (device pointers)."""
+    _fields_ = [("n", I), ("nd", I), ("nlr", I), ("c2p", P), ("dblk", P), ("doff", P), ("D", P), ("dstride", L),
+                ("lblk", P), ("uoff", P), ("voff", P), ("U", P), ("V", P), ("rank", P), ("tmp", P), ("toff", P)]
 
 
 def _load():

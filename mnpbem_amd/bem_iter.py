@@ -9,6 +9,10 @@ Hessenberg matrix is on the host. Preconditioner 'bemret' is the BEMRet LU chain
 same wavelength (as the reference's default, bem_ret_iter.py:801-1086), in complex64
 (``precond_precision='single'``) or complex128; 'none' runs unpreconditioned.
 
+``hmatrix=True`` (options ``htol``, ``kmax``, ``cleaf``, ``eta``, ``fadmiss`` as the reference,
+bem_ret_iter.py:488-580) replaces the dense operator by H-matrices: cluster tree and block
+partition on the host, entries, ACA and products on the device (``mnpbem_amd.hmatrix``).
+
 Differences from the reference: the solver accepts an initial guess ``x0`` (the reference
 always starts from zero, bem_ret_iter.py:1695), and the residual reported by ``info()`` is the
 true relative residual |b - A x| / |b| recomputed on the device after the last restart.
@@ -113,14 +117,19 @@ class DeviceGMRES:
 class BEMRetIter:
     """Iterative BEMRet. Options (reference names): ``solver='gmres'``, ``tol=1e-4``,
     ``maxit=200``, ``restart=None`` (= min(n, maxit)), ``precond='hmat'|'bemret'|None``,
-    ``preconditioner='auto'|'none'`` and ``precond_precision='single'|'double'``."""
+    ``preconditioner='auto'|'none'``, ``precond_precision='single'|'double'``, ``hmatrix``,
+    ``htol=1e-6``, ``kmax=[4, 100]``, ``cleaf=200``, ``eta=2.5``, ``fadmiss``."""
 
     def __init__(self, p, enei=None, **op):
         if p is None:
             raise ValueError("BEMRetIter: 'p' must be a ComParticle, got None.")
-        if op.get("hmatrix", False):
-            raise NotImplementedError("the H-matrix (ACA) operator is not provided; the dense device operator "
-                                      "is used with hmatrix=False")
+        self.hmatrix = bool(op.get("hmatrix", False))
+        top = lambda v: max(v) if hasattr(v, "__iter__") else v
+        self.htol = float(top(op.get("htol", 1e-6)))
+        self.kmax = int(top(op.get("kmax", [4, 100])))
+        self.cleaf = int(op.get("cleaf", 200))
+        self.eta = float(op.get("eta", 2.5))
+        self.fadmiss = op.get("fadmiss", None)
         self.p = p
         self.solver = op.get("solver", "gmres")
         if self.solver != "gmres":
@@ -129,8 +138,10 @@ class BEMRetIter:
         self.maxit = int(op.get("maxit", 200))
         self.restart = op.get("restart", None)
         pc = op.get("preconditioner", "auto")
-        pr = op.get("precond", "hmat")
-        self.use_precond = not (pc == "none" or pr is None)
+        pr = op.get("precond", None if self.hmatrix else "hmat")
+        # with hmatrix=True the reference preconditions with the dense BEMRet chain unless
+        # preconditioner='none' (bem_ret_iter.py:1688-1692, :1745-1774)
+        self.use_precond = pc != "none" if self.hmatrix else not (pc == "none" or pr is None)
         self.precond_precision = op.get("precond_precision", "single")
         self.g = op.get("green") or CompGreenRet(p, p, **{k: v for k, v in op.items()
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
@@ -145,10 +156,20 @@ class BEMRetIter:
         if self.enei is not None and np.isclose(self.enei, enei):
             return
         geo = self.g.geo
-        self._exact = engine.GreenBatch(geo, 1, torch.complex128)
         self._kmat, self._L1, self._L2, self._kvac, _, _ = geo.media([enei])
-        self._exact.assemble(self._kmat, 1)
-        self._opwork = torch.empty((1, 32, geo.n), dtype=torch.complex128, device=geo.dev)
+        self._exact = self._hm = None
+        if self.hmatrix:
+            from .hmatrix import ClusterTree, HMatrixRet
+            if getattr(self, "_tree", None) is None:
+                self._tree = ClusterTree(self.p.pos, self.cleaf, self.p.face_particle)
+            self._hm = None
+            torch.cuda.empty_cache()
+            self._hm = HMatrixRet(geo, self._tree, enei, self.htol, self.kmax, self.eta, self.fadmiss)
+            self._opwork = torch.empty((64, geo.n), dtype=torch.complex128, device=geo.dev)
+        else:
+            self._exact = engine.GreenBatch(geo, 1, torch.complex128)
+            self._exact.assemble(self._kmat, 1)
+            self._opwork = torch.empty((1, 32, geo.n), dtype=torch.complex128, device=geo.dev)
         self._pre = None
         if self.use_precond:
             prec = "mixed" if self.precond_precision == "single" else "double"
@@ -159,6 +180,12 @@ class BEMRetIter:
 
     def _apply_A(self, x, y, rhs=None):
         e, geo, n = self._exact, self.g.geo, self.g.geo.n
+        if self._hm is not None:
+            call("mnp_bem_operator_hmat", self._hm.desc_ptr, geo.nvec.data_ptr(), self._L1.data_ptr(),
+                 self._L2.data_ptr(), self._kvac.data_ptr(), x.data_ptr(),
+                 rhs.data_ptr() if rhs is not None else None, y.data_ptr(), self._opwork.data_ptr(), 64 * n, None,
+                 engine.stream_handle())
+            return
         call("mnp_bem_operator", n, 1, e.G1.data_ptr(), e.G2.data_ptr(), e.H1.data_ptr(), e.H2.data_ptr(), e.ld,
              e.stride, geo.nvec.data_ptr(), self._L1.data_ptr(), self._L2.data_ptr(), self._kvac.data_ptr(),
              x.data_ptr(), 8 * n, rhs.data_ptr() if rhs is not None else None, 8 * n, y.data_ptr(), 8 * n,

@@ -519,6 +519,10 @@ template void gemv_batched<float>(int, int, int, const void*, long, long, const 
                                   hipStream_t);
 
 // Exact FP64 operator / residual. work: 32 N complex double per wavelength.
+void op_prep(const Face& f, int batch, const void* x, long sx, void* X1, void* X2, long sX, hipStream_t s);
+void op_finish(const Face& f, int batch, const void* GX1, const void* GX2, const void* HX1, long sH1, const void* HX2,
+               long sX, const void* rhs, long sr, void* y, long sy, double* norms, hipStream_t s);
+
 void bem_operator(const Face& f, int batch, const void* G1, const void* G2, const void* H1, const void* H2, long ld,
                   long stride, const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work,
                   long sW, double* norms, hipStream_t s) {
@@ -526,8 +530,7 @@ void bem_operator(const Face& f, int batch, const void* G1, const void* G2, cons
     const long N = n;
     cx<double>* w = (cx<double>*)work;
     cx<double>*X1 = w, *X2 = w + 8 * N, *GX1 = w + 16 * N, *GX2 = w + 24 * N;
-    dim3 g((n + 255) / 256, batch);
-    hipLaunchKernelGGL(k_op_prep, g, dim3(256), 0, s, f, (const cx<double>*)x, sx, X1, X2, sW);
+    op_prep(f, batch, x, sx, X1, X2, sW, s);
     gemv_batched<double>(n, 8, batch, G1, ld, stride, X1, N, sW, GX1, N, sW, s);
     gemv_batched<double>(n, 8, batch, G2, ld, stride, X2, N, sW, GX2, N, sW, s);
     // H1 X1 is staged in y (rewritten row by row by k_op_finish), H2 X2 in X1 (consumed by then);
@@ -535,9 +538,26 @@ void bem_operator(const Face& f, int batch, const void* G1, const void* G2, cons
     cx<double>* HX1 = (cx<double>*)y;
     gemv_batched<double>(n, 8, batch, H1, ld, stride, X1, N, sW, HX1, N, sy, s);
     gemv_batched<double>(n, 8, batch, H2, ld, stride, X2, N, sW, X1, N, sW, s);  // HX2 into X1 (X2 consumed)
-    if (norms) hipMemsetAsync(norms, 0, sizeof(double) * 2 * batch, s);
-    hipLaunchKernelGGL(k_op_finish, g, dim3(256), 0, s, f, GX1, GX2, HX1, sy, X1, sW, (const cx<double>*)rhs, sr,
-                       (cx<double>*)y, sy, norms);
+    op_finish(f, batch, GX1, GX2, HX1, sy, X1, sW, rhs, sr, y, sy, norms, s);
+}
+
+__global__ void k_zero_norms(int count, double* v) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) v[i] = 0.0;
+}
+
+void op_prep(const Face& f, int batch, const void* x, long sx, void* X1, void* X2, long sX, hipStream_t s) {
+    hipLaunchKernelGGL(k_op_prep, dim3((f.n + 255) / 256, batch), dim3(256), 0, s, f, (const cx<double>*)x, sx,
+                       (cx<double>*)X1, (cx<double>*)X2, sX);
+}
+
+void op_finish(const Face& f, int batch, const void* GX1, const void* GX2, const void* HX1, long sH1, const void* HX2,
+               long sX, const void* rhs, long sr, void* y, long sy, double* norms, hipStream_t s) {
+    // norms are cleared by a kernel (no memset node if the caller captures the stream)
+    if (norms) hipLaunchKernelGGL(k_zero_norms, dim3((2 * batch + 255) / 256), dim3(256), 0, s, 2 * batch, norms);
+    hipLaunchKernelGGL(k_op_finish, dim3((f.n + 255) / 256, batch), dim3(256), 0, s, f, (const cx<double>*)GX1,
+                       (const cx<double>*)GX2, (const cx<double>*)HX1, sH1, (const cx<double>*)HX2, sX,
+                       (const cx<double>*)rhs, sr, (cx<double>*)y, sy, norms);
 }
 
 void planewave_rhs(const PW& p, int batch, void* out, long so, hipStream_t s) {

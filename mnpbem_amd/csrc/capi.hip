@@ -9,6 +9,8 @@
 //   * return value 0 = ok, -k = argument k invalid, 1000 + hipError_t = launch failure.
 #include "common.hpp"
 #include "args.hpp"
+#include "hmat.hpp"
+#include "../../include/mnpbem_hip.h"
 
 namespace mnp {
 template <typename OT>
@@ -150,7 +152,7 @@ int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, in
 }
 
 int mnp_perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, void* stream) {
-    REQUIRE(n > 0 && n <= 16384, 1);
+    REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1, 2);
     REQUIRE(ipiv && pf && qinv, 3);
     perm_batched(n, batch, ipiv, sP, pf, qinv, sQ, (hipStream_t)stream);
@@ -227,7 +229,7 @@ int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, 
                       long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, const double* L1,
                       const double* L2, const double* kvac, int nb, void* stream) {
     REQUIRE(prec == 0 || prec == 1, 1);
-    REQUIRE(n > 0 && n <= 16384, 2);
+    REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
     REQUIRE(G1 && G2 && H1 && H2 && S1 && S2 && D, 4);
     REQUIRE(ld >= n, 11);
@@ -390,6 +392,86 @@ int mnp_accum_batched(int prec_src, long count, int batch, const void* src, long
         accum_batched<float>(count, batch, src, ss, dst, sd, active, (hipStream_t)stream);
     else
         accum_batched<double>(count, batch, src, ss, dst, sd, active, (hipStream_t)stream);
+    return launch_status();
+}
+
+
+// ------------------------------------------------------------------------- H-matrices
+static GreenGeom to_geom(const mnp_green_geom* g) {
+    return GreenGeom{g->n,     g->npart,   g->order, g->pos,  g->nvec, g->area, g->fpart,
+                     g->con_in, g->con_out, g->kmat,  g->rowptr, g->rcol, g->rg,   g->rf};
+}
+static HMat to_hmat(const mnp_hmat* h) {
+    return HMat{h->n,    h->nd,   h->nlr,  h->c2p,  h->dblk, h->doff, h->D,    h->dstride,
+                h->lblk, h->uoff, h->voff, h->U,    h->V,    h->rank, h->tmp,  h->toff};
+}
+static bool geom_ok(const mnp_green_geom* g) {
+    return g && g->n > 0 && g->npart > 0 && g->order >= 0 && g->order < 8 && g->pos && g->nvec && g->area &&
+           g->fpart && g->con_in && g->con_out && g->kmat && g->rowptr && g->rcol && g->rg && g->rf;
+}
+
+int mnp_hmat_fill_dense(const mnp_green_geom* g, const int* c2p, int nd, const int* dblk, const long* doff, void* D,
+                        long dstride, void* stream) {
+    REQUIRE(geom_ok(g), 1);
+    REQUIRE(c2p, 2);
+    REQUIRE(nd >= 0, 3);
+    REQUIRE(nd == 0 || (dblk && doff && D), 4);
+    REQUIRE(dstride >= 0, 7);
+    hm_fill_dense(to_geom(g), c2p, nd, dblk, doff, D, dstride, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_hmat_aca(const mnp_green_geom* g, const int* c2p, int kind, int nlr, const int* lblk, const long* uws,
+                 const long* vws, void* U, void* V, int* rank, double htol, int kmax, void* stream) {
+    REQUIRE(geom_ok(g), 1);
+    REQUIRE(c2p, 2);
+    REQUIRE(kind >= 0 && kind < 4, 3);
+    REQUIRE(nlr >= 0, 4);
+    REQUIRE(nlr == 0 || (lblk && uws && vws && U && V && rank), 5);
+    REQUIRE(htol > 0, 11);
+    REQUIRE(kmax >= 1, 12);
+    hm_aca(to_geom(g), c2p, kind, nlr, lblk, uws, vws, U, V, rank, htol, kmax, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_hmat_compact(int nlr, const int* lblk, int kind, const int* rank, const long* uws, const long* vws,
+                     const void* Uws, const void* Vws, const long* uoff, const long* voff, void* U, void* V,
+                     void* stream) {
+    REQUIRE(nlr >= 0, 1);
+    REQUIRE(kind >= 0 && kind < 4, 3);
+    REQUIRE(nlr == 0 || (lblk && rank && uws && vws && Uws && Vws && uoff && voff && U && V), 2);
+    hm_compact(nlr, lblk, kind, rank, uws, vws, Uws, Vws, uoff, voff, U, V, (hipStream_t)stream);
+    return launch_status();
+}
+
+static bool hmat_ok(const mnp_hmat* h) {
+    return h && h->n > 0 && h->nd >= 0 && h->nlr >= 0 && h->c2p && (h->nd == 0 || (h->dblk && h->doff && h->D)) &&
+           (h->nlr == 0 || (h->lblk && h->uoff && h->voff && h->U && h->V && h->rank && h->tmp && h->toff));
+}
+
+int mnp_hmat_matvec(const mnp_hmat* h, int kind, int r, const void* x, long ldx, void* y, long ldy, void* stream) {
+    REQUIRE(hmat_ok(h), 1);
+    REQUIRE(kind >= 0 && kind < 4, 2);
+    REQUIRE(r >= 1 && r <= HM_MAXR, 3);
+    REQUIRE(x, 4);
+    REQUIRE(ldx >= h->n, 5);
+    REQUIRE(y && y != x, 6);
+    REQUIRE(ldy >= h->n, 7);
+    hm_matvec(to_hmat(h), kind, r, x, ldx, y, ldy, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_bem_operator_hmat(const mnp_hmat* h, const double* nvec, const double* L1, const double* L2,
+                          const double* kvac, const void* x, const void* rhs, void* y, void* work, long sW,
+                          double* norms, void* stream) {
+    REQUIRE(hmat_ok(h), 1);
+    REQUIRE(nvec && L1 && L2 && kvac, 2);
+    REQUIRE(x, 6);
+    REQUIRE(y && y != x, 8);
+    REQUIRE(work, 9);
+    REQUIRE(sW >= 64L * h->n, 10);
+    Face f{h->n, nvec, L1, L2, kvac};
+    bem_operator_hmat(to_hmat(h), f, x, rhs, y, work, norms, (hipStream_t)stream);
     return launch_status();
 }
 

@@ -233,9 +233,32 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
     }
 }
 
-// n <= 16384 (64 KiB of LDS holds the permutation)
+// Same for n beyond what LDS holds: the interchanges run in pf itself (global memory).
+__global__ __launch_bounds__(64) void k_perm_global(int n, const int* ipiv, long sP, int* pf, int* qinv, long sQ) {
+    const int b = blockIdx.x, t = threadIdx.x;
+    ipiv += (long)b * sP;
+    pf += (long)b * sQ;
+    qinv += (long)b * sQ;
+    for (int i = t; i < n; i += 64) pf[i] = i;
+    __syncthreads();
+    if (t == 0) {
+        for (int i = 0; i < n; ++i) {
+            int p = ipiv[i];
+            int x = pf[i];
+            pf[i] = pf[p];
+            pf[p] = x;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += 64) qinv[pf[i]] = i;
+}
+
+// the permutation is built in LDS up to 16384 rows (64 KiB), in global memory beyond
 void perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, hipStream_t s) {
-    hipLaunchKernelGGL(k_perm, dim3(batch), dim3(64), (size_t)n * sizeof(int), s, n, ipiv, sP, pf, qinv, sQ);
+    if (n <= 16384)
+        hipLaunchKernelGGL(k_perm, dim3(batch), dim3(64), (size_t)n * sizeof(int), s, n, ipiv, sP, pf, qinv, sQ);
+    else
+        hipLaunchKernelGGL(k_perm_global, dim3(batch), dim3(64), 0, s, n, ipiv, sP, pf, qinv, sQ);
 }
 
 template void getrf_batched<double>(int, int, void*, long, long, int*, long, int*, int, hipStream_t);

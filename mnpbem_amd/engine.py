@@ -75,7 +75,23 @@ class DeviceGeometry:
             self.ref_col = torch.tensor(nf.cols.astype(np.int32), device=d)
             self.ref_g = torch.tensor(np.ascontiguousarray(nf.g), dtype=f64, device=d)
             self.ref_f = torch.tensor(np.ascontiguousarray(nf.f), dtype=f64, device=d)
+        self._nf = nf
+        self._csr = None
         self._side = {}
+
+    def refined_csr(self):
+        """Refined pairs as a CSR with ascending columns per row: (rowptr, col, g, f) on the
+        device, for entry-wise evaluation (H-matrices)."""
+        if self._csr is None:
+            nf, d = self._nf, self.dev
+            order = np.lexsort((nf.cols, nf.rows))
+            rows = nf.rows[order]
+            rowptr = np.searchsorted(rows, np.arange(self.n + 1)).astype(np.int32)
+            self._csr = (torch.tensor(rowptr, device=d),
+                         torch.tensor(nf.cols[order].astype(np.int32), device=d),
+                         torch.tensor(np.ascontiguousarray(nf.g[order]), dtype=torch.float64, device=d),
+                         torch.tensor(np.ascontiguousarray(nf.f[order]), dtype=torch.float64, device=d))
+        return self._csr
 
     def side(self, medium):
         """Per face: bit 0 = inside medium is ``medium``, bit 1 = outside medium is."""

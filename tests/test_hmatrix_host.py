@@ -1,0 +1,121 @@
+"""Host part of the H-matrix path: cluster tree, block partition, ACA restatement (CPU)."""
+import numpy as np
+import pytest
+
+from mnpbem_amd import trisphere, trirod, EpsConst, ComParticle
+from mnpbem_amd.hmatrix import ClusterTree, block_partition, coverage, make_kaware_fadmiss
+from oracle.aca_oracle import aca
+
+
+def _dimer(n=144, gap=2.0, d=10.0):
+    p1 = trisphere(n, d).shift([0, 0, -(d + gap) / 2])
+    p2 = trisphere(n, d).shift([0, 0, (d + gap) / 2])
+    return ComParticle([EpsConst(1.0), EpsConst(-10 + 1j)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+
+
+def test_cluster_tree_invariants():
+    cp = ComParticle([EpsConst(1.0), EpsConst(-10 + 1j)], [trisphere(400, 10)], [[2, 1]], 1)
+    t = ClusterTree(cp.pos, cleaf=16)
+    n = cp.nfaces
+    c2p, p2c = t.ind[:, 0], t.ind[:, 1]
+    assert sorted(c2p) == list(range(n))
+    assert np.all(c2p[p2c] == np.arange(n))
+    for i in range(t.nodes):
+        lo, hi = t.cind[i]
+        pts = cp.pos[c2p[lo:hi + 1]]
+        # bounding sphere holds every centroid of the node
+        assert np.all(np.linalg.norm(pts - t.mid[i], axis=1) <= t.rad[i] * (1 + 1e-12) + 1e-12)
+        if t.is_leaf(i):
+            assert t.size(i) <= 16
+        else:
+            a, b = t.son[i]
+            assert t.cind[a, 0] == lo and t.cind[b, 1] == hi and t.cind[a, 1] + 1 == t.cind[b, 0]
+
+
+def test_dimer_tree_splits_particles_first():
+    cp = _dimer()
+    t = ClusterTree(cp.pos, cleaf=32, ipart=cp.face_particle)
+    a, b = t.son[0]
+    assert t.ipart[0] == -1
+    assert {t.ipart[a], t.ipart[b]} == {0, 1}
+    assert t.size(a) == t.size(b) == cp.nfaces // 2
+
+
+@pytest.mark.parametrize("cleaf,eta", [(16, 2.5), (32, 1.0), (64, 2.5)])
+def test_partition_covers_matrix_once(cleaf, eta):
+    cp = _dimer(256)
+    t = ClusterTree(cp.pos, cleaf=cleaf, ipart=cp.face_particle)
+    fad = make_kaware_fadmiss(2 * np.pi / 600.0, eta)
+    blocks = block_partition(t, fad, eta)
+    c = coverage(t, blocks)
+    assert c.min() == 1 and c.max() == 1
+    for i1, i2, kind in blocks:
+        if kind == 1:
+            assert t.ipart[i1] != -1 and t.ipart[i2] != -1
+            d = np.linalg.norm(t.mid[i1] - t.mid[i2])
+            assert eta * min(t.rad[i1], t.rad[i2]) < d
+        else:
+            assert t.is_leaf(i1) and t.is_leaf(i2)
+    assert any(k == 1 for _, _, k in blocks)
+
+
+def test_partition_size_limit():
+    cp = ComParticle([EpsConst(1.0), EpsConst(-10 + 1j)], [trisphere(400, 10)], [[2, 1]], 1)
+    t = ClusterTree(cp.pos, cleaf=8)
+    full = block_partition(t, eta=2.5)
+    lim = block_partition(t, eta=2.5, max_lowrank=40)
+    assert max(max(t.size(a), t.size(b)) for a, b, k in lim if k == 1) <= 40
+    assert max(max(t.size(a), t.size(b)) for a, b, k in full if k == 1) > 40
+    c = coverage(t, lim)
+    assert c.min() == 1 and c.max() == 1
+
+
+def test_kaware_admissibility():
+    k = 2 * np.pi / 100.0                       # wavelength 100
+    f = make_kaware_fadmiss(k, 2.5)
+    assert f(1.0, 1.0, 3.0)                     # k rmin < 1: geometric rule only
+    assert not f(1.0, 1.0, 2.0)
+    assert not f(20.0, 30.0, 60.0)              # k rmin > 1 and dist < wavelength -> dense
+    assert f(20.0, 30.0, 120.0)
+    f0 = make_kaware_fadmiss(0.0, 2.5)
+    assert f0(20.0, 30.0, 60.0)
+
+
+@pytest.mark.parametrize("rank", [1, 3, 7])
+def test_aca_recovers_low_rank(rank):
+    rng = np.random.default_rng(rank)
+    m, n = 60, 45
+    A = (rng.standard_normal((m, rank)) + 1j * rng.standard_normal((m, rank))) @ \
+        (rng.standard_normal((rank, n)) + 1j * rng.standard_normal((rank, n)))
+    U, V, k = aca(A, htol=1e-10, kmax=50)
+    assert k in (rank, rank + 1)
+    assert np.linalg.norm(U @ V.T - A) / np.linalg.norm(A) < 1e-12
+
+
+@pytest.mark.parametrize("htol", [1e-4, 1e-6, 1e-8])
+def test_aca_green_blocks_accuracy(htol):
+    """Retarded Green function on the admissible blocks of a sphere partition: the ACA error
+    tracks htol (median within 3 htol, worst block within 100 htol) and the rank stays near the
+    numerical rank of the truncated SVD. (Partial pivoting can stop early on unlucky blocks;
+    the reference shares that property, hmatrix.py:195-336.)"""
+    cp = ComParticle([EpsConst(1.0), EpsConst(-10 + 1j)], [trisphere(784, 20)], [[2, 1]], 1)
+    pos, area = cp.pos, cp.area
+    t = ClusterTree(pos, cleaf=32)
+    k = 2 * np.pi / 600.0
+    blocks = [(a, b) for a, b, kind in block_partition(t, make_kaware_fadmiss(k, 2.5), 2.5) if kind == 1]
+    assert len(blocks) > 20
+    errs, excess = [], []
+    for a, b in blocks[::3]:
+        r = t.ind[t.cind[a, 0]:t.cind[a, 1] + 1, 0]
+        c = t.ind[t.cind[b, 0]:t.cind[b, 1] + 1, 0]
+        d = np.linalg.norm(pos[r, None] - pos[None, c], axis=2)
+        A = area[c][None] / d * np.exp(1j * k * d)
+        U, V, rk = aca(A, htol=htol, kmax=100)
+        errs.append(np.linalg.norm(U @ V.T - A) / np.linalg.norm(A))
+        s = np.linalg.svd(A, compute_uv=False)
+        tail = np.sqrt(np.cumsum(s[::-1] ** 2))[::-1] / np.linalg.norm(s)
+        excess.append(rk - int(np.sum(tail > htol)))
+    errs = np.array(errs)
+    assert np.median(errs) < 3 * htol
+    assert errs.max() < 100 * htol
+    assert -1 <= min(excess) and max(excess) <= 6
