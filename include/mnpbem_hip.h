@@ -27,9 +27,10 @@ extern "C" {
 /* ABI revision of this header (1). */
 int mnp_abi_version(void);
 
-/* Test hook: one wave runs v_mfma_f64_16x16x4f64 and v_mfma_f32_16x16x4f32 on known operands
- * and writes the 64 x 4 result registers of each to out[0..255] and out[256..511] (device
- * doubles). No counterpart in the reference. */
+/* Test hook: one wave runs v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32 and
+ * v_mfma_f32_32x32x2f32 on known operands and writes their result registers to out[0..255],
+ * out[256..511] (64 x 4 each) and out[512..1535] (64 x 16) (device doubles). No counterpart in
+ * the reference. */
 int mnp_probe_mfma_layout(double* out, void* stream);
 
 /* GEMM timing hooks for roofline reports (no counterpart in the reference): while enabled,

@@ -147,6 +147,7 @@ class BEMRetIter:
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
         p._mnp_geo = self.g.geo
         self.enei = None
+        self.timings = {}     # seconds of the last H-matrix build, preconditioner build and GMRES run
         self._flag = self._relres = self._iter = None
         self._hist = None
         if enei is not None:
@@ -164,7 +165,9 @@ class BEMRetIter:
                 self._tree = ClusterTree(self.p.pos, self.cleaf, self.p.face_particle)
             self._hm = None
             torch.cuda.empty_cache()
+            t0 = _now(geo.dev)
             self._hm = HMatrixRet(geo, self._tree, enei, self.htol, self.kmax, self.eta, self.fadmiss)
+            self.timings["hmatrix"] = _now(geo.dev) - t0
             self._opwork = torch.empty((64, geo.n), dtype=torch.complex128, device=geo.dev)
         else:
             self._exact = engine.GreenBatch(geo, 1, torch.complex128)
@@ -175,7 +178,9 @@ class BEMRetIter:
             prec = "mixed" if self.precond_precision == "single" else "double"
             # the preconditioner's factor precision; refinement is done by GMRES itself
             self._pre = engine.RetSolverBatch(geo, 1, "single" if prec == "mixed" else "double")
+            t0 = _now(geo.dev)
             self._pre.factor([enei])
+            self.timings["precond"] = _now(geo.dev) - t0
         self.enei = enei
 
     def _apply_A(self, x, y, rhs=None):
@@ -219,7 +224,9 @@ class BEMRetIter:
             if x0 is not None:
                 xg = torch.tensor(np.asarray(_packed(x0, ip, n)).ravel(), dtype=torch.complex128,
                                   device=self.g.geo.dev)
+            t0 = _now(self.g.geo.dev)
             x, flag, relres, it, hist = gm.solve(self._apply_A, self._apply_M, b, xg, self.tol, self.maxit)
+            self.timings["gmres"] = _now(self.g.geo.dev) - t0
             out[ip] = x.view(8, n).cpu().numpy()
             infos.append((flag, relres, it, hist))
         self._flag, self._relres, self._iter, self._hist = infos[-1]
@@ -241,6 +248,12 @@ class BEMRetIter:
     @property
     def history(self):
         return self._hist
+
+
+def _now(dev):
+    import time
+    torch.cuda.synchronize(dev)
+    return time.perf_counter()
 
 
 def _packed(x0, ipol, n):

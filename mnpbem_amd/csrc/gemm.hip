@@ -1,19 +1,16 @@
 // Batched complex GEMM update C -= A * B on the matrix cores.
 //
 // Column-major, interleaved complex operands. A workgroup of 256 lanes (4 waves) owns a
-// 64x64 tile of C; each wave a 32x32 quarter built from 2x2 MFMA 16x16 tiles. The K loop
-// stages 16-deep slices of A and B through LDS split into real/imaginary planes (A stored
-// negated, so the MFMA accumulation directly yields -A*B). The complex product uses four
-// real MFMAs per 16x16x4 step (re*re, -im*im, re*im, im*re).
+// 64x64 tile of C, each wave a 32x32 quarter: one v_mfma_f32_32x32x2f32 tile (complex float)
+// or 2x2 v_mfma_f64_16x16x4f64 tiles (complex double). K slices of 16 are staged through LDS
+// split into real/imaginary planes (A stored negated, so the accumulation yields -A*B), and
+// the next slice is prefetched into registers while the current one is multiplied. A complex
+// product takes four real MFMAs (re*re, -im*im, re*im, im*re).
 //
 // The MFMA is fed (B^T, A^T) rather than (A, B): its result row index then runs over C's
-// columns and its lane index over C's rows, so each group of 16 lanes writes 16 consecutive
-// rows of one column (a contiguous 256 B / 128 B segment in column-major C).
-//
-// Lane layout of v_mfma_*_16x16x4 (64 lanes, l): operand-A lane l holds A[l%16][l/16],
-// operand-B lane l holds B[l/16][l%16]; result register r of lane l holds D[row][l%16] with
-// row = l/16 + 4 r for the f64 instruction and row = 4 (l/16) + r for the f32 one (measured
-// on gfx950 by k_probe_layout; Mfma<T>::drow encodes it).
+// columns and its lane index over C's rows, so each lane group writes consecutive rows of one
+// column (a contiguous segment in column-major C). Lane layouts are in Mfma<T> below
+// (measured on gfx950 by k_probe_layout).
 #include "common.hpp"
 
 #include <vector>
@@ -61,11 +58,19 @@ void prof_collect(double* ms, double* flops, double* launches) {
     g_prof.clear();
 }
 
+// Matrix-core tile of one wave: 32 x 32 of C.
+//   float : one v_mfma_f32_32x32x2f32 per real product, K step 2; operand a lane l holds
+//           P[l%32][l/32], operand b lane l holds Q[l/32][l%32]; result register r of lane l
+//           holds D[(r%4) + 8(r/4) + 4(l/32)][l%32] (measured on gfx950).
+//   double: 2 x 2 v_mfma_f64_16x16x4f64, K step 4; operand a lane l holds P[l%16][l/16],
+//           operand b lane l holds Q[l/16][l%16]; result register r of lane l holds
+//           D[l/16 + 4r][l%16] (measured on gfx950 by k_probe_layout).
 template <typename T>
 struct Mfma;
 template <>
 struct Mfma<double> {
     typedef double acc_t __attribute__((ext_vector_type(4)));
+    static constexpr int KS = 4, TILE = 16, NT = 2, NR = 4;
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -73,20 +78,29 @@ struct Mfma<double> {
 };
 template <>
 struct Mfma<float> {
-    typedef float acc_t __attribute__((ext_vector_type(4)));
+    typedef float acc_t __attribute__((ext_vector_type(16)));
+    static constexpr int KS = 2, TILE = 32, NT = 1, NR = 16;
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+    static __device__ __forceinline__ int drow(int lane, int r) { return (r & 3) + 8 * (r >> 2) + 4 * (lane >> 5); }
 };
 
 constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 4;
+constexpr int GNA = GBM * GBK / 256, GNB = GBK * GBN / 256;   // complex loads per lane per K slice
 
+// C -= A B: 256 lanes (4 waves, 2 x 2) per 64 x 64 tile of C; K slices of 16 staged through
+// LDS as real/imaginary planes (A negated, so accumulation yields -A B directly); the next
+// slice is loaded into registers while the current one is multiplied. Each complex product
+// takes four real MFMAs. The MFMA is fed (B^T, A^T) so that its lane index runs over C's rows
+// and the stores of a lane group are contiguous in column-major C.
 template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cx<T>* __restrict__ A, long lda,
                                                   long sA, const cx<T>* __restrict__ B, long ldb, long sB,
                                                   cx<T>* __restrict__ C, long ldc, long sC) {
-    using acc_t = typename Mfma<T>::acc_t;
+    using M = Mfma<T>;
+    using acc_t = typename M::acc_t;
+    constexpr int TL = M::TILE, NT = M::NT, KS = M::KS;
     __shared__ T As_re[GBK][GBM + GPAD], As_im[GBK][GBM + GPAD];
     __shared__ T Bs_re[GBK][GBN + GPAD], Bs_im[GBK][GBN + GPAD];
     const int bz = blockIdx.z;
@@ -96,71 +110,79 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cx<
     const int row0 = blockIdx.x * GBM, col0 = blockIdx.y * GBN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-    acc_t cre[2][2], cim[2][2];
+    acc_t cre[NT][NT], cim[NT][NT];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NT; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NT; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
-
-    for (int k0 = 0; k0 < k; k0 += GBK) {
-        // A slice: 64 rows x 16 k, thread -> row t%64, k = t/64 + 4q
+            for (int r = 0; r < M::NR; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
+    cx<T> pa[GNA], pb[GNB];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int ii = t & 63, kk = (t >> 6) + 4 * q;
-            int gi = row0 + ii, gk = k0 + kk;
-            cx<T> v = mk<T>(0, 0);
-            if (gi < m && gk < k) v = A[gi + (long)gk * lda];
-            As_re[kk][ii] = -v.re;
-            As_im[kk][ii] = -v.im;
+        for (int q = 0; q < GNA; ++q) {
+            const int idx = t + 256 * q, ii = idx % GBM, kk = idx / GBM;
+            const int gi = row0 + ii, gk = k0 + kk;
+            pa[q] = (gi < m && gk < k) ? A[gi + (long)gk * lda] : mk<T>(0, 0);
         }
-        // B slice: 16 k x 64 cols, thread -> k = t%16, col = t/16 + 16q
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            int kk = t & 15, jj = (t >> 4) + 16 * q;
-            int gk = k0 + kk, gj = col0 + jj;
-            cx<T> v = mk<T>(0, 0);
-            if (gk < k && gj < n) v = B[gk + (long)gj * ldb];
-            Bs_re[kk][jj] = v.re;
-            Bs_im[kk][jj] = v.im;
+        for (int q = 0; q < GNB; ++q) {
+            const int idx = t + 256 * q, kk = idx % GBK, jj = idx / GBK;
+            const int gk = k0 + kk, gj = col0 + jj;
+            pb[q] = (gk < k && gj < n) ? B[gk + (long)gj * ldb] : mk<T>(0, 0);
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < k; k0 += GBK) {
+#pragma unroll
+        for (int q = 0; q < GNA; ++q) {
+            const int idx = t + 256 * q, ii = idx % GBM, kk = idx / GBM;
+            As_re[kk][ii] = -pa[q].re;
+            As_im[kk][ii] = -pa[q].im;
+        }
+#pragma unroll
+        for (int q = 0; q < GNB; ++q) {
+            const int idx = t + 256 * q, kk = idx % GBK, jj = idx / GBK;
+            Bs_re[kk][jj] = pb[q].re;
+            Bs_im[kk][jj] = pb[q].im;
         }
         __syncthreads();
+        if (k0 + GBK < k) load(k0 + GBK);
 #pragma unroll
-        for (int ks = 0; ks < GBK; ks += 4) {
-            int kk = ks + (lane >> 4);
-            T are[2], aim[2], bre[2], bim[2];
+        for (int ks = 0; ks < GBK; ks += KS) {
+            const int kk = ks + lane / TL;
+            T are[NT], aim[NT], bre[NT], bim[NT];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                are[a] = As_re[kk][wm + a * 16 + (lane & 15)];
-                aim[a] = As_im[kk][wm + a * 16 + (lane & 15)];
+            for (int a = 0; a < NT; ++a) {
+                are[a] = As_re[kk][wm + a * TL + lane % TL];
+                aim[a] = As_im[kk][wm + a * TL + lane % TL];
             }
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                bre[b] = Bs_re[kk][wn + b * 16 + (lane & 15)];
-                bim[b] = Bs_im[kk][wn + b * 16 + (lane & 15)];
+            for (int b = 0; b < NT; ++b) {
+                bre[b] = Bs_re[kk][wn + b * TL + lane % TL];
+                bim[b] = Bs_im[kk][wn + b * TL + lane % TL];
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < NT; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
+                for (int b = 0; b < NT; ++b) {
                     // D[j][i] = sum_k B[k][j] * A[i][k]; A already negated
-                    cre[a][b] = Mfma<T>::op(bre[b], are[a], cre[a][b]);
-                    cre[a][b] = Mfma<T>::op(-bim[b], aim[a], cre[a][b]);
-                    cim[a][b] = Mfma<T>::op(bim[b], are[a], cim[a][b]);
-                    cim[a][b] = Mfma<T>::op(bre[b], aim[a], cim[a][b]);
+                    cre[a][b] = M::op(bre[b], are[a], cre[a][b]);
+                    cim[a][b] = M::op(bim[b], are[a], cim[a][b]);
+                    cre[a][b] = M::op(-bim[b], aim[a], cre[a][b]);
+                    cim[a][b] = M::op(bre[b], aim[a], cim[a][b]);
                 }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NT; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NT; ++b)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int gi = row0 + wm + a * 16 + (lane & 15);
-                int gj = col0 + wn + b * 16 + Mfma<T>::drow(lane, r);
+            for (int r = 0; r < M::NR; ++r) {
+                const int gi = row0 + wm + a * TL + lane % TL;
+                const int gj = col0 + wn + b * TL + M::drow(lane, r);
                 if (gi < m && gj < n) {
                     cx<T>* c = C + gi + (long)gj * ldc;
                     cx<T> v = *c;
@@ -198,9 +220,10 @@ template void gemm_sub<float>(int, int, int, const void*, long, long, const void
                               int, hipStream_t);
 
 // ---------------------------------------------------------------------------------------------
-// Lane-layout probe: D = A(16x4) * B(4x16) with A[i][k] = i + 16 k and B[k][j] = j + 1 for
-// k = 1, 0 otherwise, so D[i][j] = (i + 16)(j + 1). One wave per instruction; out[0..255] =
-// f64 result registers (lane * 4 + r), out[256..511] = f32 result registers.
+// Lane-layout probe: D = P * Q with P[x][k] = x + 16 k (16x16x4) or x + 16 k (32x32x2) and
+// Q[k][y] = y + 1 for k = 1, 0 otherwise, so D[x][y] = (x + 16)(y + 1). One wave per
+// instruction; out[0..255] = f64 16x16x4 result registers (lane * 4 + r), out[256..511] =
+// f32 16x16x4, out[512..1535] = f32 32x32x2 (lane * 16 + r).
 __global__ void k_probe_layout(double* out) {
     int l = threadIdx.x;
     int i = l & 15, kk = l >> 4;
@@ -219,6 +242,16 @@ __global__ void k_probe_layout(double* out) {
         acc_t c = {0, 0, 0, 0};
         c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
         for (int r = 0; r < 4; ++r) out[256 + l * 4 + r] = (double)c[r];
+    }
+    {
+        typedef float acc_t __attribute__((ext_vector_type(16)));
+        const int x = l & 31, k2 = l >> 5;
+        float a = (float)(x + 16 * k2);
+        float b = (k2 == 1) ? (float)(x + 1) : 0.0f;
+        acc_t c;
+        for (int r = 0; r < 16; ++r) c[r] = 0;
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+        for (int r = 0; r < 16; ++r) out[512 + l * 16 + r] = (double)c[r];
     }
 }
 

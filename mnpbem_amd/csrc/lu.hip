@@ -13,6 +13,8 @@
 // (P b = b[pf]) and its inverse, so the solves gather instead of swapping sequentially.
 #include "common.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 namespace mnp {
 
 template <typename T>
@@ -91,6 +93,167 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(int n, int j0, int nb, 
         }
         __syncthreads();
     }
+}
+
+// Panel factorisation spread over the whole device (cooperative launch, grid G x batch).
+// Each thread holds one row of the (n - j0) x w panel (w <= 32) in registers, 256 rows per
+// workgroup. Per column: every workgroup publishes its best pivot candidate (value, row index
+// and the candidate row itself) and the workgroup holding the diagonal row publishes that row;
+// after one grid barrier every workgroup picks the same pivot (largest |re|+|im|, smallest
+// row on ties), the two rows trade places and all rows below the diagonal are eliminated.
+// Published data alternate between two buffers by column parity, so one barrier per column
+// suffices. Same pivots and arithmetic order per element as k_panel.
+constexpr int CP_T = 256, CP_W = 32, CP_MIN_ROWS = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(CP_T) void k_panel_coop(int n, int j0, int w, cx<T>* A, long lda, long sA, int* ipiv,
+                                                     long sP, int* info, T* cv, int* ci, cx<T>* crow, cx<T>* drow) {
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    __shared__ T red_v[CP_T / 64];
+    __shared__ int red_i[CP_T / 64];
+    __shared__ T gv[CP_T];
+    __shared__ int gi[CP_T];
+    __shared__ cx<T> urow[CP_W];
+    __shared__ int piv_s, gp_s;
+    const int G = gridDim.x, g = blockIdx.x, b = blockIdx.y, t = threadIdx.x;
+    const int batch = gridDim.y;
+    A += (long)b * sA;
+    ipiv += (long)b * sP;
+    const int row = j0 + g * CP_T + t;
+    const bool have = row < n;
+    cx<T> x[CP_W];
+#pragma unroll
+    for (int c = 0; c < CP_W; ++c) x[c] = (have && c < w) ? A[row + (long)(j0 + c) * lda] : mk<T>(0, 0);
+#pragma clang loop unroll(full)
+    for (int c = 0; c < CP_W; ++c) {
+        if (c < w) {
+            const int col = j0 + c, par = c & 1;
+            const long slot = ((long)par * batch + b) * G + g;
+            // local candidate
+            T best = (have && row >= col) ? cabs1(x[c]) : (T)-1;
+            int bi = (have && row >= col) ? row : n;
+            for (int off = 32; off > 0; off >>= 1) {
+                T ov = __shfl_xor(best, off, 64);
+                int oi = __shfl_xor(bi, off, 64);
+                if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+            }
+            if ((t & 63) == 0) { red_v[t >> 6] = best; red_i[t >> 6] = bi; }
+            __syncthreads();
+            T bv = red_v[0];
+            int bx = red_i[0];
+#pragma unroll
+            for (int q = 1; q < CP_T / 64; ++q)
+                if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) { bv = red_v[q]; bx = red_i[q]; }
+            if (t == 0) { cv[slot] = bv; ci[slot] = bx; }
+            if (have && row == bx)
+#pragma unroll
+                for (int u = 0; u < CP_W; ++u)
+                    if (u < w) crow[slot * CP_W + u] = x[u];
+            if (have && row == col)
+#pragma unroll
+                for (int u = 0; u < CP_W; ++u)
+                    if (u < w) drow[((long)par * batch + b) * CP_W + u] = x[u];
+            grid.sync();
+            // global pivot: same choice in every workgroup
+            const long base = ((long)par * batch + b) * G;
+            T v2 = (T)-1;
+            int i2 = n, g2 = 0;
+            for (int q = t; q < G; q += CP_T) {
+                T qv = cv[base + q];
+                int qi = ci[base + q];
+                if (qv > v2 || (qv == v2 && qi < i2)) { v2 = qv; i2 = qi; g2 = q; }
+            }
+            gv[t] = v2;
+            gi[t] = i2;
+            __syncthreads();
+            if (t == 0) {
+                T bv2 = gv[0];
+                int bi2 = gi[0], bg = 0;
+                const int lim = G < CP_T ? G : CP_T;
+                for (int q = 1; q < lim; ++q)
+                    if (gv[q] > bv2 || (gv[q] == bv2 && gi[q] < bi2)) { bv2 = gv[q]; bi2 = gi[q]; bg = q; }
+                // the winning entry's workgroup: entries q, q + CP_T, ... were reduced into slot q
+                int wg = bg;
+                for (int q = bg; q < G; q += CP_T)
+                    if (cv[base + q] == bv2 && ci[base + q] == bi2) { wg = q; break; }
+                piv_s = bi2;
+                gp_s = wg;
+            }
+            (void)g2;
+            __syncthreads();
+            const int p = piv_s;
+            if (t < w) urow[t] = crow[(base + gp_s) * CP_W + t];
+            __syncthreads();
+            if (g == 0 && t == 0) ipiv[col] = p;
+            // trade rows col and p
+            if (have && row == p && p != col)
+#pragma unroll
+                for (int u = 0; u < CP_W; ++u)
+                    if (u < w) x[u] = drow[((long)par * batch + b) * CP_W + u];
+            if (have && row == col)
+#pragma unroll
+                for (int u = 0; u < CP_W; ++u)
+                    if (u < w) x[u] = urow[u];
+            const cx<T> piv = urow[c];
+            const bool zero = piv.re == 0 && piv.im == 0;
+            if (zero && g == 0 && t == 0 && info[b] == 0) info[b] = col + 1;
+            if (!zero && have && row > col) {
+                const cx<T> l = x[c] * crecip(piv);
+                x[c] = l;
+#pragma unroll
+                for (int u = c + 1; u < CP_W; ++u)
+                    if (u < w) x[u] = fms(x[u], l, urow[u]);
+            }
+        }
+    }
+    if (have)
+#pragma unroll
+        for (int c = 0; c < CP_W; ++c)
+            if (c < w) A[row + (long)(j0 + c) * lda] = x[c];
+}
+
+// Scratch of the cooperative panel (grows on demand, kept for the process).
+static void* g_coop_ws = nullptr;
+static size_t g_coop_bytes = 0;
+
+template <typename T>
+static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+                       hipStream_t s) {
+    static int capacity = -1;
+    if (capacity < 0) {
+        int dev = 0, per_cu = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_panel_coop<T>, CP_T, 0) != hipSuccess) per_cu = 0;
+        capacity = per_cu * cus;
+    }
+    const int G = (n - j0 + CP_T - 1) / CP_T;
+    // one grid barrier per column costs more than a single workgroup walking a short panel
+    if (n - j0 < CP_MIN_ROWS || w > CP_W || (long)G * batch > capacity) return false;
+    const size_t ncand = 2 * (size_t)batch * G;
+    const size_t bytes = ncand * (sizeof(T) + sizeof(int) + CP_W * sizeof(cx<T>)) + 2 * (size_t)batch * CP_W *
+                         sizeof(cx<T>) + 256;
+    if (bytes > g_coop_bytes) {
+        if (g_coop_ws) {
+            hipStreamSynchronize(s);
+            hipFree(g_coop_ws);
+        }
+        if (hipMalloc(&g_coop_ws, bytes) != hipSuccess) {
+            g_coop_ws = nullptr;
+            g_coop_bytes = 0;
+            return false;
+        }
+        g_coop_bytes = bytes;
+    }
+    char* base = (char*)g_coop_ws;
+    cx<T>* crow = (cx<T>*)base;
+    cx<T>* drow = crow + ncand * CP_W;
+    T* cv = (T*)(drow + 2 * (size_t)batch * CP_W);
+    int* ci = (int*)(cv + ncand);
+    void* args[] = {&n, &j0, &w, &a, &lda, &sA, &ipiv, &sP, &info, &cv, &ci, &crow, &drow};
+    return hipLaunchCooperativeKernel((const void*)k_panel_coop<T>, dim3(G, batch), dim3(CP_T), args, 0, s) ==
+           hipSuccess;
 }
 
 // apply interchanges ipiv[r0 .. r1) to the columns [a0, a1) and [b0, b1)
@@ -205,8 +368,9 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
         // factor the tall outer panel A[J0:n, J0:JE]
         for (int j0 = J0; j0 < JE; j0 += nb) {
             int w = (JE - j0 < nb) ? (JE - j0) : nb;
-            hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv, sP,
-                               info);
+            if (!panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, s))
+                hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv,
+                                   sP, info);
             laswp<T>(j0, j0 + w, J0, j0, j0 + w, JE, a, lda, sA, ipiv, sP, batch, s);
             int rest = JE - j0 - w;
             if (rest > 0) {

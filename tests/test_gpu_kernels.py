@@ -41,15 +41,22 @@ def _rand(rng, *shape):
 
 def test_mfma_layout_probe():
     banner("mfma layout probe")
-    out = torch.zeros(512, dtype=torch.float64, device=_dev())
+    out = torch.zeros(1536, dtype=torch.float64, device=_dev())
     _lib().call("mnp_probe_mfma_layout", out.data_ptr(), _sh())
-    got = out.cpu().numpy().reshape(2, 64, 4)
+    o = out.cpu().numpy()
+    got = o[:512].reshape(2, 64, 4)
     l = np.arange(64)[:, None]
     r = np.arange(4)[None, :]
     # D[i][j] = (i + 16)(j + 1); register r of lane l holds row l/16 + 4r (f64) and
-    # 4(l/16) + r (f32), column l % 16 -- the mapping the GEMM epilogue is written for
+    # 4(l/16) + r (f32), column l % 16
     assert np.array_equal(got[0], (l // 16 + 4 * r + 16) * ((l % 16) + 1))
     assert np.array_equal(got[1], (4 * (l // 16) + r + 16) * ((l % 16) + 1))
+    # 32x32x2: register r of lane l holds row (r%4) + 8(r/4) + 4(l/32), column l % 32 -- the
+    # mapping the complex-float GEMM epilogue is written for
+    g32 = o[512:].reshape(64, 16)
+    r16 = np.arange(16)[None, :]
+    rows = (r16 % 4) + 8 * (r16 // 4) + 4 * (l // 32)
+    assert np.array_equal(g32, (rows + 16) * ((l % 32) + 1))
 
 
 @pytest.mark.parametrize("prec", [0, 1])
@@ -155,3 +162,49 @@ def test_solves(prec):
         assert np.max(np.abs(_np(tX, b) - X)) / np.max(np.abs(X)) < tol
         y = A[b] @ Rhs[b]
         assert np.max(np.abs(_np(ty, b) - y)) / np.max(np.abs(y)) < (1e-13 if prec == 0 else 1e-5)
+
+
+def test_getrf_multiworkgroup_panel_matches_lapack():
+    """Panels taller than one workgroup (cooperative launch): pivots and factors as LAPACK."""
+    banner("getrf n=4700 batch 2 (panel over several workgroups)")
+    rng = np.random.default_rng(11)
+    n, B = 4700, 2
+    A = [_rand(rng, n, n) for _ in range(B)]
+    t = _cm(A, torch.complex128)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    _lib().call("mnp_getrf_batched", 0, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    assert np.all(info.cpu().numpy() == 0)
+    for b in range(B):
+        lu, piv = sla.lu_factor(A[b])
+        assert np.array_equal(ipiv[b].cpu().numpy(), piv)
+        assert np.max(np.abs(_np(t, b) - lu)) < 1e-10 * np.max(np.abs(lu))
+
+
+def test_large_single_matrix_factor_and_solve():
+    """n = 17000 complex64, one matrix: multi-workgroup panels and the permutation beyond
+    16384 rows; normwise backward error of the solve."""
+    banner("getrf/getrs n=17000 complex64")
+    n = 17000
+    g = torch.Generator(device=_dev()).manual_seed(3)
+    A = torch.randn((n, n), dtype=torch.complex64, device=_dev(), generator=g)   # column-major = A^T
+    b = torch.randn((1, n), dtype=torch.complex64, device=_dev(), generator=g)
+    LU = A.clone()
+    ipiv = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(1, dtype=torch.int32, device=_dev())
+    L = _lib()
+    L.call("mnp_getrf_batched", 1, n, 1, LU.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    pf = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    q = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    L.call("mnp_perm_batched", n, 1, ipiv.data_ptr(), n, pf.data_ptr(), q.data_ptr(), n, _sh())
+    x = torch.zeros_like(b)
+    L.call("mnp_getrs_batched", 1, n, 1, 1, LU.data_ptr(), n, n * n, pf.data_ptr(), n, b.data_ptr(), n, n,
+           x.data_ptr(), n, n, _sh())
+    assert int(info.item()) == 0
+    p = pf[0].long()
+    assert torch.equal(torch.sort(p).values, torch.arange(n, device=_dev()))
+    # the stored array is M^T with M the factored matrix: M x = b  <=>  x^T A = b^T
+    r = (x @ A) - b
+    berr = float(torch.linalg.vector_norm(r) / (torch.linalg.matrix_norm(A) * torch.linalg.vector_norm(x)))
+    print("backward error", berr)
+    assert berr < 2e-6
