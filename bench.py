@@ -71,6 +71,110 @@ def cpu_baseline(cp, nsample):
                        "{:.1f} s; geometry set-up excluded)".format(len(lams), s.n, dt))
 
 
+def gmres25k():
+    """Rod dimer (two 20x80 nm capped rods end to end, 2 nm gap, ~24.5k faces), gold Drude in
+    vacuum, 750 nm, axial polarisation: BEMRetIter with H-matrices (eta 2.5, leaf 64, ACA tol
+    1e-6, kmax 100, low-rank factors complex64) and GMRES(50) to 1e-6. time_to_tol = H-matrix
+    build + preconditioner + GMRES; the host geometry set-up is reported separately."""
+    import torch
+    import mnpbem_amd as M
+    from mnpbem_amd.bem_iter import BEMRetIter
+    t0 = time.perf_counter()
+    p1 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, -41.0])
+    p2 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, 41.0])
+    cp = M.ComParticle([M.EpsConst(1.0), M.EpsDrude(10, 9.065, 0.0708)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+    g = M.CompGreenRet(cp, cp)
+    g.geo
+    exc = M.PlaneWaveRet([0, 0, 1.0], [1.0, 0, 0])
+    e = exc(cp, 750.0)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    it = BEMRetIter(cp, tol=1e-6, maxit=500, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5,
+                    green=g)
+    t0 = time.perf_counter()
+    sig, _ = it.solve(e)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    flag, relres, niter = it.info()
+    st = it._hm.stats
+    out = dict(workload="BEMRetIter gold-Drude rod dimer 2x(20x80 nm), 2 nm gap, n={}, 750 nm, axial; H-matrix "
+                        "eta 2.5 leaf 64 htol 1e-6; GMRES(50) to 1e-6, BEMRet-chain preconditioner "
+                        "(complex64)".format(cp.nfaces),
+               faces=cp.nfaces, time_to_tol_s=round(total, 3), hmatrix_build_s=round(it.timings.get("hmatrix", 0), 3),
+               precond_s=round(it.timings.get("precond", 0), 3), gmres_s=round(it.timings.get("gmres", 0), 3),
+               iterations=int(niter), relres=float(relres), converged=flag == 0, ext=float(exc.extinction(sig)),
+               compression=[round(c, 4) for c in st["compression"]], rank_max=st.get("rank_max"),
+               bytes_per_operator_apply=st["bytes_per_apply"], host_setup_s=round(setup, 3))
+    del it, g
+    torch.cuda.empty_cache()
+    return out
+
+
+def particles64(seed=20261203):
+    """64 seeded particles of uneven size: spheres, prolate ellipsoids (aspect 1-4) and sphere
+    gap dimers (gap 1-5 nm), target face counts uniform in 300-3000 (nearest tabulated sphere
+    point set; a dimer splits the count over its two spheres), gold Drude in eps 1.33."""
+    import mnpbem_amd as M
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(64):
+        kind = int(rng.integers(3))
+        nf = int(rng.integers(300, 3001))
+        drude = M.EpsDrude(10, 9.065, 0.0708)
+        if kind == 0:
+            cp = M.ComParticle([M.EpsConst(1.33), drude], [M.trisphere(nf // 2 + 2, 20)], [[2, 1]], 1)
+        elif kind == 1:
+            asp = float(rng.uniform(1, 4))
+            cp = M.ComParticle([M.EpsConst(1.33), drude], [M.triellipsoid(nf // 2 + 2, [8, 8, 8 * asp])], [[2, 1]], 1)
+        else:
+            gap = float(rng.uniform(1, 5))
+            s1 = M.trisphere(nf // 4 + 2, 20).shift([0, 0, -(10 + gap / 2)])
+            s2 = M.trisphere(nf // 4 + 2, 20).shift([0, 0, 10 + gap / 2])
+            cp = M.ComParticle([M.EpsConst(1.33), drude], [s1, s2], [[2, 1], [2, 1]], 1, 2)
+        out.append(cp)
+    return out
+
+
+def batched64():
+    """Each particle's 32 wavelengths (450-900 nm) as one batched mixed-precision sweep (x-polarised
+    plane wave along z): solves/s over all 2048 solves and p50/p99 of the per-solve time
+    (particle sweep time / 32); geometry set-up excluded."""
+    import torch
+    import mnpbem_amd as M
+    t0 = time.perf_counter()
+    parts = particles64()
+    greens = [M.CompGreenRet(cp, cp) for cp in parts]
+    for g in greens:
+        g.geo
+    setup = time.perf_counter() - t0
+    lams = np.linspace(450, 900, 32)
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    per = []
+    faces = []
+    fallbacks = 0
+    # first particle once untimed (library load, first allocations)
+    M.BEMRet(parts[0], batch=32, precision="mixed", green=greens[0]).solve_spectrum(exc, lams)
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for cp, g in zip(parts, greens):
+        t1 = time.perf_counter()
+        out = M.BEMRet(cp, batch=32, precision="mixed", green=g).solve_spectrum(exc, lams)
+        torch.cuda.synchronize()
+        per.append((time.perf_counter() - t1) / len(lams))
+        faces.append(cp.nfaces)
+        fallbacks += len(out.get("fallback", []))
+        torch.cuda.empty_cache()
+    total = time.perf_counter() - t_all
+    per = np.array(per)
+    return dict(workload="64 seeded particles (spheres, prolate ellipsoids, gap dimers), faces {}-{}, gold Drude in "
+                         "eps 1.33, 32 wavelengths 450-900 nm each, batch 32, mixed precision".format(min(faces),
+                                                                                                   max(faces)),
+                solves=len(parts) * len(lams), solves_per_s=round(len(parts) * len(lams) / total, 3),
+                per_solve_ms_p50=round(1e3 * float(np.percentile(per, 50)), 3),
+                per_solve_ms_p99=round(1e3 * float(np.percentile(per, 99)), 3),
+                fp64_fallbacks=fallbacks, host_setup_s=round(setup, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +185,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--no-gmres25k", action="store_true")
+    ap.add_argument("--no-batched64", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this pool has one MI355X and the wavelength sweep "
@@ -168,6 +274,12 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
+    solvers.clear()
+    torch.cuda.empty_cache()
+    if not args.no_batched64:
+        line["batched64"] = batched64()
+    if not args.no_gmres25k:
+        line["gmres25k"] = gmres25k()
     print(json.dumps(line), flush=True)
 
 

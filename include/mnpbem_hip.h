@@ -60,7 +60,8 @@ int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda,
                          long sB, void* C, long ldc, long sC, int batch, void* stream);
 
 /* In-place LU with partial pivoting of `batch` n x n matrices (panel width nb, 1..64).
- * Replaces lu_factor_dispatch (utils/gpu.py:145-191: scipy zgetrf / cupyx getrf). */
+ * info[b]: 0 ok, j > 0 zero pivot in column j-1, -1 a multi-workgroup panel's barrier timed out
+ * (factors invalid). Replaces lu_factor_dispatch (utils/gpu.py:145-191: scipy zgetrf / cupyx getrf). */
 int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                       void* stream);
 

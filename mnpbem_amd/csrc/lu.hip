@@ -13,7 +13,8 @@
 // (P b = b[pf]) and its inverse, so the solves gather instead of swapping sequentially.
 #include "common.hpp"
 
-#include <hip/hip_cooperative_groups.h>
+#include <stdint.h>
+#include <stdlib.h>
 
 namespace mnp {
 
@@ -99,17 +100,40 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(int n, int j0, int nb, 
 // Each thread holds one row of the (n - j0) x w panel (w <= 32) in registers, 256 rows per
 // workgroup. Per column: every workgroup publishes its best pivot candidate (value, row index
 // and the candidate row itself) and the workgroup holding the diagonal row publishes that row;
-// after one grid barrier every workgroup picks the same pivot (largest |re|+|im|, smallest
-// row on ties), the two rows trade places and all rows below the diagonal are eliminated.
-// Published data alternate between two buffers by column parity, so one barrier per column
-// suffices. Same pivots and arithmetic order per element as k_panel.
-constexpr int CP_T = 256, CP_W = 32, CP_MIN_ROWS = 4096;
+// after one barrier among the matrix's workgroups every workgroup picks the same pivot
+// (largest |re|+|im|, smallest row on ties), the two rows trade places and all rows below the
+// diagonal are eliminated. Published data alternate between two buffers by column parity, so
+// one barrier per column suffices. Same pivots and arithmetic order per element as k_panel.
+//
+// The barrier is a counter per (matrix, panel) in global memory (zeroed when the factorisation
+// starts): each workgroup adds 1 with release semantics and waits (acquire) until all G have
+// arrived for the column. The cooperative launch guarantees that all workgroups are resident.
+// The wait is bounded: after CP_SPIN_LIMIT polls it gives up and flags info[b] = -1.
+constexpr int CP_T = 256, CP_W = 32;
+constexpr long CP_SPIN_LIMIT = 1L << 22;
+static int g_coop_min_rows = -1;   // tall-panel threshold (rows below the diagonal block)
+
+__device__ __forceinline__ void matrix_barrier(unsigned* cnt, unsigned target, int* flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        long spins = 0;
+        while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > CP_SPIN_LIMIT) {
+                *flag = -1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
 
 template <typename T>
 __global__ __launch_bounds__(CP_T) void k_panel_coop(int n, int j0, int w, cx<T>* A, long lda, long sA, int* ipiv,
-                                                     long sP, int* info, T* cv, int* ci, cx<T>* crow, cx<T>* drow) {
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
+                                                     long sP, int* info, T* cv, int* ci, cx<T>* crow, cx<T>* drow,
+                                                     unsigned* cnt) {
     __shared__ T red_v[CP_T / 64];
     __shared__ int red_i[CP_T / 64];
     __shared__ T gv[CP_T];
@@ -154,7 +178,7 @@ __global__ __launch_bounds__(CP_T) void k_panel_coop(int n, int j0, int w, cx<T>
 #pragma unroll
                 for (int u = 0; u < CP_W; ++u)
                     if (u < w) drow[((long)par * batch + b) * CP_W + u] = x[u];
-            grid.sync();
+            matrix_barrier(cnt + (long)b * n + j0, (unsigned)((c + 1) * G), info + b);
             // global pivot: same choice in every workgroup
             const long base = ((long)par * batch + b) * G;
             T v2 = (T)-1;
@@ -213,27 +237,37 @@ __global__ __launch_bounds__(CP_T) void k_panel_coop(int n, int j0, int w, cx<T>
             if (c < w) A[row + (long)(j0 + c) * lda] = x[c];
 }
 
-// Scratch of the cooperative panel (grows on demand, kept for the process).
+// Scratch of the cooperative panel (grows on demand, kept for the process): candidates,
+// diagonal rows and the barrier counters (batch x n).
 static void* g_coop_ws = nullptr;
 static size_t g_coop_bytes = 0;
 
+struct CoopWS {
+    void* crow;
+    void* drow;
+    void* cv;
+    int* ci;
+    unsigned* cnt;
+    int gmax;
+};
+
+__global__ void k_zero_u32(long count, unsigned* p) {
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) p[i] = 0u;
+}
+
+// Sizes the scratch for panels of up to n rows and clears the counters; false if unusable.
 template <typename T>
-static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* ipiv, long sP, int* info, int batch,
-                       hipStream_t s) {
-    static int capacity = -1;
-    if (capacity < 0) {
-        int dev = 0, per_cu = 0, cus = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_panel_coop<T>, CP_T, 0) != hipSuccess) per_cu = 0;
-        capacity = per_cu * cus;
+static bool coop_prepare(int n, int batch, hipStream_t s, CoopWS& ws) {
+    if (g_coop_min_rows < 0) {
+        const char* e = getenv("MNP_PANEL_COOP_MIN_ROWS");
+        g_coop_min_rows = e ? atoi(e) : 4096;
     }
-    const int G = (n - j0 + CP_T - 1) / CP_T;
-    // one grid barrier per column costs more than a single workgroup walking a short panel
-    if (n - j0 < CP_MIN_ROWS || w > CP_W || (long)G * batch > capacity) return false;
-    const size_t ncand = 2 * (size_t)batch * G;
-    const size_t bytes = ncand * (sizeof(T) + sizeof(int) + CP_W * sizeof(cx<T>)) + 2 * (size_t)batch * CP_W *
-                         sizeof(cx<T>) + 256;
+    if (n < g_coop_min_rows) return false;
+    const int gmax = (n + CP_T - 1) / CP_T;
+    const size_t ncand = 2 * (size_t)batch * gmax;
+    const size_t bytes = ncand * CP_W * sizeof(cx<T>) + 2 * (size_t)batch * CP_W * sizeof(cx<T>) +
+                         ncand * (sizeof(T) + sizeof(int)) + (size_t)batch * n * sizeof(unsigned) + 1024;
     if (bytes > g_coop_bytes) {
         if (g_coop_ws) {
             hipStreamSynchronize(s);
@@ -246,12 +280,43 @@ static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* i
         }
         g_coop_bytes = bytes;
     }
-    char* base = (char*)g_coop_ws;
-    cx<T>* crow = (cx<T>*)base;
-    cx<T>* drow = crow + ncand * CP_W;
-    T* cv = (T*)(drow + 2 * (size_t)batch * CP_W);
-    int* ci = (int*)(cv + ncand);
-    void* args[] = {&n, &j0, &w, &a, &lda, &sA, &ipiv, &sP, &info, &cv, &ci, &crow, &drow};
+    char* p = (char*)g_coop_ws;
+    ws.crow = p;
+    p += ncand * CP_W * sizeof(cx<T>);
+    ws.drow = p;
+    p += 2 * (size_t)batch * CP_W * sizeof(cx<T>);
+    ws.cv = p;
+    p += ncand * sizeof(T);
+    ws.ci = (int*)p;
+    p += ncand * sizeof(int);
+    ws.cnt = (unsigned*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    ws.gmax = gmax;
+    const long nc = (long)batch * n;
+    hipLaunchKernelGGL(k_zero_u32, dim3((nc + 255) / 256), dim3(256), 0, s, nc, ws.cnt);
+    return true;
+}
+
+template <typename T>
+static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+                       const CoopWS& ws, hipStream_t s) {
+    static int capacity = -1;
+    if (capacity < 0) {
+        int dev = 0, per_cu = 0, cus = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_panel_coop<T>, CP_T, 0) != hipSuccess) per_cu = 0;
+        capacity = per_cu * cus;
+    }
+    const int G = (n - j0 + CP_T - 1) / CP_T;
+    // a barrier per column costs more than a single workgroup walking a short panel
+    if (n - j0 < g_coop_min_rows || w > CP_W || G > ws.gmax || (long)G * batch > capacity) return false;
+    // candidate slots are laid out for G = gmax
+    T* cv = (T*)ws.cv;
+    int* ci = ws.ci;
+    cx<T>* crow = (cx<T>*)ws.crow;
+    cx<T>* drow = (cx<T>*)ws.drow;
+    unsigned* cnt = ws.cnt;
+    void* args[] = {&n, &j0, &w, &a, &lda, &sA, &ipiv, &sP, &info, &cv, &ci, &crow, &drow, &cnt};
     return hipLaunchCooperativeKernel((const void*)k_panel_coop<T>, dim3(G, batch), dim3(CP_T), args, 0, s) ==
            hipSuccess;
 }
@@ -362,13 +427,15 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
                    hipStream_t s) {
     cx<T>* a = (cx<T>*)A;
     const int NB = (NBO > nb) ? (NBO / nb) * nb : nb;
+    CoopWS ws{};
+    const bool coop = coop_prepare<T>(n, batch, s, ws);
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int W = (n - J0 < NB) ? (n - J0) : NB;
         const int JE = J0 + W;
         // factor the tall outer panel A[J0:n, J0:JE]
         for (int j0 = J0; j0 < JE; j0 += nb) {
             int w = (JE - j0 < nb) ? (JE - j0) : nb;
-            if (!panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, s))
+            if (!coop || !panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, ws, s))
                 hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv,
                                    sP, info);
             laswp<T>(j0, j0 + w, J0, j0, j0 + w, JE, a, lda, sA, ipiv, sP, batch, s);

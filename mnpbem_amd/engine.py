@@ -193,6 +193,10 @@ class RetSolverBatch:
         call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
              _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info), _p(g.nvec),
              _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
+        info = self.check_info()
+        if np.any(info < 0):
+            raise RuntimeError("LU panel synchronisation timed out (info {}); factors are invalid".format(info))
+        self.stats["zero_pivot"] = int(np.count_nonzero(info))
         return self
 
     def check_info(self):
