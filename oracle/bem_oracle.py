@@ -24,7 +24,7 @@ the reference algorithm of MNPBEM's retarded direct solver:
 
 Parity pins (see tests/test_oracle.py): the reference's MATLAB validation
 tables for trisphere(144, 20) in gold (ext/sca/abs over 41 wavelengths), the
-closed-form Mie series for a sphere, quadrature exactness, the Fuchs-Liu
+Mie series for a sphere (pinned itself by the reference's Mie table), quadrature exactness, the Fuchs-Liu
 closed-surface identity, brute-force integration of the refined
 coefficients on fine rules, LU backward error, and ext = sca + abs >= 0.
 
@@ -575,3 +575,36 @@ def solve_spectrum(surface, face_particle, inout, eps_funcs, enei_list, pol, dir
     if return_fields:
         out["fields"] = fields
     return out
+
+
+def mie_sphere(enei, diameter, eps_in, eps_out, lmax=20):
+    """Extinction and scattering cross sections (nm^2) of a sphere from the Mie series
+    (Bohren & Huffman, Absorption and Scattering of Light by Small Particles, ch. 4):
+    x = k a, m = n_in / n_out, a_n = (m psi_n(mx) psi_n'(x) - psi_n(x) psi_n'(mx)) /
+    (m psi_n(mx) xi_n'(x) - xi_n(x) psi_n'(mx)), b_n likewise with m moved to the other terms,
+    C_ext = 2 pi / k^2 sum (2n+1) Re(a_n + b_n), C_sca = 2 pi / k^2 sum (2n+1)(|a_n|^2 + |b_n|^2),
+    with the Riccati-Bessel functions psi_n(z) = z j_n(z), xi_n(z) = z h_n^(1)(z). lmax = 20
+    terms as the toolbox's miesolver default (mie/miesolver options)."""
+    from scipy.special import spherical_jn, spherical_yn
+    nb = np.sqrt(complex(eps_out))
+    k = 2 * np.pi / enei * nb.real
+    m = np.sqrt(complex(eps_in)) / nb
+    x = k * 0.5 * diameter
+    n = np.arange(1, lmax + 1)
+
+    def psi(z):
+        return z * spherical_jn(n, z), spherical_jn(n, z) + z * spherical_jn(n, z, derivative=True)
+
+    def xi(z):
+        h = spherical_jn(n, z) + 1j * spherical_yn(n, z)
+        dh = spherical_jn(n, z, derivative=True) + 1j * spherical_yn(n, z, derivative=True)
+        return z * h, h + z * dh
+
+    px, dpx = psi(x)
+    pmx, dpmx = psi(m * x)
+    xx, dxx = xi(x)
+    an = (m * pmx * dpx - px * dpmx) / (m * pmx * dxx - xx * dpmx)
+    bn = (pmx * dpx - m * px * dpmx) / (pmx * dxx - m * xx * dpmx)
+    ext = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * np.real(an + bn))
+    sca = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * (np.abs(an) ** 2 + np.abs(bn) ** 2))
+    return float(ext), float(sca)

@@ -141,6 +141,27 @@ def test_oracle_matches_matlab_sphere(gold_sphere):
     assert np.max(np.abs(out["ext"] - out["sca"] - ml[:, 3]) / ml[:, 3]) < 1e-9
 
 
+def test_mie_matches_matlab_table():
+    """validation/03_bemret/sphere/data/mie_matlab.csv: 20 nm gold sphere in vacuum."""
+    mie = np.loadtxt(os.path.join(GOLDEN, "mie_sphere_matlab.csv"), delimiter=",", skiprows=1)
+    for lam, ext, sca, _ in mie:
+        e, s_ = O.mie_sphere(lam, 20.0, O.eps_table(lam, GOLD)[0], 1.0)
+        assert abs(e - ext) / ext < 1e-7
+        assert abs(s_ - sca) / sca < 1e-7
+
+
+def test_oracle_bem_close_to_mie(gold_sphere):
+    """Independent physics check: the BEM cross sections of trisphere(144, 20) follow the Mie
+    series within the discretisation error of a 284-face mesh: a few per cent of the peak
+    (relative errors grow in the red tail, where the cross sections are small), same peak."""
+    ml, out = gold_sphere
+    mie = np.array([O.mie_sphere(lam, 20.0, O.eps_table(lam, GOLD)[0], 1.0) for lam in ml[:, 0]])
+    assert np.max(np.abs(out["ext"] - mie[:, 0])) / mie[:, 0].max() < 0.03
+    assert np.max(np.abs(out["sca"] - mie[:, 1])) / mie[:, 1].max() < 0.05
+    assert np.argmax(out["ext"]) == np.argmax(mie[:, 0])
+    assert np.all(np.abs(out["ext"] - mie[:, 0])[:15] / mie[:15, 0] < 0.03)
+
+
 def test_energy_balance_and_operator(gold_sphere):
     ml, out = gold_sphere
     assert np.all(out["ext"] - out["sca"] > 0)     # absorption >= 0
