@@ -28,6 +28,13 @@ DRUDE = (10.0, 9.065, 0.0708)
 ROD_N = [25, 11, 34]
 
 
+# HBM bytes per k_gemm_sub<float> launch of this sweep from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (one 32-wavelength batch, 998 launches), each scaled by the counter's reading for a
+# 256 MiB copy (FETCH x2.0, WRITE x1.0); the commands and raw averages are in the file named.
+GEMM_TRAFFIC_BYTES = int((188988.7 * 2.0 + 149433.2 * 1.0) * 1024)
+GEMM_TRAFFIC_SOURCE = "profiles/gemm_pmc_rod2496.md"
+
+
 def build_workload():
     import mnpbem_amd as M
     rod = M.trirod(10.0, 40.0, ROD_N, triangles=True)
@@ -227,10 +234,10 @@ def main():
     _lib.call("mnp_prof_enable", 1)
     sweep("mixed")
     torch.cuda.synchronize()
-    prof = np.zeros(3)
+    prof = np.zeros(4)
     _lib.call("mnp_prof_collect", prof.ctypes.data)
     _lib.call("mnp_prof_enable", 0)
-    gemm_ms, gemm_flops, gemm_launches = prof
+    gemm_ms, gemm_flops, gemm_launches, gemm_bytes = prof
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
 
     dt_mixed, out_mixed = results["mixed"]
@@ -269,8 +276,9 @@ def main():
                    "faces": n, "wavelengths": len(lams), "batch": args.batch, "nb": args.nb},
         "roofline": {"bound": "mfma", "kernel": "k_gemm_sub<float>", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": None, "gemm_ms_per_sweep": round(gemm_ms, 3),
-                     "gemm_launches_per_sweep": int(gemm_launches)},
+                     "traffic": GEMM_TRAFFIC_BYTES, "traffic_source": GEMM_TRAFFIC_SOURCE,
+                     "algorithmic_bytes_per_launch": round(gemm_bytes / max(gemm_launches, 1)),
+                     "gemm_ms_per_sweep": round(gemm_ms, 3), "gemm_launches_per_sweep": int(gemm_launches)},
         "cpu_baseline": cpu,
     }
     line.update(extra)

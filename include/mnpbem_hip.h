@@ -36,7 +36,8 @@ int mnp_probe_mfma_layout(double* out, void* stream);
 /* GEMM timing hooks for roofline reports (no counterpart in the reference): while enabled,
  * each matrix-core GEMM launch is bracketed by HIP events on its stream; collect returns
  * out[0] = GEMM kernel ms, out[1] = algorithmic flops (8 m n k per complex GEMM),
- * out[2] = launches since the previous collect (host array; waits for the events). */
+ * out[2] = launches since the previous collect, out[3] = algorithmic bytes (A and B read once,
+ * C read and written once) (host array of 4; waits for the events). */
 int mnp_prof_enable(int on);
 int mnp_prof_collect(double* out);
 
@@ -217,6 +218,24 @@ int mnp_hmat_matvec(const mnp_hmat* h, int kind, int r, const void* x, long ldx,
 int mnp_bem_operator_hmat(const mnp_hmat* h, const double* nvec, const double* L1, const double* L2,
                           const double* kvac, const void* x, const void* rhs, void* y, void* work, long sW,
                           double* norms, void* stream);
+
+
+/* ---------------------------------------------------------------------------------------------
+ * Near-field set-up (wavelength independent). */
+
+/* Taylor coefficients of near pairs (GreenRetRefined._refine_offdiagonal,
+ * greenfun/greenret_refined.py:287-516): pair p = (observer rows[p], source cols[p]) is
+ * integrated over the source's quadrature points qpts[qptr[j] .. qptr[j+1]) (3 doubles each)
+ * with weights qw; results go to g[slot[p] (order+1) + n] and f[(slot[p] 3 + c)(order+1) + n]
+ * (order < 8). */
+int mnp_near_terms(int npair, const int* rows, const int* cols, const long* slot, const double* pos, const int* qptr,
+                   const double* qpts, const double* qw, int order, double* g, double* f, void* stream);
+
+/* Collocated column sums of the quasistatic surface derivative over one closed surface
+ * (closed-surface correction, compgreen_stat.py:219-464 / compgreen_ret.py:192-272):
+ * out[jj] = sum_{i in loc} -area_i nvec_i.(pos_i - pos_j) / |pos_i - pos_j|^3 for j = ind[jj]. */
+int mnp_colsum_far(int nloc, const int* loc, int nind, const int* ind, const double* pos, const double* nvec,
+                   const double* area, double* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -41,6 +41,8 @@ _SIGS = {
     "mnp_hmat_compact": [I, P, I, P, P, P, P, P, P, P, P, P, P],
     "mnp_hmat_matvec": [P, I, I, P, L, P, L, P],
     "mnp_bem_operator_hmat": [P, P, P, P, P, P, P, P, P, L, P, P],
+    "mnp_near_terms": [I, P, P, P, P, P, P, P, I, P, P, P],
+    "mnp_colsum_far": [I, P, I, P, P, P, P, P, P],
 }
 
 

@@ -53,7 +53,10 @@ void probe_layout(double*, hipStream_t);
 void prof_enable(bool);
 void mdot(long, int, const void*, long, const void*, double*, hipStream_t);
 void mupdate(long, int, const void*, long, const double*, void*, hipStream_t);
-void prof_collect(double*, double*, double*);
+void prof_collect(double*, double*, double*, double*);
+void near_terms(int, const int*, const int*, const long*, const double*, const int*, const double*, const double*, int,
+                double*, double*, hipStream_t);
+void colsum_far(int, const int*, int, const int*, const double*, const double*, const double*, double*, hipStream_t);
 }  // namespace mnp
 
 using namespace mnp;
@@ -88,7 +91,7 @@ int mnp_prof_enable(int on) {
 // GEMM), out[2] = launches, since the previous collect. Host array; waits for the events.
 int mnp_prof_collect(double* out) {
     REQUIRE(out, 1);
-    prof_collect(out, out + 1, out + 2);
+    prof_collect(out, out + 1, out + 2, out + 3);
     return 0;
 }
 
@@ -472,6 +475,30 @@ int mnp_bem_operator_hmat(const mnp_hmat* h, const double* nvec, const double* L
     REQUIRE(sW >= 64L * h->n, 10);
     Face f{h->n, nvec, L1, L2, kvac};
     bem_operator_hmat(to_hmat(h), f, x, rhs, y, work, norms, (hipStream_t)stream);
+    return launch_status();
+}
+
+
+// ------------------------------------------------------------------------- near-field set-up
+int mnp_near_terms(int npair, const int* rows, const int* cols, const long* slot, const double* pos, const int* qptr,
+                   const double* qpts, const double* qw, int order, double* g, double* f, void* stream) {
+    REQUIRE(npair >= 0, 1);
+    REQUIRE(npair == 0 || (rows && cols && slot), 2);
+    REQUIRE(pos && qptr && qpts && qw, 5);
+    REQUIRE(order >= 0 && order < 8, 9);
+    REQUIRE(g && f, 10);
+    near_terms(npair, rows, cols, slot, pos, qptr, qpts, qw, order, g, f, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_colsum_far(int nloc, const int* loc, int nind, const int* ind, const double* pos, const double* nvec,
+                   const double* area, double* out, void* stream) {
+    REQUIRE(nloc >= 0, 1);
+    REQUIRE(nloc == 0 || loc, 2);
+    REQUIRE(nind >= 0, 3);
+    REQUIRE(nind == 0 || (ind && out), 4);
+    REQUIRE(pos && nvec && area, 5);
+    colsum_far(nloc, loc, nind, ind, pos, nvec, area, out, (hipStream_t)stream);
     return launch_status();
 }
 

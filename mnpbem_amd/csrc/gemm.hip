@@ -21,7 +21,7 @@ namespace mnp {
 // reporting. Off by default; see mnp_prof_enable / mnp_prof_collect in capi.hip.
 struct ProfRec {
     hipEvent_t a, b;
-    double flops;
+    double flops, bytes;
 };
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
@@ -40,21 +40,24 @@ static hipEvent_t pool_get() {
 
 void prof_enable(bool on) { g_prof_on = on; }
 
-// Sum of GEMM kernel time (ms) and algorithmic flops since the last call; waits for the events.
-void prof_collect(double* ms, double* flops, double* launches) {
-    double t = 0, f = 0;
+// Sum of GEMM kernel time (ms), algorithmic flops and bytes since the last call; waits for the
+// events.
+void prof_collect(double* ms, double* flops, double* launches, double* bytes) {
+    double t = 0, f = 0, by = 0;
     for (auto& r : g_prof) {
         hipEventSynchronize(r.b);
         float e = 0;
         hipEventElapsedTime(&e, r.a, r.b);
         t += e;
         f += r.flops;
+        by += r.bytes;
         g_pool.push_back(r.a);
         g_pool.push_back(r.b);
     }
     *ms = t;
     *flops = f;
     *launches = (double)g_prof.size();
+    *bytes = by;
     g_prof.clear();
 }
 
@@ -204,6 +207,8 @@ void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void*
         rec.a = pool_get();
         rec.b = pool_get();
         rec.flops = 8.0 * m * (double)n * k * batch;
+        // A and B read once, C read and written once
+        rec.bytes = 2.0 * sizeof(T) * ((double)m * k + (double)k * n + 2.0 * m * n) * batch;
         hipEventRecord(rec.a, s);
     }
     hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, m, n, k, (const cx<T>*)A, lda, sA, (const cx<T>*)B,

@@ -65,7 +65,7 @@ class CompGreenRet:
         self.p1 = self.p2 = p1
         self.deriv = "cart"
         self.nearfield = NearField(p1, rel_cutoff=op.get("RelCutoff", 3.0), abs_cutoff=op.get("AbsCutoff", 0.0),
-                                   order=op.get("order", 5))
+                                   order=op.get("order", 5), device=device())
         self.con = [[c for c in row] for row in connectivity(p1.inout)]
         self._geo = None
 

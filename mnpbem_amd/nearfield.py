@@ -1,6 +1,7 @@
-"""Near-field refinement of the retarded Green function (host setup).
+"""Near-field refinement of the retarded Green function (set-up).
 
-Everything here is wavelength independent and runs once per geometry:
+Everything here is wavelength independent and runs once per geometry (host numpy, or with
+``device`` the near-pair integrals and the closed-surface column sums in HIP kernels):
 
 * ``refine_pairs``: which (observer i, source j) face pairs are integrated
   rather than collocated: 2 = self term (centroid distance 0), 1 = near
@@ -120,10 +121,32 @@ def _grouped_points(part, faces, polar):
     return groups
 
 
-def refine_coefficients(part, rows, cols, kind, order=ORDER, chunk=200000):
+def _near_terms_device(part, rows, cols, nidx, order, g, f, dev):
+    """Near-pair coefficients on the device (``mnp_near_terms``), written into g[nidx], f[nidx]."""
+    import torch
+    from . import engine
+    from ._lib import call
+    pts, w, owner = part.quad(np.arange(part.nfaces))
+    o = np.argsort(owner, kind="stable")
+    qptr = np.concatenate([[0], np.cumsum(np.bincount(owner, minlength=part.nfaces))]).astype(np.int32)
+    T = lambda a, dt=None: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    tp = T(part.pos, torch.float64)
+    tq, tqp, tw = T(qptr), T(pts[o], torch.float64), T(w[o], torch.float64)
+    tr, tc = T(rows[nidx].astype(np.int32)), T(cols[nidx].astype(np.int32))
+    slot = T(np.arange(len(nidx), dtype=np.int64))
+    tg = torch.empty((len(nidx), order + 1), dtype=torch.float64, device=dev)
+    tf = torch.empty((len(nidx), 3, order + 1), dtype=torch.float64, device=dev)
+    call("mnp_near_terms", len(nidx), tr.data_ptr(), tc.data_ptr(), slot.data_ptr(), tp.data_ptr(), tq.data_ptr(),
+         tqp.data_ptr(), tw.data_ptr(), order, tg.data_ptr(), tf.data_ptr(), engine.stream_handle())
+    g[nidx] = tg.cpu().numpy()
+    f[nidx] = tf.cpu().numpy()
+
+
+def refine_coefficients(part, rows, cols, kind, order=ORDER, chunk=200000, device=None):
     """Coefficients g (n_ref, order+1) and f (n_ref, 3, order+1) (real) for the
     pairs of ``refine_pairs`` on one surface ``part`` (pos, nvec, tvec1, tvec2,
-    quad, quadpol)."""
+    quad, quadpol). ``device``: a torch device to integrate the near pairs there
+    (``mnp_near_terms``); self terms stay on the host."""
     n_ref = len(rows)
     g = np.zeros((n_ref, order + 1))
     f = np.zeros((n_ref, 3, order + 1))
@@ -139,7 +162,9 @@ def refine_coefficients(part, rows, cols, kind, order=ORDER, chunk=200000):
             f[sidx[sel]] = ff
     # near pairs, grouped by source face
     nidx = np.nonzero(kind == 1)[0]
-    if len(nidx):
+    if len(nidx) and device is not None:
+        _near_terms_device(part, rows, cols, nidx, order, g, f, device)
+    elif len(nidx):
         src = np.unique(cols[nidx])
         slot = np.full(part.nfaces, -1, dtype=np.int64)
         slot[src] = np.arange(len(src))
@@ -158,20 +183,37 @@ def refine_coefficients(part, rows, cols, kind, order=ORDER, chunk=200000):
     return g, f
 
 
-def stat_colsum(part, rows, cols, kind, f0_normal, loc, ind, chunk=4096):
+def _colsum_far_device(part, loc, ind, dev):
+    import torch
+    from . import engine
+    from ._lib import call
+    T = lambda a, dt=None: torch.tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    tl, ti = T(loc.astype(np.int32)), T(ind.astype(np.int32))
+    out = torch.empty(len(ind), dtype=torch.float64, device=dev)
+    tp, tn, ta = T(part.pos, torch.float64), T(part.nvec, torch.float64), T(part.area, torch.float64)
+    call("mnp_colsum_far", len(loc), tl.data_ptr(), len(ind), ti.data_ptr(), tp.data_ptr(), tn.data_ptr(),
+         ta.data_ptr(), out.data_ptr(), engine.stream_handle())
+    return out.cpu().numpy()
+
+
+def stat_colsum(part, rows, cols, kind, f0_normal, loc, ind, chunk=4096, device=None):
     """f_j = sum_{i in loc} a_i F_ij / a_j for j in ``ind``, with F the refined
     quasistatic surface derivative: far entries -n_i.(r_i - r_j) a_j / d^3,
-    refined entries ``f0_normal`` (= n_i . f_0 of the pair)."""
+    refined entries ``f0_normal`` (= n_i . f_0 of the pair). ``device``: a torch
+    device for the O(n^2) collocated sums (``mnp_colsum_far``)."""
     pos, nvec, area = part.pos, part.nvec, part.area
     loc = np.asarray(loc)
     ind = np.asarray(ind)
-    acc = np.zeros(len(ind))
-    for s in range(0, len(loc), chunk):
-        li = loc[s:s + chunk]
-        r = pos[li][:, None, :] - pos[ind][None, :, :]
-        d = np.maximum(np.sqrt(np.sum(r * r, axis=2)), _TINY)
-        nr = np.einsum("id,ijd->ij", nvec[li], r)
-        acc += np.sum(-area[li][:, None] * nr / d ** 3, axis=0)
+    if device is not None:
+        acc = _colsum_far_device(part, loc, ind, device)
+    else:
+        acc = np.zeros(len(ind))
+        for s in range(0, len(loc), chunk):
+            li = loc[s:s + chunk]
+            r = pos[li][:, None, :] - pos[ind][None, :, :]
+            d = np.maximum(np.sqrt(np.sum(r * r, axis=2)), _TINY)
+            nr = np.einsum("id,ijd->ij", nvec[li], r)
+            acc += np.sum(-area[li][:, None] * nr / d ** 3, axis=0)
     # swap the collocated value of refined pairs for the integrated one
     col_slot = np.full(part.nfaces, -1, dtype=np.int64)
     col_slot[ind] = np.arange(len(ind))
@@ -189,14 +231,17 @@ def stat_colsum(part, rows, cols, kind, f0_normal, loc, ind, chunk=4096):
 
 class NearField:
     """Refinement data of a composite particle: pairs, coefficients, and the
-    closed-surface correction already folded into the self terms."""
+    closed-surface correction already folded into the self terms.
 
-    def __init__(self, cp, rel_cutoff=3.0, abs_cutoff=0.0, order=ORDER):
+    ``device`` (a torch device) integrates the near pairs and the O(n^2) column sums of the
+    closed-surface correction in HIP kernels; None keeps every step on the host."""
+
+    def __init__(self, cp, rel_cutoff=3.0, abs_cutoff=0.0, order=ORDER, device=None):
         part = cp.pc
         rad = part.bradius()
         self.order = order
         self.rows, self.cols, self.kind = refine_pairs(part.pos, rad, part.pos, rad, rel_cutoff, abs_cutoff)
-        self.g, self.f = refine_coefficients(part, self.rows, self.cols, self.kind, order)
+        self.g, self.f = refine_coefficients(part, self.rows, self.cols, self.kind, order, device=device)
         nvec = part.nvec
         f0n = np.einsum("md,md->m", nvec[self.rows], self.f[:, :, 0])
         self.closed_correction = np.zeros(part.nfaces)
@@ -204,7 +249,7 @@ class NearField:
         sd = np.nonzero(self.kind == 2)[0]
         diag_slot[self.rows[sd]] = sd
         for ind, loc, direction in cp.closed_sets():
-            fj = stat_colsum(part, self.rows, self.cols, self.kind, f0n, loc, ind)
+            fj = stat_colsum(part, self.rows, self.cols, self.kind, f0n, loc, ind, device=device)
             corr = -2.0 * np.pi * direction - fj
             self.closed_correction[ind] = corr
             slots = diag_slot[ind]
