@@ -13,6 +13,7 @@
 // (measured on gfx950 by k_probe_layout).
 #include "common.hpp"
 
+#include <stdint.h>
 #include <vector>
 
 namespace mnp {
@@ -196,6 +197,80 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cx<
             }
 }
 
+// Complex-float variant for 16-byte aligned operands (even leading dimensions): K slices of
+// 8, each lane loads two consecutive complex values of A (rows i, i+1) or B (k, k+1) with one
+// 16-byte load, LDS holds interleaved (re, im) pairs read with one 8-byte load per operand.
+constexpr int VBK = 8;
+__global__ __launch_bounds__(256) void k_gemm_sub_c64v(int m, int n, int k, const cx<float>* __restrict__ A,
+                                                       long lda, long sA, const cx<float>* __restrict__ B, long ldb,
+                                                       long sB, cx<float>* __restrict__ C, long ldc, long sC) {
+    typedef float acc16 __attribute__((ext_vector_type(16)));
+    __shared__ float2 As[VBK][GBM + 2];
+    __shared__ float2 Bs[VBK][GBN + 2];
+    const int bz = blockIdx.z;
+    A += (long)bz * sA;
+    B += (long)bz * sB;
+    C += (long)bz * sC;
+    const int row0 = blockIdx.x * GBM, col0 = blockIdx.y * GBN;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    acc16 cre, cim;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { cre[r] = 0; cim[r] = 0; }
+    // lane t moves rows 2 (t % 32), 2 (t % 32) + 1 of A at k = t / 32, and k = 2 (t % 4), 2 (t % 4) + 1
+    // of B at column t / 4
+    const int ai = 2 * (t % 32), ak = t / 32, bk = 2 * (t % 4), bj = t / 4;
+    float4 pa, pb;
+    auto load = [&](int k0) {
+        const int gi = row0 + ai, gk = k0 + ak;
+        pa = make_float4(0, 0, 0, 0);
+        if (gk < k) {
+            const cx<float>* p = A + gi + (long)gk * lda;
+            if (gi + 1 < m) pa = *(const float4*)p;
+            else if (gi < m) { pa.x = p->re; pa.y = p->im; }
+        }
+        const int gk2 = k0 + bk, gj = col0 + bj;
+        pb = make_float4(0, 0, 0, 0);
+        if (gj < n) {
+            const cx<float>* p = B + gk2 + (long)gj * ldb;
+            if (gk2 + 1 < k) pb = *(const float4*)p;
+            else if (gk2 < k) { pb.x = p->re; pb.y = p->im; }
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < k; k0 += VBK) {
+        As[ak][ai] = make_float2(-pa.x, -pa.y);
+        As[ak][ai + 1] = make_float2(-pa.z, -pa.w);
+        Bs[bk][bj] = make_float2(pb.x, pb.y);
+        Bs[bk + 1][bj] = make_float2(pb.z, pb.w);
+        __syncthreads();
+        if (k0 + VBK < k) load(k0 + VBK);
+#pragma unroll
+        for (int ks = 0; ks < VBK; ks += 2) {
+            const int kk = ks + (lane >> 5);
+            const float2 a = As[kk][wm + (lane & 31)];
+            const float2 b = Bs[kk][wn + (lane & 31)];
+            cre = __builtin_amdgcn_mfma_f32_32x32x2f32(b.x, a.x, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f32_32x32x2f32(b.y, a.x, cim, 0, 0, 0);
+            cre = __builtin_amdgcn_mfma_f32_32x32x2f32(-b.y, a.y, cre, 0, 0, 0);
+            cim = __builtin_amdgcn_mfma_f32_32x32x2f32(b.x, a.y, cim, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int gi = row0 + wm + (lane & 31);
+        const int gj = col0 + wn + Mfma<float>::drow(lane, r);
+        if (gi < m && gj < n) {
+            cx<float>* c = C + gi + (long)gj * ldc;
+            cx<float> v = *c;
+            v.re += cre[r];
+            v.im += cim[r];
+            *c = v;
+        }
+    }
+}
+
 // C -= A * B for `batch` independent problems.
 template <typename T>
 void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb, long sB, void* C,
@@ -211,8 +286,15 @@ void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void*
         rec.bytes = 2.0 * sizeof(T) * ((double)m * k + (double)k * n + 2.0 * m * n) * batch;
         hipEventRecord(rec.a, s);
     }
-    hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, m, n, k, (const cx<T>*)A, lda, sA, (const cx<T>*)B,
-                       ldb, sB, (cx<T>*)C, ldc, sC);
+    // 16-byte loads need even leading dimensions and batch strides and 16-byte aligned bases
+    const bool vec = sizeof(T) == 4 && ((lda | ldb | sA | sB) & 1) == 0 && ((uintptr_t)A & 15) == 0 &&
+                     ((uintptr_t)B & 15) == 0;
+    if (vec)
+        hipLaunchKernelGGL(k_gemm_sub_c64v, grid, dim3(256), 0, s, m, n, k, (const cx<float>*)A, lda, sA,
+                           (const cx<float>*)B, ldb, sB, (cx<float>*)C, ldc, sC);
+    else
+        hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, m, n, k, (const cx<T>*)A, lda, sA, (const cx<T>*)B,
+                           ldb, sB, (cx<T>*)C, ldc, sC);
     if (g_prof_on) {
         hipEventRecord(rec.b, s);
         g_prof.push_back(rec);

@@ -59,21 +59,25 @@ def test_mfma_layout_probe():
     assert np.array_equal(g32, (rows + 16) * ((l % 32) + 1))
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_gemm_sub(prec):
-    banner("gemm_sub prec={}".format(prec))
+@pytest.mark.parametrize("prec,m,n,k,lda,ldb", [(0, 150, 97, 45, 150, 45), (1, 150, 97, 45, 150, 45),
+                                                 (1, 149, 97, 46, 150, 46), (1, 64, 64, 8, 64, 8),
+                                                 (1, 131, 70, 37, 132, 38), (0, 131, 70, 37, 132, 38)])
+def test_gemm_sub(prec, m, n, k, lda, ldb):
+    """Both complex-float kernels (16-byte loads for even leading dimensions, scalar loads
+    otherwise) and the complex-double one, with ragged edges and padded leading dimensions."""
+    banner("gemm_sub prec={} m={} n={} k={} lda={} ldb={}".format(prec, m, n, k, lda, ldb))
     rng = np.random.default_rng(1)
-    m, n, k, B = 150, 97, 45, 3
+    B_ = 3
     dt = torch.complex128 if prec == 0 else torch.complex64
-    A = [_rand(rng, m, k) for _ in range(B)]
-    Bm = [_rand(rng, k, n) for _ in range(B)]
-    Cm = [_rand(rng, m, n) for _ in range(B)]
+    A = [_rand(rng, lda, k) for _ in range(B_)]
+    Bm = [_rand(rng, ldb, n) for _ in range(B_)]
+    Cm = [_rand(rng, m, n) for _ in range(B_)]
     tA, tB, tC = _cm(A, dt), _cm(Bm, dt), _cm(Cm, dt)
-    _lib().call("mnp_gemm_sub_batched", prec, m, n, k, tA.data_ptr(), m, m * k, tB.data_ptr(), k, k * n,
-                tC.data_ptr(), m, m * n, B, _sh())
+    _lib().call("mnp_gemm_sub_batched", prec, m, n, k, tA.data_ptr(), lda, lda * k, tB.data_ptr(), ldb, ldb * n,
+                tC.data_ptr(), m, m * n, B_, _sh())
     tol = 1e-13 if prec == 0 else 2e-5
-    for b in range(B):
-        ref = Cm[b] - A[b] @ Bm[b]
+    for b in range(B_):
+        ref = Cm[b] - A[b][:m] @ Bm[b][:k]
         assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < tol
 
 
