@@ -422,6 +422,24 @@ static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, 
 // factored by k_panel inside the outer block.
 constexpr int NBO = 128;
 
+// Lookahead: the trailing update of an outer block is split into the next block's columns
+// (on the caller's stream) and the rest (on a second stream, default priority). The rest runs
+// while the next block's panels are factored: those touch only the next block's columns, and
+// the interchanges outside the block wait for the rest to finish (event).
+struct SideStream {
+    hipStream_t st = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+};
+static SideStream& side_stream() {
+    static SideStream ss;
+    if (!ss.st) {
+        hipStreamCreateWithFlags(&ss.st, hipStreamNonBlocking);
+        hipEventCreateWithFlags(&ss.ready, hipEventDisableTiming);
+        hipEventCreateWithFlags(&ss.done, hipEventDisableTiming);
+    }
+    return ss;
+}
+
 template <typename T>
 void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                    hipStream_t s) {
@@ -429,6 +447,8 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
     const int NB = (NBO > nb) ? (NBO / nb) * nb : nb;
     CoopWS ws{};
     const bool coop = coop_prepare<T>(n, batch, s, ws);
+    SideStream& side = side_stream();
+    bool pending = false;   // a rest update is running on the side stream
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int W = (n - J0 < NB) ? (n - J0) : NB;
         const int JE = J0 + W;
@@ -446,7 +466,12 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
                             lda, sA, a + (j0 + w) + (long)(j0 + w) * lda, lda, sA, batch, s);
             }
         }
-        // interchanges of the outer panel on the columns outside it
+        // interchanges of the outer panel on the columns outside it (after the previous block's
+        // rest update, which reads the left columns and writes the right ones)
+        if (pending) {
+            hipStreamWaitEvent(s, side.done, 0);
+            pending = false;
+        }
         laswp<T>(J0, JE, 0, J0, JE, n, a, lda, sA, ipiv, sP, batch, s);
         if (JE < n) {
             // U12 = L11^{-1} A12 for the W x (n - JE) block row, nb rows at a time
@@ -457,11 +482,22 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
                     gemm_sub<T>(JE - j0 - w, n - JE, w, a + (j0 + w) + (long)j0 * lda, lda, sA,
                                 a + j0 + (long)JE * lda, lda, sA, a + (j0 + w) + (long)JE * lda, lda, sA, batch, s);
             }
-            // trailing update with k = W
-            gemm_sub<T>(n - JE, n - JE, W, a + JE + (long)J0 * lda, lda, sA, a + J0 + (long)JE * lda, lda, sA,
+            // trailing update with k = W: the next block's columns here, the rest on the side stream
+            const int first = (n - JE < NB) ? (n - JE) : NB;
+            gemm_sub<T>(n - JE, first, W, a + JE + (long)J0 * lda, lda, sA, a + J0 + (long)JE * lda, lda, sA,
                         a + JE + (long)JE * lda, lda, sA, batch, s);
+            const int rest = n - JE - first;
+            if (rest > 0) {
+                hipEventRecord(side.ready, s);
+                hipStreamWaitEvent(side.st, side.ready, 0);
+                gemm_sub<T>(n - JE, rest, W, a + JE + (long)J0 * lda, lda, sA, a + J0 + (long)(JE + first) * lda, lda,
+                            sA, a + JE + (long)(JE + first) * lda, lda, sA, batch, side.st);
+                hipEventRecord(side.done, side.st);
+                pending = true;
+            }
         }
     }
+    if (pending) hipStreamWaitEvent(s, side.done, 0);
 }
 
 // Same for n beyond what LDS holds: the interchanges run in pf itself (global memory).
