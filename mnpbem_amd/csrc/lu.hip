@@ -96,6 +96,213 @@ __global__ __launch_bounds__(PANEL_THREADS) void k_panel(int n, int j0, int nb, 
     }
 }
 
+// Register-resident panel: one workgroup of 1024 lanes per matrix holds the whole (n - j0) x w
+// panel (n - j0 <= 1024 RPT rows, lane t owns rows j0 + t + 1024 q) in registers, CW columns at
+// a time (left-looking over column chunks):
+//   1. load the chunk's columns with the interchanges of the earlier chunks applied;
+//   2. U part: forward substitution of the top rows with the unit-lower triangle of the earlier
+//      chunks (in LDS), rows below: subtract L(rows, earlier columns) * U;
+//   3. factor the chunk column by column: pivot = largest |re|+|im| on or below the diagonal,
+//      smallest row on ties (as k_panel / izamax), the two rows trade places through LDS, rows
+//      below are scaled and updated;
+//   4. write the chunk back and apply its interchanges to the earlier chunks' columns.
+// Interchanges outside the panel are applied by k_laswp afterwards, as for k_panel.
+constexpr int RP_T = 1024, RP_RPT = 3, RP_MAXW = 32;
+template <typename T>
+struct RegChunk;
+template <>
+struct RegChunk<float> {
+    static constexpr int CW = 8;
+};
+template <>
+struct RegChunk<double> {
+    static constexpr int CW = 4;
+};
+static bool g_panel_reg = getenv("MNP_PANEL_REG") ? atoi(getenv("MNP_PANEL_REG")) != 0 : true;
+
+template <typename T, int CW>
+__global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>* A, long lda, long sA, int* ipiv,
+                                                    long sP, int* info) {
+    constexpr int NW = RP_T / 64;
+    __shared__ T red_v[NW];
+    __shared__ int red_i[NW];
+    __shared__ cx<T> urow[CW], crow[CW];
+    __shared__ cx<T> ublk[RP_MAXW][CW];
+    __shared__ int sw_p[RP_MAXW];          // pivot row chosen for panel column c (global index)
+    __shared__ int aff[2 * CW], src_s[2 * CW], naff_s;
+    __shared__ int piv_s;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    A += (long)b * sA;
+    ipiv += (long)b * sP;
+    const int nc = (w + CW - 1) / CW;
+    cx<T> x[RP_RPT][CW];
+    for (int k = 0; k < nc; ++k) {
+        const int c0 = k * CW;
+        const int cw = min(CW, w - c0);
+        // 1. load with the earlier interchanges applied (track each row back through the swaps)
+#pragma unroll
+        for (int q = 0; q < RP_RPT; ++q) {
+            const int r = j0 + t + RP_T * q;
+            int src = r;
+            for (int c = c0 - 1; c >= 0; --c) {
+                const int rc = j0 + c, pc = sw_p[c];
+                if (src == rc) src = pc;
+                else if (src == pc) src = rc;
+            }
+#pragma unroll
+            for (int c = 0; c < CW; ++c)
+                x[q][c] = (r < n && c < cw) ? A[src + (long)(j0 + c0 + c) * lda] : mk<T>(0, 0);
+        }
+        if (c0 > 0) {
+            // 2a. U rows j0 .. j0+c0-1 of the chunk: forward substitution (lane c per column)
+            if (t < c0)
+#pragma unroll
+                for (int c = 0; c < CW; ++c) ublk[t][c] = x[0][c];
+            __syncthreads();
+            if (t < cw) {
+                for (int i = 1; i < c0; ++i) {
+                    cx<T> v = ublk[i][t];
+                    for (int qq = 0; qq < i; ++qq) v = fms(v, A[(j0 + i) + (long)(j0 + qq) * lda], ublk[qq][t]);
+                    ublk[i][t] = v;
+                }
+            }
+            __syncthreads();
+            if (t < c0)
+#pragma unroll
+                for (int c = 0; c < CW; ++c) x[0][c] = ublk[t][c];
+            // 2b. rows below: x -= L(row, 0..c0-1) * U
+#pragma unroll
+            for (int q = 0; q < RP_RPT; ++q) {
+                const int r = j0 + t + RP_T * q;
+                if (r < n && r >= j0 + c0) {
+                    for (int s2 = 0; s2 < c0; ++s2) {
+                        const cx<T> l = A[r + (long)(j0 + s2) * lda];
+#pragma unroll
+                        for (int c = 0; c < CW; ++c) x[q][c] = fms(x[q][c], l, ublk[s2][c]);
+                    }
+                }
+            }
+        }
+        // 3. factor the chunk
+#pragma unroll
+        for (int cc = 0; cc < CW; ++cc) {
+            if (cc < cw) {
+                const int col = j0 + c0 + cc;
+                T best = (T)-1;
+                int bi = n;
+#pragma unroll
+                for (int q = 0; q < RP_RPT; ++q) {
+                    const int r = j0 + t + RP_T * q;
+                    if (r < n && r >= col) {
+                        const T v = cabs1(x[q][cc]);
+                        if (v > best) { best = v; bi = r; }
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    T ov = __shfl_xor(best, off, 64);
+                    int oi = __shfl_xor(bi, off, 64);
+                    if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+                }
+                if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
+                __syncthreads();
+                if (t == 0) {
+                    T bv = red_v[0];
+                    int bx = red_i[0];
+                    for (int q = 1; q < NW; ++q)
+                        if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) { bv = red_v[q]; bx = red_i[q]; }
+                    if (bx >= n) bx = col;
+                    piv_s = bx;
+                    sw_p[c0 + cc] = bx;
+                    ipiv[col] = bx;
+                }
+                __syncthreads();
+                const int p = piv_s;
+                // rows col and p trade places (chunk columns)
+#pragma unroll
+                for (int q = 0; q < RP_RPT; ++q) {
+                    const int r = j0 + t + RP_T * q;
+                    if (r == p)
+#pragma unroll
+                        for (int c = 0; c < CW; ++c) urow[c] = x[q][c];
+                    if (r == col)
+#pragma unroll
+                        for (int c = 0; c < CW; ++c) crow[c] = x[q][c];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < RP_RPT; ++q) {
+                    const int r = j0 + t + RP_T * q;
+                    if (r == p && p != col)
+#pragma unroll
+                        for (int c = 0; c < CW; ++c) x[q][c] = crow[c];
+                    if (r == col)
+#pragma unroll
+                        for (int c = 0; c < CW; ++c) x[q][c] = urow[c];
+                }
+                const cx<T> piv = urow[cc];
+                const bool zero = piv.re == 0 && piv.im == 0;
+                if (zero && t == 0 && info[b] == 0) info[b] = col + 1;
+                if (!zero) {
+                    const cx<T> inv = crecip(piv);
+#pragma unroll
+                    for (int q = 0; q < RP_RPT; ++q) {
+                        const int r = j0 + t + RP_T * q;
+                        if (r < n && r > col) {
+                            const cx<T> l = x[q][cc] * inv;
+                            x[q][cc] = l;
+#pragma unroll
+                            for (int c = cc + 1; c < CW; ++c) x[q][c] = fms(x[q][c], l, urow[c]);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // 4. write back; then this chunk's interchanges on the earlier chunks' columns
+#pragma unroll
+        for (int q = 0; q < RP_RPT; ++q) {
+            const int r = j0 + t + RP_T * q;
+            if (r < n)
+#pragma unroll
+                for (int c = 0; c < CW; ++c)
+                    if (c < cw) A[r + (long)(j0 + c0 + c) * lda] = x[q][c];
+        }
+        if (c0 > 0) {
+            if (t == 0) {
+                // rows touched by this chunk's swaps and, for each, the row whose values it receives
+                int na = 0;
+                for (int cc = 0; cc < cw; ++cc) {
+                    const int rr[2] = {j0 + c0 + cc, sw_p[c0 + cc]};
+                    for (int e = 0; e < 2; ++e) {
+                        bool seen = false;
+                        for (int a2 = 0; a2 < na; ++a2) seen |= aff[a2] == rr[e];
+                        if (!seen) aff[na++] = rr[e];
+                    }
+                }
+                for (int a2 = 0; a2 < na; ++a2) {
+                    int src = aff[a2];
+                    for (int cc = cw - 1; cc >= 0; --cc) {
+                        const int rc = j0 + c0 + cc, pc = sw_p[c0 + cc];
+                        if (src == rc) src = pc;
+                        else if (src == pc) src = rc;
+                    }
+                    src_s[a2] = src;
+                }
+                naff_s = na;
+            }
+            __syncthreads();
+            const int na = naff_s;
+            cx<T> v = mk<T>(0, 0);
+            const bool act = t < na * c0;
+            const int a2 = act ? t / c0 : 0, cc = act ? t % c0 : 0;
+            if (act) v = A[src_s[a2] + (long)(j0 + cc) * lda];
+            __syncthreads();
+            if (act) A[aff[a2] + (long)(j0 + cc) * lda] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // Panel factorisation spread over the whole device (cooperative launch, grid G x batch).
 // Each thread holds one row of the (n - j0) x w panel (w <= 32) in registers, 256 rows per
 // workgroup. Per column: every workgroup publishes its best pivot candidate (value, row index
@@ -448,6 +655,7 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
     CoopWS ws{};
     const bool coop = coop_prepare<T>(n, batch, s, ws);
     SideStream& side = side_stream();
+    static const bool lookahead = getenv("MNP_LU_LOOKAHEAD") ? atoi(getenv("MNP_LU_LOOKAHEAD")) != 0 : true;
     bool pending = false;   // a rest update is running on the side stream
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int W = (n - J0 < NB) ? (n - J0) : NB;
@@ -455,9 +663,15 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
         // factor the tall outer panel A[J0:n, J0:JE]
         for (int j0 = J0; j0 < JE; j0 += nb) {
             int w = (JE - j0 < nb) ? (JE - j0) : nb;
-            if (!coop || !panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, ws, s))
-                hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv,
-                                   sP, info);
+            if (!coop || !panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, ws, s)) {
+                // complex float only: complex double needs 4-column chunks and runs slower than k_panel
+                if (sizeof(T) == 4 && n - j0 <= RP_T * RP_RPT && w <= RP_MAXW && g_panel_reg)
+                    hipLaunchKernelGGL((k_panel_reg<T, RegChunk<T>::CW>), dim3(batch), dim3(RP_T), 0, s, n, j0, w, a, lda,
+                                       sA, ipiv, sP, info);
+                else
+                    hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(PANEL_THREADS), 0, s, n, j0, w, a, lda, sA, ipiv,
+                                       sP, info);
+            }
             laswp<T>(j0, j0 + w, J0, j0, j0 + w, JE, a, lda, sA, ipiv, sP, batch, s);
             int rest = JE - j0 - w;
             if (rest > 0) {
@@ -483,7 +697,7 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
                                 a + j0 + (long)JE * lda, lda, sA, a + (j0 + w) + (long)JE * lda, lda, sA, batch, s);
             }
             // trailing update with k = W: the next block's columns here, the rest on the side stream
-            const int first = (n - JE < NB) ? (n - JE) : NB;
+            const int first = lookahead ? ((n - JE < NB) ? (n - JE) : NB) : n - JE;
             gemm_sub<T>(n - JE, first, W, a + JE + (long)J0 * lda, lda, sA, a + J0 + (long)JE * lda, lda, sA,
                         a + JE + (long)JE * lda, lda, sA, batch, s);
             const int rest = n - JE - first;
