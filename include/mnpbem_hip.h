@@ -237,6 +237,33 @@ int mnp_near_terms(int npair, const int* rows, const int* cols, const long* slot
 int mnp_colsum_far(int nloc, const int* loc, int nind, const int* ind, const double* pos, const double* nvec,
                    const double* area, double* out, void* stream);
 
+
+/* ---------------------------------------------------------------------------------------------
+ * Learned GMRES warm start (no counterpart in the reference: BEMRetIter starts GMRES from zero,
+ * bem_ret_iter.py:1695). A per-face MLP; all arrays row-major float, device pointers. */
+
+/* C[M][N] = op(A)[M][K] op(B)[K][N] on the BF16 matrix cores (operands rounded to BF16, FP32
+ * accumulation); op = transpose when ta / tb is 1. epi: 0 Z = AB, 1 Z += AB, 2 Z = AB + bias and
+ * H = gelu(Z) (erf form), 3 Z = AB + bias (2 and 3 without transposes). */
+int mnp_mlp_gemm(int ta, int tb, int epi, int M, int N, int K, const float* A, long lda, const float* B, long ldb,
+                 const float* bias, float* Z, float* H, long ldc, void* stream);
+
+/* dZ = dH * gelu'(Z), elementwise over count floats. */
+int mnp_mlp_gelu_bwd(long count, const float* dH, const float* Z, float* dZ, void* stream);
+
+/* out[j] = sum_i X[i][j] (bias gradients). */
+int mnp_mlp_colsum(int M, int N, const float* X, long ld, float* out, void* stream);
+
+/* loss[0] = mean (Y - T)^2 over count floats, dY = 2 (Y - T) / count. */
+int mnp_mlp_mse(long count, const float* Y, const float* T, float* dY, float* loss, void* stream);
+
+/* One Adam step (bias-corrected, `step` >= 1) on count parameters p with gradients g and moments m, v. */
+int mnp_mlp_adam(long count, float* p, const float* g, float* m, float* v, double lr, double b1, double b2, double eps,
+                 int step, void* stream);
+
+/* Test hook: result registers of one v_mfma_f32_32x32x8bf16_1k on known operands (64 x 16 floats). */
+int mnp_probe_mfma_bf16(float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

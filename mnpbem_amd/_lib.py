@@ -43,6 +43,12 @@ _SIGS = {
     "mnp_bem_operator_hmat": [P, P, P, P, P, P, P, P, P, L, P, P],
     "mnp_near_terms": [I, P, P, P, P, P, P, P, I, P, P, P],
     "mnp_colsum_far": [I, P, I, P, P, P, P, P, P],
+    "mnp_mlp_gemm": [I, I, I, I, I, I, P, L, P, L, P, P, P, L, P],
+    "mnp_mlp_gelu_bwd": [L, P, P, P, P],
+    "mnp_mlp_colsum": [I, I, P, L, P, P],
+    "mnp_mlp_mse": [L, P, P, P, P, P],
+    "mnp_mlp_adam": [L, P, P, P, P, C.c_double, C.c_double, C.c_double, C.c_double, I, P],
+    "mnp_probe_mfma_bf16": [P, P],
 }
 
 

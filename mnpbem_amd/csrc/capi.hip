@@ -57,6 +57,13 @@ void prof_collect(double*, double*, double*, double*);
 void near_terms(int, const int*, const int*, const long*, const double*, const int*, const double*, const double*, int,
                 double*, double*, hipStream_t);
 void colsum_far(int, const int*, int, const int*, const double*, const double*, const double*, double*, hipStream_t);
+int mlp_gemm(int, int, int, int, int, int, const float*, long, const float*, long, const float*, float*, float*, long,
+             hipStream_t);
+void mlp_gelu_bwd(long, const float*, const float*, float*, hipStream_t);
+void mlp_colsum(int, int, const float*, long, float*, hipStream_t);
+void mlp_mse(long, const float*, const float*, float*, float*, hipStream_t);
+void mlp_adam(long, float*, const float*, float*, float*, float, float, float, float, int, hipStream_t);
+void mlp_probe(float*, hipStream_t);
 }  // namespace mnp
 
 using namespace mnp;
@@ -499,6 +506,61 @@ int mnp_colsum_far(int nloc, const int* loc, int nind, const int* ind, const dou
     REQUIRE(nind == 0 || (ind && out), 4);
     REQUIRE(pos && nvec && area, 5);
     colsum_far(nloc, loc, nind, ind, pos, nvec, area, out, (hipStream_t)stream);
+    return launch_status();
+}
+
+
+// ------------------------------------------------------------------------- warm-start MLP
+int mnp_mlp_gemm(int ta, int tb, int epi, int M, int N, int K, const float* A, long lda, const float* B, long ldb,
+                 const float* bias, float* Z, float* H, long ldc, void* stream) {
+    REQUIRE(ta == 0 || ta == 1, 1);
+    REQUIRE(tb == 0 || tb == 1, 2);
+    REQUIRE(epi >= 0 && epi <= 3 && !(epi >= 2 && (ta || tb)), 3);
+    REQUIRE(M > 0 && N > 0 && K > 0, 4);
+    REQUIRE(A && lda >= (ta ? M : K), 7);
+    REQUIRE(B && ldb >= (tb ? K : N), 9);
+    REQUIRE(epi < 2 || bias, 11);
+    REQUIRE(Z && ldc >= N, 12);
+    REQUIRE(epi != 2 || H, 13);
+    if (mlp_gemm(ta, tb, epi, M, N, K, A, lda, B, ldb, bias, Z, H, ldc, (hipStream_t)stream) != 0) return -3;
+    return launch_status();
+}
+
+int mnp_mlp_gelu_bwd(long count, const float* dH, const float* Z, float* dZ, void* stream) {
+    REQUIRE(count > 0, 1);
+    REQUIRE(dH && Z && dZ, 2);
+    mlp_gelu_bwd(count, dH, Z, dZ, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_mlp_colsum(int M, int N, const float* X, long ld, float* out, void* stream) {
+    REQUIRE(M > 0 && N > 0, 1);
+    REQUIRE(X && ld >= N, 3);
+    REQUIRE(out, 5);
+    mlp_colsum(M, N, X, ld, out, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_mlp_mse(long count, const float* Y, const float* T, float* dY, float* loss, void* stream) {
+    REQUIRE(count > 0, 1);
+    REQUIRE(Y && T && dY && loss, 2);
+    mlp_mse(count, Y, T, dY, loss, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_mlp_adam(long count, float* p, const float* g, float* m, float* v, double lr, double b1, double b2, double eps,
+                 int step, void* stream) {
+    REQUIRE(count > 0, 1);
+    REQUIRE(p && g && m && v, 2);
+    REQUIRE(lr > 0 && b1 >= 0 && b1 < 1 && b2 >= 0 && b2 < 1 && eps > 0, 6);
+    REQUIRE(step >= 1, 10);
+    mlp_adam(count, p, g, m, v, (float)lr, (float)b1, (float)b2, (float)eps, step, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_probe_mfma_bf16(float* out, void* stream) {
+    REQUIRE(out, 1);
+    mlp_probe(out, (hipStream_t)stream);
     return launch_status();
 }
 
