@@ -87,9 +87,7 @@ def gmres25k():
     import mnpbem_amd as M
     from mnpbem_amd.bem_iter import BEMRetIter
     t0 = time.perf_counter()
-    p1 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, -41.0])
-    p2 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, 41.0])
-    cp = M.ComParticle([M.EpsConst(1.0), M.EpsDrude(10, 9.065, 0.0708)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+    cp = _dimer25k()
     g = M.CompGreenRet(cp, cp)
     g.geo
     exc = M.PlaneWaveRet([0, 0, 1.0], [1.0, 0, 0])
@@ -115,6 +113,93 @@ def gmres25k():
     del it, g
     torch.cuda.empty_cache()
     return out
+
+
+def _dimer25k():
+    import mnpbem_amd as M
+    p1 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, -41.0])
+    p2 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, 41.0])
+    return M.ComParticle([M.EpsConst(1.0), M.EpsDrude(10, 9.065, 0.0708)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+
+
+def warmstart25k(nlam=11, seed=7):
+    """Learned warm start on the rod dimer: the MLP is trained on direct solves of 12 seeded
+    rods and ellipsoids (0.5k-3k faces, 6 wavelengths each, 450-950 nm), then GMRES(50) to 1e-6
+    with H-matrices runs at nlam wavelengths over 600-900 nm from the zero start, the previous
+    wavelength's solution and the learned start. The preconditioner (complex64 BEMRet chain) is
+    factored once at 750 nm and reused across the sweep. Times: GMRES plus, for the learned
+    start, feature extraction and inference."""
+    import torch
+    import mnpbem_amd as M
+    from mnpbem_amd import warmstart as W
+    from mnpbem_amd.bem_iter import BEMRetIter
+    pol, direc = [0, 0, 1.0], [1.0, 0, 0]
+    drude = M.EpsDrude(10, 9.065, 0.0708)
+    rng = np.random.default_rng(seed)
+    cps = []
+    for i in range(12):
+        if i % 2 == 0:
+            d = float(rng.uniform(8, 20))
+            L = float(rng.uniform(2.5, 6.0)) * d
+            part = M.trirod(d, L, [int(rng.integers(20, 36)), 9, int(rng.integers(14, 40))], triangles=True)
+        else:
+            a = float(rng.uniform(5, 10))
+            part = M.triellipsoid(int(rng.choice([400, 576, 784, 1024])), [a, a, a * float(rng.uniform(1, 4))])
+        cps.append(M.ComParticle([M.EpsConst(1.0), drude], [part], [[2, 1]], 1))
+    t0 = time.perf_counter()
+    X, T = W.training_set(cps, rng.uniform(450, 950, 6), pol, direc)
+    t_data = time.perf_counter() - t0
+    net = W.WarmStartMLP(seed=0)
+    t0 = time.perf_counter()
+    hist = net.fit(X, T, steps=1500, batch=8192, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    cp = _dimer25k()
+    g = M.CompGreenRet(cp, cp)
+    exc = M.PlaneWaveRet(pol, direc)
+    H = W.mean_curvature(cp)
+    it = BEMRetIter(cp, tol=1e-6, maxit=400, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5,
+                    green=g, precond_wavelength=750.0)
+    modes = ("zero", "previous", "learned", "previous+learned")
+    res = {m: dict(iterations=[], time_s=0.0, initial_residual=[]) for m in modes}
+    prev = {}
+    precond_s = 0.0
+    for lam in np.linspace(600, 900, nlam):
+        e = exc(cp, lam)
+        for m in modes:
+            t0 = time.perf_counter()
+            x0 = None
+            if m == "previous":
+                x0 = prev.get(m)
+            elif m in ("learned", "previous+learned"):
+                xl = W.guess(cp, net.predict(W.features(cp, lam, pol, direc, curvature=H)), pol)
+                # candidates judged by the residual: the start is their best combination
+                x0 = [xl] if m == "learned" else [prev.get(m), xl]
+            t_guess = time.perf_counter() - t0
+            sig, _ = it.solve(e, x0=x0)
+            if m == modes[0]:
+                precond_s += it.timings.get("precond", 0.0)
+            flag, relres, niter = it.info()
+            r = res[m]
+            r["iterations"].append(int(niter))
+            r["initial_residual"].append(round(float(it.history[0]), 4))
+            r["time_s"] += it.timings["gmres"] + t_guess
+            r["converged"] = r.get("converged", True) and flag == 0
+            prev[m] = sig
+    for m in modes:
+        res[m]["time_s"] = round(res[m]["time_s"], 3)
+        res[m]["total_iterations"] = int(sum(res[m]["iterations"]))
+    del it, g
+    torch.cuda.empty_cache()
+    return dict(workload="rod dimer n={}, {} wavelengths 600-900 nm, GMRES(50) to 1e-6, H-matrices, complex64 "
+                         "BEMRet-chain preconditioner factored once at 750 nm".format(cp.nfaces, nlam),
+                training=dict(solves=len(cps) * 6, samples=int(X.shape[0]), data_s=round(t_data, 2),
+                              train_s=round(t_train, 2), steps=1500, loss_first=round(hist[0], 4),
+                              loss_last=round(hist[-1], 4)),
+                precond_once_s=round(precond_s, 3), starts=res,
+                speedup_learned_vs_zero=round(res["zero"]["time_s"] / max(res["learned"]["time_s"], 1e-9), 3),
+                speedup_learned_vs_previous=round(res["previous"]["time_s"] /
+                                                  max(res["previous+learned"]["time_s"], 1e-9), 3))
 
 
 def particles64(seed=20261203):
@@ -194,6 +279,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-gmres25k", action="store_true")
     ap.add_argument("--no-batched64", action="store_true")
+    ap.add_argument("--no-warmstart", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this pool has one MI355X and the wavelength sweep "
@@ -288,6 +374,8 @@ def main():
         line["batched64"] = batched64()
     if not args.no_gmres25k:
         line["gmres25k"] = gmres25k()
+    if not args.no_warmstart:
+        line["warmstart25k"] = warmstart25k()
     print(json.dumps(line), flush=True)
 
 

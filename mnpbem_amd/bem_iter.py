@@ -118,7 +118,9 @@ class BEMRetIter:
     """Iterative BEMRet. Options (reference names): ``solver='gmres'``, ``tol=1e-4``,
     ``maxit=200``, ``restart=None`` (= min(n, maxit)), ``precond='hmat'|'bemret'|None``,
     ``preconditioner='auto'|'none'``, ``precond_precision='single'|'double'``, ``hmatrix``,
-    ``htol=1e-6``, ``kmax=[4, 100]``, ``cleaf=200``, ``eta=2.5``, ``fadmiss``."""
+    ``htol=1e-6``, ``kmax=[4, 100]``, ``cleaf=200``, ``eta=2.5``, ``fadmiss``; not in the
+    reference: ``precond_wavelength`` (factor the preconditioner at that wavelength once and
+    reuse it for a sweep) and ``solve(exc, x0=...)``."""
 
     def __init__(self, p, enei=None, **op):
         if p is None:
@@ -143,6 +145,9 @@ class BEMRetIter:
         # preconditioner='none' (bem_ret_iter.py:1688-1692, :1745-1774)
         self.use_precond = pc != "none" if self.hmatrix else not (pc == "none" or pr is None)
         self.precond_precision = op.get("precond_precision", "single")
+        # None: the preconditioner is the BEMRet chain of each solve's wavelength (reference);
+        # a wavelength: the chain is factored there once and reused for every later solve
+        self.precond_wavelength = op.get("precond_wavelength", None)
         self.g = op.get("green") or CompGreenRet(p, p, **{k: v for k, v in op.items()
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
         p._mnp_geo = self.g.geo
@@ -173,13 +178,18 @@ class BEMRetIter:
             self._exact = engine.GreenBatch(geo, 1, torch.complex128)
             self._exact.assemble(self._kmat, 1)
             self._opwork = torch.empty((1, 32, geo.n), dtype=torch.complex128, device=geo.dev)
-        self._pre = None
-        if self.use_precond:
-            prec = "mixed" if self.precond_precision == "single" else "double"
-            # the preconditioner's factor precision; refinement is done by GMRES itself
-            self._pre = engine.RetSolverBatch(geo, 1, "single" if prec == "mixed" else "double")
+        keep = (self.precond_wavelength is not None and getattr(self, "_pre", None) is not None
+                and np.isclose(self._pre_enei, self.precond_wavelength))
+        self.timings["precond"] = 0.0
+        if not keep:
+            self._pre = None
+        if self.use_precond and not keep:
+            torch.cuda.empty_cache()
+            # complex64 (default) or complex128 chain; refinement is done by GMRES itself
+            self._pre = engine.RetSolverBatch(geo, 1, "single" if self.precond_precision == "single" else "double")
+            self._pre_enei = enei if self.precond_wavelength is None else float(self.precond_wavelength)
             t0 = _now(geo.dev)
-            self._pre.factor([enei])
+            self._pre.factor([self._pre_enei])
             self.timings["precond"] = _now(geo.dev) - t0
         self.enei = enei
 
@@ -203,9 +213,30 @@ class BEMRetIter:
         n = self.g.geo.n
         z.copy_(self._pre.solve(v.view(1, 8, n)).view(-1))
 
+    def _best_start(self, b, guesses):
+        """Residual-minimising combination x0 = sum_i c_i g_i of candidate guesses: one operator
+        application per guess, then the small least-squares problem min |b - sum c_i A g_i|
+        on the host. A useless guess gets a coefficient near zero, so the start is never worse
+        than zero (|b - A x0| <= |b|)."""
+        AG = []
+        for gv in guesses:
+            y = torch.empty_like(b)
+            self._apply_A(gv, y, rhs=None)
+            AG.append(y)
+        Am = torch.stack(AG, dim=1)                       # (L, k)
+        M_ = (Am.conj().T @ Am).cpu().numpy()
+        r = (Am.conj().T @ b).cpu().numpy()
+        c = np.linalg.lstsq(M_, r, rcond=1e-12)[0]
+        x0 = torch.zeros_like(b)
+        for ci, gv in zip(c, guesses):
+            x0.add_(gv, alpha=complex(ci))
+        self.start_coefficients = c
+        return x0
+
     def solve(self, exc, x0=None):
         """Solve for excitation ``exc``; ``x0`` (optional) = packed initial guess or a
-        CompStruct with sig1, h1, sig2, h2. Returns (sig, self)."""
+        CompStruct with sig1, h1, sig2, h2, or a list of such guesses (then the start is their
+        residual-minimising combination, see ``_best_start``). Returns (sig, self)."""
         enei = exc.enei
         self._init_matrices(enei)
         p, n = self.p, self.p.nfaces
@@ -221,10 +252,14 @@ class BEMRetIter:
         for ip in range(npol):
             b = torch.tensor(_pack_rhs(p, exc, k0, eps1, eps2, ip).ravel(), device=self.g.geo.dev)
             xg = None
-            if x0 is not None:
+            t0 = _now(self.g.geo.dev)
+            if isinstance(x0, (list, tuple)):
+                cands = [torch.tensor(np.asarray(_packed(g_, ip, n)).ravel(), dtype=torch.complex128,
+                                      device=self.g.geo.dev) for g_ in x0 if g_ is not None]
+                xg = self._best_start(b, cands) if cands else None
+            elif x0 is not None:
                 xg = torch.tensor(np.asarray(_packed(x0, ip, n)).ravel(), dtype=torch.complex128,
                                   device=self.g.geo.dev)
-            t0 = _now(self.g.geo.dev)
             x, flag, relres, it, hist = gm.solve(self._apply_A, self._apply_M, b, xg, self.tol, self.maxit)
             self.timings["gmres"] = _now(self.g.geo.dev) - t0
             out[ip] = x.view(8, n).cpu().numpy()

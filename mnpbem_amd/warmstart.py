@@ -10,8 +10,12 @@ on the BF16 matrix cores with FP32 accumulation (``mnp_mlp_gemm``), training use
 (``mnp_mlp_adam``) on FP32 master weights. GMRES judges every result: the guess only changes
 where the iteration starts.
 
-Features per face: centroid (3), unit normal (3), area, local mean curvature, Re/Im of the
-inside eps(lambda), cos/sin of the excitation phase k (dir . r) at the centroid.
+Features per face: centroid relative to its particle's centre (3), unit normal (3), the
+particle's half-length along the polarisation, local mean curvature, Re/Im of the inside
+eps(lambda), cos/sin of the excitation phase k (dir . r) at the centroid. In a solve the
+learned guess is one candidate of BEMRetIter's residual-minimising start (with the previous
+wavelength's solution when there is one), so a poor prediction costs one operator
+application and never a worse start than zero.
 Targets: Re/Im sigma2, Re/Im (h1 . pol).
 """
 import numpy as np
@@ -43,13 +47,28 @@ def mean_curvature(cp, k=7):
 
 
 def features(cp, enei, pol, direc, medium=1, curvature=None):
-    """(n, 12) float64 feature matrix of ComParticle ``cp`` at wavelength ``enei``."""
-    pos, nvec, area = cp.pos, cp.nvec, cp.area
+    """(n, 12) float64 feature matrix of ComParticle ``cp`` at wavelength ``enei``: position
+    relative to the face's particle centre (3), unit normal (3), the particle's half-length
+    along the polarisation, local mean curvature, Re/Im inside eps, cos/sin of the excitation
+    phase. Mesh-density independent (no face area) and independent of where the particle sits,
+    except through the phase."""
+    pos, nvec = cp.pos, cp.nvec
     H = mean_curvature(cp) if curvature is None else curvature
+    p = np.asarray(pol, float)
+    p = p / np.linalg.norm(p)
+    fp = cp.face_particle
+    rel = np.empty_like(pos)
+    half = np.empty(len(pos))
+    for ip in np.unique(fp):
+        idx = fp == ip
+        c = pos[idx].mean(axis=0)
+        rel[idx] = pos[idx] - c
+        s = rel[idx] @ p
+        half[idx] = 0.5 * (s.max() - s.min())
     eps_in = np.asarray(cp.eps1(enei))
     eps_med, k_med = cp.eps[medium - 1](enei)
     phase = np.real(k_med) * (pos @ np.asarray(direc, float))
-    return np.column_stack([pos, nvec, area, H, eps_in.real, eps_in.imag, np.cos(phase), np.sin(phase)])
+    return np.column_stack([rel, nvec, half, H, eps_in.real, eps_in.imag, np.cos(phase), np.sin(phase)])
 
 
 def targets(cp, sig, pol):
