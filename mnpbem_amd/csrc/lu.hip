@@ -887,6 +887,8 @@ template void rsolve_batched<float>(int, int, int, const void*, long, long, cons
 
 // ---------------------------------------------------------------------------------------------
 // Left solve with a few right-hand sides: X = A^{-1} B = U^{-1} L^{-1} (B[pf]).
+constexpr int TS_LOADS = 32;
+
 // k_trsv: one workgroup (512 lanes) per matrix solves an n <= 256 triangle in 64-row blocks:
 // the diagonal block is staged in LDS by all lanes, wave q solves right-hand side q with lane
 // shuffles (lane = row), then every lane updates the rows below (above) with the block's
@@ -947,15 +949,16 @@ __global__ __launch_bounds__(TS_THREADS) void k_trsv(int n, int r, const cx<T>* 
                 if (q < r) acc[q] = X[i + (long)q * ldx];
             const cx<T>* Ai = A + i + (long)i0 * lda;
             int c = 0;
-            for (; c + 8 <= bs; c += 8) {
-                cx<T> a8[8];
+            // 32 independent loads in flight per lane (the update is latency bound)
+            for (; c + TS_LOADS <= bs; c += TS_LOADS) {
+                cx<T> av[TS_LOADS];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) a8[u] = Ai[(long)(c + u) * lda];
+                for (int u = 0; u < TS_LOADS; ++u) av[u] = Ai[(long)(c + u) * lda];
 #pragma unroll
-                for (int u = 0; u < 8; ++u)
+                for (int u = 0; u < TS_LOADS; ++u)
 #pragma unroll
                     for (int q = 0; q < TS_MAXR; ++q)
-                        if (q < r) acc[q] = fms(acc[q], a8[u], xs[c + u][q]);
+                        if (q < r) acc[q] = fms(acc[q], av[u], xs[c + u][q]);
             }
             for (; c < bs; ++c) {
                 cx<T> a = Ai[(long)c * lda];
