@@ -255,7 +255,6 @@ def batched64():
         per.append((time.perf_counter() - t1) / len(lams))
         faces.append(cp.nfaces)
         fallbacks += len(out.get("fallback", []))
-        torch.cuda.empty_cache()
     total = time.perf_counter() - t_all
     per = np.array(per)
     return dict(workload="64 seeded particles (spheres, prolate ellipsoids, gap dimers), faces {}-{}, gold Drude in "
@@ -360,7 +359,7 @@ def main():
         "config": {"workload": "BEMRet gold-Drude capped rod 10x40 nm, n={} curved triangles, eps_out=1.77, "
                                "axial plane wave, 201 wavelengths 500-1000 nm, batch {}".format(n, args.batch),
                    "faces": n, "wavelengths": len(lams), "batch": args.batch, "nb": args.nb},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_sub<float>", "achieved": round(achieved, 3),
+        "roofline": {"bound": "mfma", "kernel": "k_gemm_sub_c64v (complex-float GEMM)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": GEMM_TRAFFIC_BYTES, "traffic_source": GEMM_TRAFFIC_SOURCE,
                      "algorithmic_bytes_per_launch": round(gemm_bytes / max(gemm_launches, 1)),

@@ -81,8 +81,9 @@ class BEMRet:
     def _get_solver(self, batch, precision):
         s = self._solver
         if s is None or s.batch != batch or s.precision != precision:
+            # the previous solver's buffers go back to torch's caching allocator and are reused
+            # (no empty_cache: re-mapping tens of GB per solver costs more than the solves)
             self._solver = None
-            torch.cuda.empty_cache()
             s = engine.RetSolverBatch(self.g.geo, batch, precision, self.nb, self.refine_tol, self.max_refine)
             self._solver = s
         return s
@@ -126,8 +127,10 @@ class BEMRet:
         return self.init(enei)
 
     def clear(self):
+        """Drops the factors and returns their memory to the device."""
         self._solver = None
         self.enei = None
+        torch.cuda.empty_cache()
         return self
 
     # ------------------------------------------------------------------ device sweep

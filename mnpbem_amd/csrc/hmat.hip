@@ -313,37 +313,62 @@ __global__ __launch_bounds__(256) void k_hm_mv_dense(const int* blk, const long*
     }
 }
 
-// tmp[toff[b] + q HM_MAXR + c] = sum_j V[j, q] x[c0 + j, c]; one wave per term q
+// tmp[toff[b] + q HM_MAXR + c] = sum_j V[j, q] x[c0 + j, c]. The block's slice of x is staged
+// through LDS in chunks of 256 rows; each wave accumulates 4 terms q at once (lanes over j) and
+// reduces across lanes once per group of 16 terms.
+constexpr int VT_Q = 4, VT_CH = 256;
 __global__ __launch_bounds__(256) void k_hm_mv_vt(const int* lr, const long* voff, const int* rank, int kind,
                                                   const cx<float>* V, int r, const cx<double>* x, long ldx,
                                                   cx<double>* tmp, const long* toff) {
-    const int b = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ cx<double> xs[VT_CH][HM_MAXR];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int c0 = lr[4 * b + 1], n = lr[4 * b + 3];
     const int rk = rank[4 * b + kind];
     const cx<float>* Vb = V + voff[b];
-    for (int q = w; q < rk; q += 4) {
-        cx<double> acc[HM_MAXR];
+    for (int g0 = 0; g0 < rk; g0 += 4 * VT_Q) {
+        const int q0 = g0 + w * VT_Q;
+        cx<double> acc[VT_Q][HM_MAXR];
 #pragma unroll
-        for (int c = 0; c < HM_MAXR; ++c) acc[c] = mk(0.0, 0.0);
-        for (int j = lane; j < n; j += 64) {
-            cx<float> v = Vb[j + (long)q * n];
+        for (int a = 0; a < VT_Q; ++a)
 #pragma unroll
-            for (int c = 0; c < HM_MAXR; ++c)
-                if (c < r) {
-                    cx<double> xv = x[c0 + j + (long)c * ldx];
-                    acc[c].re += v.re * xv.re - v.im * xv.im;
-                    acc[c].im += v.re * xv.im + v.im * xv.re;
-                }
-        }
-#pragma unroll
-        for (int c = 0; c < HM_MAXR; ++c) {
-            double sr = acc[c].re, si = acc[c].im;
-            for (int o = 32; o > 0; o >>= 1) {
-                sr += __shfl_down(sr, o, 64);
-                si += __shfl_down(si, o, 64);
+            for (int c = 0; c < HM_MAXR; ++c) acc[a][c] = mk(0.0, 0.0);
+        for (int j0 = 0; j0 < n; j0 += VT_CH) {
+            const int cnt = min(VT_CH, n - j0);
+            __syncthreads();
+            for (int e = t; e < VT_CH * HM_MAXR; e += 256) {
+                const int jj = e % VT_CH, c = e / VT_CH;
+                if (jj < cnt) xs[jj][c] = (c < r) ? x[c0 + j0 + jj + (long)c * ldx] : mk(0.0, 0.0);
             }
-            if (lane == 0 && c < r) tmp[toff[b] + (long)q * HM_MAXR + c] = mk(sr, si);
+            __syncthreads();
+            for (int jj = lane; jj < cnt; jj += 64) {
+                cx<float> v[VT_Q];
+#pragma unroll
+                for (int a = 0; a < VT_Q; ++a)
+                    v[a] = (q0 + a < rk) ? Vb[j0 + jj + (long)(q0 + a) * n] : mk(0.0f, 0.0f);
+#pragma unroll
+                for (int c = 0; c < HM_MAXR; ++c) {
+                    if (c < r) {
+                        const cx<double> xv = xs[jj][c];
+#pragma unroll
+                        for (int a = 0; a < VT_Q; ++a) {
+                            acc[a][c].re += v[a].re * xv.re - v[a].im * xv.im;
+                            acc[a][c].im += v[a].re * xv.im + v[a].im * xv.re;
+                        }
+                    }
+                }
+            }
         }
+#pragma unroll
+        for (int a = 0; a < VT_Q; ++a)
+#pragma unroll
+            for (int c = 0; c < HM_MAXR; ++c) {
+                double sr = acc[a][c].re, si = acc[a][c].im;
+                for (int o = 32; o > 0; o >>= 1) {
+                    sr += __shfl_down(sr, o, 64);
+                    si += __shfl_down(si, o, 64);
+                }
+                if (lane == 0 && c < r && q0 + a < rk) tmp[toff[b] + (long)(q0 + a) * HM_MAXR + c] = mk(sr, si);
+            }
     }
 }
 
@@ -432,7 +457,7 @@ void hm_matvec(const HMat& h, int kind, int r, const void* x, long ldx, void* y,
         const long* vo = h.voff + (long)kind * h.nlr;
         hipLaunchKernelGGL(k_hm_mv_vt, dim3(h.nlr), dim3(256), 0, s, h.lblk, vo, h.rank, kind, (const cx<float>*)h.V,
                            r, (const cx<double>*)x, ldx, (cx<double>*)h.tmp, h.toff);
-        hipLaunchKernelGGL(k_hm_mv_u, dim3(h.nlr, 4), dim3(256), 0, s, h.lblk, uo, h.rank, kind, (const cx<float>*)h.U,
+        hipLaunchKernelGGL(k_hm_mv_u, dim3(h.nlr, 1), dim3(256), 0, s, h.lblk, uo, h.rank, kind, (const cx<float>*)h.U,
                            r, (const cx<double>*)h.tmp, h.toff, (double*)y, ldy);
     }
 }

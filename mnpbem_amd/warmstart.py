@@ -225,5 +225,5 @@ def training_set(cps, lams, pol, direc, precision="mixed"):
             sig, _ = bem.solve(exc(cp, lam))
             Xs.append(features(cp, lam, pol, direc, curvature=H))
             Ts.append(targets(cp, sig, pol))
-        bem.clear()
+        del bem
     return np.vstack(Xs), np.vstack(Ts)
