@@ -889,6 +889,21 @@ template void rsolve_batched<float>(int, int, int, const void*, long, long, cons
 // Left solve with a few right-hand sides: X = A^{-1} B = U^{-1} L^{-1} (B[pf]).
 constexpr int TS_LOADS = 32;
 
+// value of v in lane c (c uniform across the wave) through scalar register reads
+__device__ __forceinline__ float lane_read(float v, int c) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), c));
+}
+__device__ __forceinline__ double lane_read(double v, int c) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffLL), c);
+    const int hi = __builtin_amdgcn_readlane((int)(u >> 32), c);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> lane_bcast(cx<T> v, int c) {
+    return mk<T>(lane_read(v.re, c), lane_read(v.im, c));
+}
+
 // k_trsv: one workgroup (512 lanes) per matrix solves an n <= 256 triangle in 64-row blocks:
 // the diagonal block is staged in LDS by all lanes, wave q solves right-hand side q with lane
 // shuffles (lane = row), then every lane updates the rows below (above) with the block's
@@ -903,6 +918,7 @@ __global__ __launch_bounds__(TS_THREADS) void k_trsv(int n, int r, const cx<T>* 
                                                      long ldx, long sX) {
     __shared__ cx<T> D[64][65];
     __shared__ cx<T> xs[64][TS_MAXR];
+    __shared__ cx<T> dinv[64];
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63;
     A += (long)b * sA;
     X += (long)b * sX;
@@ -916,21 +932,24 @@ __global__ __launch_bounds__(TS_THREADS) void k_trsv(int n, int r, const cx<T>* 
             D[rr][cc] = (rr < bs && cc < bs) ? A[(i0 + rr) + (long)(i0 + cc) * lda] : mk<T>(0, 0);
         }
         __syncthreads();
-        // one wave per right-hand side
+        if (!LOWER && t < bs) dinv[t] = crecip(D[t][t]);
+        __syncthreads();
+        // one wave per right-hand side; the solved entry is broadcast with a register read of
+        // lane c (no LDS round trip on the dependency chain), U's diagonal by its reciprocal
         const int wave = t >> 6;
         for (int q = wave; q < r; q += TS_THREADS / 64) {
             {
                 cx<T> v = lane < bs ? X[i0 + lane + (long)q * ldx] : mk<T>(0, 0);
                 if (LOWER) {
                     for (int c = 0; c < bs; ++c) {
-                        T xr = __shfl(v.re, c, 64), xi = __shfl(v.im, c, 64);
-                        if (lane > c) v = fms(v, D[lane][c], mk(xr, xi));
+                        const cx<T> xc = lane_bcast(v, c);
+                        if (lane > c) v = fms(v, D[lane][c], xc);
                     }
                 } else {
                     for (int c = bs - 1; c >= 0; --c) {
-                        if (lane == c) v = cdiv(v, D[c][c]);
-                        T xr = __shfl(v.re, c, 64), xi = __shfl(v.im, c, 64);
-                        if (lane < c) v = fms(v, D[lane][c], mk(xr, xi));
+                        if (lane == c) v = v * dinv[c];
+                        const cx<T> xc = lane_bcast(v, c);
+                        if (lane < c) v = fms(v, D[lane][c], xc);
                     }
                 }
                 if (lane < bs) {
