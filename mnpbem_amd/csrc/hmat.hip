@@ -457,7 +457,7 @@ void hm_matvec(const HMat& h, int kind, int r, const void* x, long ldx, void* y,
         const long* vo = h.voff + (long)kind * h.nlr;
         hipLaunchKernelGGL(k_hm_mv_vt, dim3(h.nlr), dim3(256), 0, s, h.lblk, vo, h.rank, kind, (const cx<float>*)h.V,
                            r, (const cx<double>*)x, ldx, (cx<double>*)h.tmp, h.toff);
-        hipLaunchKernelGGL(k_hm_mv_u, dim3(h.nlr, 1), dim3(256), 0, s, h.lblk, uo, h.rank, kind, (const cx<float>*)h.U,
+        hipLaunchKernelGGL(k_hm_mv_u, dim3(h.nlr, 4), dim3(256), 0, s, h.lblk, uo, h.rank, kind, (const cx<float>*)h.U,
                            r, (const cx<double>*)h.tmp, h.toff, (double*)y, ldy);
     }
 }
