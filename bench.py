@@ -302,6 +302,7 @@ def main():
         solvers["double"] = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="double", green=g)
 
     results = {}
+    solvers_tuning = {}
     for prec in solvers:
         for _ in range(args.warmup):
             sweep(prec)
@@ -312,6 +313,7 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         results[prec] = (dt, out)
+        solvers_tuning[prec] = solvers[prec].tuning
         solvers[prec]._solver = None if prec == "double" else solvers[prec]._solver
         torch.cuda.empty_cache()
 
@@ -358,7 +360,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "BEMRet gold-Drude capped rod 10x40 nm, n={} curved triangles, eps_out=1.77, "
                                "axial plane wave, 201 wavelengths 500-1000 nm, batch {}".format(n, args.batch),
-                   "faces": n, "wavelengths": len(lams), "batch": args.batch, "nb": args.nb},
+                   "faces": n, "wavelengths": len(lams), "batch": args.batch,
+                   "lu_tuning_mixed": solvers_tuning.get("mixed"), "lu_tuning_fp64": solvers_tuning.get("double")},
         "roofline": {"bound": "mfma", "kernel": "k_gemm_sub_c64v (complex-float GEMM)", "achieved": round(achieved, 3),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": GEMM_TRAFFIC_BYTES, "traffic_source": GEMM_TRAFFIC_SOURCE,

@@ -264,6 +264,16 @@ int mnp_mlp_adam(long count, float* p, const float* g, float* m, float* v, doubl
 /* Test hook: result registers of one v_mfma_f32_32x32x8bf16_1k on known operands (64 x 16 floats). */
 int mnp_probe_mfma_bf16(float* out, void* stream);
 
+
+/* ---------------------------------------------------------------------------------------------
+ * LU tuning knobs (process wide, host values; no counterpart in the reference). key 0: outer
+ * block width of the blocked LU and right-side solves (16..1024 columns, default 128); 1:
+ * lookahead (trailing update split over two streams) on/off; 2: register-resident panel for
+ * complex float on/off; 3: panel height from which the multi-workgroup panel is used (>= 256).
+ * mnpbem_amd.autotune picks them per problem shape by timing. */
+int mnp_set_tuning(int key, int value);
+int mnp_get_tuning(int key, int* value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,82 @@
+"""Autotuner for the blocked LU of the BEMRet chain.
+
+The batched LU (lu.hip) has three shape parameters that trade panel latency against GEMM
+efficiency: the panel width ``nb`` (16 or 32 columns), the outer block width (the K of the big
+trailing GEMM: 64, 128 or 256) and lookahead (trailing update split over two streams). Which
+combination is fastest depends on n, the batch and the precision. ``tune_lu`` times every
+candidate once on random matrices of the problem's shape (one untimed call first) and keeps the
+fastest for the process; ``knobs`` applies a choice around a factorisation and restores the
+previous settings afterwards, so other callers keep theirs.
+"""
+import contextlib
+import ctypes
+import time
+
+import torch
+
+from ._lib import call
+
+_CACHE = {}
+_KEYS = {"nbo": 0, "lookahead": 1}
+
+
+def _get(key):
+    v = ctypes.c_int(0)
+    call("mnp_get_tuning", key, ctypes.addressof(v))
+    return v.value
+
+
+@contextlib.contextmanager
+def knobs(cfg):
+    """Set the library's LU knobs from cfg (keys 'nbo', 'lookahead') for the duration."""
+    if not cfg:
+        yield
+        return
+    old = {k: _get(i) for k, i in _KEYS.items()}
+    try:
+        for k, i in _KEYS.items():
+            if k in cfg:
+                call("mnp_set_tuning", i, int(cfg[k]))
+        yield
+    finally:
+        for k, i in _KEYS.items():
+            call("mnp_set_tuning", i, old[k])
+
+
+def candidates():
+    return [dict(nb=nb, nbo=nbo, lookahead=la) for nb in (16, 32) for nbo in (64, 128, 256) for la in (0, 1)
+            if nbo >= nb]
+
+
+def tune_lu(n, batch, dtype, dev, reps=1, seed=0):
+    """Fastest (nb, nbo, lookahead) for `batch` LUs of size n in `dtype` (complex64 or
+    complex128); cached per (n, batch, dtype). Returns the config with its time in ms."""
+    key = (int(n), int(batch), str(dtype))
+    if key in _CACHE:
+        return _CACHE[key]
+    prec = 0 if dtype == torch.complex128 else 1
+    g = torch.Generator(device=dev).manual_seed(seed)
+    A = torch.randn((batch, n, n), dtype=dtype, device=dev, generator=g)
+    W = torch.empty_like(A)
+    ipiv = torch.empty((batch, n), dtype=torch.int32, device=dev)
+    info = torch.empty(batch, dtype=torch.int32, device=dev)
+    st = torch.cuda.current_stream(dev).cuda_stream
+    best = None
+    for cfg in candidates():
+        with knobs(cfg):
+            times = []
+            for rep in range(reps + 1):
+                W.copy_(A)
+                torch.cuda.synchronize(dev)
+                t0 = time.perf_counter()
+                call("mnp_getrf_batched", prec, n, batch, W.data_ptr(), n, n * n, ipiv.data_ptr(), n,
+                     info.data_ptr(), cfg["nb"], st)
+                torch.cuda.synchronize(dev)
+                if rep:
+                    times.append(time.perf_counter() - t0)
+        ms = 1e3 * min(times)
+        if best is None or ms < best["ms"]:
+            best = dict(cfg, ms=round(ms, 3))
+    del A, W
+    _CACHE[key] = best
+    return best

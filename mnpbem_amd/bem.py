@@ -9,7 +9,8 @@ wavelength sweep, batching wavelengths.
 Options: ``precision`` = 'double' (default, complex128 LU chain as the reference),
 'mixed' (complex64 LU chain + FP64 iterative refinement on the exact operator) or
 'single'; ``batch`` = wavelengths factored together; ``nb`` = LU panel width;
-``refine_tol``, ``max_refine`` for the refinement.
+``refine_tol``, ``max_refine`` for the refinement; ``autotune`` = pick the LU shape (panel
+width, outer block width, lookahead) by timing (default: on when batch >= 8).
 """
 import numpy as np
 import torch
@@ -68,6 +69,9 @@ class BEMRet:
         self.nb = int(op.get("nb", 32))
         self.refine_tol = float(op.get("refine_tol", 1e-12))
         self.max_refine = int(op.get("max_refine", 12))
+        # LU shape autotuning (autotune.py): default on for batched sweeps (batch >= 8)
+        self.autotune = op.get("autotune", None)
+        self.tuning = None
         self.g = op.get("green") or CompGreenRet(p, p, **{k: v for k, v in op.items()
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
         p._mnp_geo = self.g.geo
@@ -84,7 +88,14 @@ class BEMRet:
             # the previous solver's buffers go back to torch's caching allocator and are reused
             # (no empty_cache: re-mapping tens of GB per solver costs more than the solves)
             self._solver = None
-            s = engine.RetSolverBatch(self.g.geo, batch, precision, self.nb, self.refine_tol, self.max_refine)
+            nb, tuning = self.nb, None
+            if (self.autotune if self.autotune is not None else batch >= 8):
+                from . import autotune
+                fdt = torch.complex128 if precision == "double" else torch.complex64
+                tuning = autotune.tune_lu(self.g.geo.n, batch, fdt, self.g.geo.dev)
+                nb = tuning["nb"]
+            self.tuning = tuning
+            s = engine.RetSolverBatch(self.g.geo, batch, precision, nb, self.refine_tol, self.max_refine, tuning)
             self._solver = s
         return s
 

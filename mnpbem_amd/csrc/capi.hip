@@ -64,6 +64,8 @@ void mlp_colsum(int, int, const float*, long, float*, hipStream_t);
 void mlp_mse(long, const float*, const float*, float*, float*, hipStream_t);
 void mlp_adam(long, float*, const float*, float*, float*, float, float, float, float, int, hipStream_t);
 void mlp_probe(float*, hipStream_t);
+int set_tuning(int, int);
+int get_tuning(int);
 }  // namespace mnp
 
 using namespace mnp;
@@ -562,6 +564,20 @@ int mnp_probe_mfma_bf16(float* out, void* stream) {
     REQUIRE(out, 1);
     mlp_probe(out, (hipStream_t)stream);
     return launch_status();
+}
+
+
+// ------------------------------------------------------------------------- tuning knobs
+int mnp_set_tuning(int key, int value) {
+    REQUIRE(key >= 0 && key <= 3, 1);
+    return set_tuning(key, value) == 0 ? 0 : -2;
+}
+
+int mnp_get_tuning(int key, int* value) {
+    REQUIRE(key >= 0 && key <= 3, 1);
+    REQUIRE(value, 2);
+    *value = get_tuning(key);
+    return 0;
 }
 
 }  // extern "C"

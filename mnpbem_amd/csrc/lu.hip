@@ -627,7 +627,36 @@ static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, 
 // Two-level blocking: outer blocks of NBO columns carry the large trailing update (a GEMM with
 // k = NBO, so C is read and written once per NBO columns), inner panels of nb columns are
 // factored by k_panel inside the outer block.
-constexpr int NBO = 128;
+static int g_nbo = getenv("MNP_LU_NBO") ? atoi(getenv("MNP_LU_NBO")) : 128;
+static bool g_lookahead = getenv("MNP_LU_LOOKAHEAD") ? atoi(getenv("MNP_LU_LOOKAHEAD")) != 0 : true;
+static int coop_min_rows() {
+    if (g_coop_min_rows < 0) {
+        const char* e = getenv("MNP_PANEL_COOP_MIN_ROWS");
+        g_coop_min_rows = e ? atoi(e) : 4096;
+    }
+    return g_coop_min_rows;
+}
+
+// Tuning knobs (process wide): 0 = outer block width (columns, >= nb), 1 = lookahead on/off,
+// 2 = register-resident panel on/off, 3 = rows from which panels use the cooperative kernel.
+int set_tuning(int key, int value) {
+    switch (key) {
+        case 0: if (value < 16 || value > 1024) return -2; g_nbo = value; return 0;
+        case 1: g_lookahead = value != 0; return 0;
+        case 2: g_panel_reg = value != 0; return 0;
+        case 3: if (value < 256) return -2; g_coop_min_rows = value; return 0;
+        default: return -1;
+    }
+}
+int get_tuning(int key) {
+    switch (key) {
+        case 0: return g_nbo;
+        case 1: return g_lookahead ? 1 : 0;
+        case 2: return g_panel_reg ? 1 : 0;
+        case 3: return coop_min_rows();
+        default: return -1;
+    }
+}
 
 // Lookahead: the trailing update of an outer block is split into the next block's columns
 // (on the caller's stream) and the rest (on a second stream, default priority). The rest runs
@@ -651,11 +680,11 @@ template <typename T>
 void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                    hipStream_t s) {
     cx<T>* a = (cx<T>*)A;
-    const int NB = (NBO > nb) ? (NBO / nb) * nb : nb;
+    const int NB = (g_nbo > nb) ? (g_nbo / nb) * nb : nb;
     CoopWS ws{};
     const bool coop = coop_prepare<T>(n, batch, s, ws);
     SideStream& side = side_stream();
-    static const bool lookahead = getenv("MNP_LU_LOOKAHEAD") ? atoi(getenv("MNP_LU_LOOKAHEAD")) != 0 : true;
+    const bool lookahead = g_lookahead;
     bool pending = false;   // a rest update is running on the side stream
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int W = (n - J0 < NB) ? (n - J0) : NB;
@@ -834,7 +863,7 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
     const cx<T>* a = (const cx<T>*)LU;
     cx<T>* bm = (cx<T>*)B;
     dim3 g((m + 127) / 128, batch);
-    const int NB = (NBO > nb) ? (NBO / nb) * nb : nb;
+    const int NB = (g_nbo > nb) ? (g_nbo / nb) * nb : nb;
     auto upper = [&](int j0, int w) {
         if (nb <= 32)
             hipLaunchKernelGGL((k_rtrsm_upper<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);

@@ -154,10 +154,11 @@ class RetSolverBatch:
     exact complex128 operator; wavelengths whose refinement stalls are redone in 'double').
     """
 
-    def __init__(self, geo, batch, precision="double", nb=32, refine_tol=1e-12, max_refine=12):
+    def __init__(self, geo, batch, precision="double", nb=32, refine_tol=1e-12, max_refine=12, tuning=None):
         if precision not in ("double", "single", "mixed"):
             raise ValueError("precision must be 'double', 'single' or 'mixed'")
         self.geo, self.batch, self.precision, self.nb = geo, batch, precision, nb
+        self.tuning = tuning          # LU knobs applied around the factorisation (autotune)
         self.refine_tol, self.max_refine = refine_tol, max_refine
         n, d = geo.n, geo.dev
         fdt = C128 if precision == "double" else C64
@@ -190,9 +191,11 @@ class RetSolverBatch:
             self.exact.assemble(self.kmat, nl)
         self.fac.assemble(self.kmat, nl)
         f = self.fac
-        call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
-             _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info), _p(g.nvec),
-             _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
+        from .autotune import knobs
+        with knobs(self.tuning):
+            call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
+                 _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info), _p(g.nvec),
+                 _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
         info = self.check_info()
         if np.any(info < 0):
             raise RuntimeError("LU panel synchronisation timed out (info {}); factors are invalid".format(info))

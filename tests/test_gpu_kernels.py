@@ -212,3 +212,36 @@ def test_large_single_matrix_factor_and_solve():
     berr = float(torch.linalg.vector_norm(r) / (torch.linalg.matrix_norm(A) * torch.linalg.vector_norm(x)))
     print("backward error", berr)
     assert berr < 2e-6
+
+
+def test_autotune_picks_a_candidate_and_restores_knobs():
+    from mnpbem_amd import autotune
+    import ctypes
+    banner("lu autotune n=300 batch 4")
+    before = []
+    for key in (0, 1):
+        v = ctypes.c_int(0)
+        _lib().call("mnp_get_tuning", key, ctypes.addressof(v))
+        before.append(v.value)
+    cfg = autotune.tune_lu(300, 4, torch.complex64, _dev())
+    assert {k: cfg[k] for k in ("nb", "nbo", "lookahead")} in autotune.candidates()
+    assert cfg["ms"] > 0
+    after = []
+    for key in (0, 1):
+        v = ctypes.c_int(0)
+        _lib().call("mnp_get_tuning", key, ctypes.addressof(v))
+        after.append(v.value)
+    assert after == before
+    # a factorisation under every candidate gives LAPACK's pivots on a random matrix
+    rng = np.random.default_rng(9)
+    A = _rand(rng, 300, 300)
+    lu, piv = sla.lu_factor(A)
+    for c in autotune.candidates():
+        t = _cm([A], torch.complex128)
+        ipiv = torch.zeros((1, 300), dtype=torch.int32, device=_dev())
+        info = torch.zeros(1, dtype=torch.int32, device=_dev())
+        with autotune.knobs(c):
+            _lib().call("mnp_getrf_batched", 0, 300, 1, t.data_ptr(), 300, 300 * 300, ipiv.data_ptr(), 300,
+                        info.data_ptr(), c["nb"], _sh())
+        assert np.array_equal(ipiv[0].cpu().numpy(), piv)
+        assert np.max(np.abs(_np(t, 0) - lu)) < 1e-10 * np.max(np.abs(lu))
