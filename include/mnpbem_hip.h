@@ -274,6 +274,15 @@ int mnp_probe_mfma_bf16(float* out, void* stream);
 int mnp_set_tuning(int key, int value);
 int mnp_get_tuning(int key, int* value);
 
+
+/* Potentials and fields on the faces from a solution, for one region (outside = 0 inside with
+ * sigma1/h1, 1 outside with sigma2/h2): CompGreenRet.potential and .field
+ * (greenfun/compgreen_ret.py:912-1040) for particles whose regions do not nest. sig: n complex,
+ * h: n x 3 complex (row-major), k0: vacuum wavenumber; out: n x 14 complex per face
+ * (phi, phip, a[3], ap[3], e[3], h[3]). Entries come from g (mnp_green_geom) on the fly. */
+int mnp_surface_eval(const mnp_green_geom* g, int outside, double k0, const void* sig, const void* h, void* out,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,7 @@ _SIGS = {
     "mnp_mlp_adam": [L, P, P, P, P, C.c_double, C.c_double, C.c_double, C.c_double, I, P],
     "mnp_probe_mfma_bf16": [P, P],
     "mnp_set_tuning": [I, I],
+    "mnp_surface_eval": [P, I, C.c_double, P, P, P, P],
     "mnp_get_tuning": [I, P],
 }
 

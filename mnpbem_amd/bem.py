@@ -134,6 +134,16 @@ class BEMRet:
     def __truediv__(self, exc):
         return self.solve(exc)[0]
 
+    def potential(self, sig, inout=2):
+        """Potentials and surface derivatives of solution ``sig`` outside (inout=2, default) or
+        inside (1) of the particle surface (bem_ret.py:1833-1859, via CompGreenRet.potential)."""
+        return self.g.potential(sig, inout)
+
+    def field(self, sig, inout=2):
+        """Electric and magnetic fields of solution ``sig`` outside (inout=2, default) or
+        inside (1) of the particle surface (bem_ret.py:1861-1888, via CompGreenRet.field)."""
+        return self.g.field(sig, inout)
+
     def __call__(self, enei):
         return self.init(enei)
 

@@ -276,6 +276,16 @@ class BEMRetIter:
     def __truediv__(self, exc):
         return self.solve(exc)[0]
 
+    def potential(self, sig, inout=2):
+        """Potentials and surface derivatives of solution ``sig`` outside (inout=2, default) or
+        inside (1) of the particle surface (bem_ret.py:1833-1859, via CompGreenRet.potential)."""
+        return self.g.potential(sig, inout)
+
+    def field(self, sig, inout=2):
+        """Electric and magnetic fields of solution ``sig`` outside (inout=2, default) or
+        inside (1) of the particle surface (bem_ret.py:1861-1888, via CompGreenRet.field)."""
+        return self.g.field(sig, inout)
+
     def info(self):
         """(flag, relres, iter) of the last solve (bem_iter.py:191-194)."""
         return self._flag, self._relres, self._iter

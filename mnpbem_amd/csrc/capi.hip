@@ -580,4 +580,16 @@ int mnp_get_tuning(int key, int* value) {
     return 0;
 }
 
+
+// ------------------------------------------------------------------------- surface potentials / fields
+int mnp_surface_eval(const mnp_green_geom* g, int outside, double k0, const void* sig, const void* h, void* out,
+                     void* stream) {
+    REQUIRE(geom_ok(g), 1);
+    REQUIRE(outside == 0 || outside == 1, 2);
+    REQUIRE(k0 > 0, 3);
+    REQUIRE(sig && h && out, 4);
+    surface_eval(to_geom(g), outside, k0, sig, h, out, (hipStream_t)stream);
+    return launch_status();
+}
+
 }  // extern "C"

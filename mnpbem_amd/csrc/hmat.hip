@@ -426,6 +426,115 @@ __global__ void k_hm_zero(long count, cx<double>* z) {
     if (i < count) z[i] = mk(0.0, 0.0);
 }
 
+// ---------------------------------------------------------------------------- surface fields
+// G and the Cartesian derivative Gp of pair (i, j) in region `outside` (0 inside, 1 outside):
+//   far:     G = a_j/d e^{ikd},  Gp = (x_i - x_j) (ik - 1/d)/d^2 a_j e^{ikd}
+//   refined: G = sum g_n (ik)^n e^{ikd},  Gp = sum f_n (ik)^n e^{ikd}   (greenret_refined.py:812-844)
+__device__ void gp_entry(const GreenGeom& a, int outside, int i, int j, cx<double>& G, cx<double> (&Gp)[3]) {
+    G = mk(0.0, 0.0);
+    Gp[0] = Gp[1] = Gp[2] = mk(0.0, 0.0);
+    const int pa = a.fpart[i], pb = a.fpart[j];
+    const int m = outside ? a.con_out[pa * a.npart + pb] : a.con_in[pa * a.npart + pb];
+    if (m <= 0) return;
+    const cx<double> k = reinterpret_cast<const cx<double>*>(a.kmat)[m - 1];
+    const double rx = a.pos[3 * i] - a.pos[3 * j];
+    const double ry = a.pos[3 * i + 1] - a.pos[3 * j + 1];
+    const double rz = a.pos[3 * i + 2] - a.pos[3 * j + 2];
+    const double d = sqrt(rx * rx + ry * ry + rz * rz);
+    int lo = a.rowptr[i], hi = a.rowptr[i + 1] - 1, p = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = a.rcol[mid];
+        if (c == j) {
+            p = mid;
+            break;
+        }
+        if (c < j) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    const cx<double> ph = expikd(k, d);
+    if (p >= 0) {
+        const int no = a.order + 1;
+        const cx<double> ik = mk(-k.im, k.re);
+        cx<double> sg = mk(0.0, 0.0), sf[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+        for (int q = no - 1; q >= 0; --q) {
+            sg = ik * sg + mk(a.rg[(long)p * no + q], 0.0);
+            for (int c = 0; c < 3; ++c) sf[c] = ik * sf[c] + mk(a.rf[((long)p * 3 + c) * no + q], 0.0);
+        }
+        G = sg * ph;
+        for (int c = 0; c < 3; ++c) Gp[c] = sf[c] * ph;
+    } else if (i != j) {
+        const double inv_d = 1.0 / d, aj = a.area[j];
+        G = (aj * inv_d) * ph;
+        const cx<double> f = (aj * inv_d * inv_d) * (mk(-k.im - inv_d, k.re) * ph);
+        Gp[0] = rx * f;
+        Gp[1] = ry * f;
+        Gp[2] = rz * f;
+    }
+}
+
+// One wave per observer face i: potentials phi = G s, phip = H s, a = G h, ap = H h and fields
+// e = i k0 G h - Hp s, h = sum_j Hp_ij x h_j of region `outside` (H = n_i.Gp +- 2 pi on the
+// self term, Hp = Gp +- 2 pi n_i; + inside, - outside), CompGreenRet.potential / .field
+// (compgreen_ret.py:912-1040). s: n complex, hv: n x 3 complex (row-major); out: n x 14 complex
+// (phi, phip, a[3], ap[3], e[3], h[3]).
+__global__ __launch_bounds__(256) void k_surface_eval(GreenGeom a, int outside, double k0, const cx<double>* sg,
+                                                      const cx<double>* hv, cx<double>* out) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.n) return;
+    const double nx = a.nvec[3 * i], ny = a.nvec[3 * i + 1], nz = a.nvec[3 * i + 2];
+    const double s2pi = outside ? -HM_TWO_PI : HM_TWO_PI;
+    cx<double> acc[14];
+#pragma unroll
+    for (int q = 0; q < 14; ++q) acc[q] = mk(0.0, 0.0);
+    for (int j = lane; j < a.n; j += 64) {
+        cx<double> G, Gp[3];
+        gp_entry(a, outside, i, j, G, Gp);
+        const int pa = a.fpart[i], pb = a.fpart[j];
+        const int m = outside ? a.con_out[pa * a.npart + pb] : a.con_in[pa * a.npart + pb];
+        cx<double> H = nx * Gp[0] + ny * Gp[1] + nz * Gp[2];
+        cx<double> P[3] = {Gp[0], Gp[1], Gp[2]};
+        if (i == j && m > 0) {
+            H.re += s2pi;
+            P[0].re += s2pi * nx;
+            P[1].re += s2pi * ny;
+            P[2].re += s2pi * nz;
+        }
+        const cx<double> s = sg[j];
+        const cx<double> h[3] = {hv[3 * j], hv[3 * j + 1], hv[3 * j + 2]};
+        acc[0] = acc[0] + G * s;
+        acc[1] = acc[1] + H * s;
+        for (int c = 0; c < 3; ++c) {
+            acc[2 + c] = acc[2 + c] + G * h[c];
+            acc[5 + c] = acc[5 + c] + H * h[c];
+            acc[8 + c] = acc[8 + c] + P[c] * s;
+        }
+        acc[11] = acc[11] + (P[1] * h[2] - P[2] * h[1]);
+        acc[12] = acc[12] + (P[2] * h[0] - P[0] * h[2]);
+        acc[13] = acc[13] + (P[0] * h[1] - P[1] * h[0]);
+    }
+#pragma unroll
+    for (int q = 0; q < 14; ++q)
+        for (int o = 32; o > 0; o >>= 1) {
+            acc[q].re += __shfl_down(acc[q].re, o, 64);
+            acc[q].im += __shfl_down(acc[q].im, o, 64);
+        }
+    if (lane == 0) {
+        cx<double>* o = out + 14L * i;
+        for (int q = 0; q < 8; ++q) o[q] = acc[q];
+        const cx<double> ik0 = mk(0.0, k0);
+        for (int c = 0; c < 3; ++c) o[8 + c] = ik0 * acc[2 + c] - acc[8 + c];
+        for (int c = 0; c < 3; ++c) o[11 + c] = acc[11 + c];
+    }
+}
+
+void surface_eval(const GreenGeom& a, int outside, double k0, const void* s, const void* hv, void* out,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_surface_eval, dim3((a.n + 3) / 4), dim3(256), 0, st, a, outside, k0, (const cx<double>*)s,
+                       (const cx<double>*)hv, (cx<double>*)out);
+}
+
 // ---------------------------------------------------------------------------- launchers
 void hm_fill_dense(const GreenGeom& a, const int* c2p, int nd, const int* blk, const long* off, void* D, long kstride,
                    hipStream_t s) {

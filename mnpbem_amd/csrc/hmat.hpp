@@ -55,5 +55,7 @@ void hm_compact(int nlr, const int* lr, int kind, const int* rank, const long* u
 void hm_matvec(const HMat& h, int kind, int r, const void* x, long ldx, void* y, long ldy, hipStream_t s);
 void bem_operator_hmat(const HMat& h, const Face& f, const void* x, const void* rhs, void* y, void* work,
                        double* norms, hipStream_t s);
+void surface_eval(const GreenGeom& a, int outside, double k0, const void* s, const void* hv, void* out,
+                  hipStream_t st);
 
 }  // namespace mnp

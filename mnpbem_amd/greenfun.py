@@ -99,6 +99,63 @@ class CompGreenRet:
             m = m + (4 * np.pi if key == "H1" else -4 * np.pi) * np.eye(m.shape[0])
         return m
 
+    # ------------------------------------------------------------ potentials and fields
+    def _surface_eval(self, sig, inout):
+        """Per polarisation, the 14 complex columns (phi, phip, a, ap, e, h) of region ``inout``
+        on the faces, from the kernel that evaluates G, Gp and H entry-wise (no matrices). With
+        non-nested particles only the sources of the same region contribute (the cross-region
+        connectivities are empty), so inside uses (sig1, h1) and outside (sig2, h2)."""
+        from ._lib import call, GreenGeomC
+        from .engine import stream_handle
+        from .hmatrix import C_addr
+        if inout not in (1, 2):
+            raise ValueError("inout must be 1 (inside) or 2 (outside)")
+        geo, enei = self.geo, float(sig.enei)
+        s = np.asarray(sig.sig1 if inout == 1 else sig.sig2, complex)
+        h = np.asarray(sig.h1 if inout == 1 else sig.h2, complex)
+        n = geo.n
+        if s.shape[0] != n or h.shape[:2] != (n, 3):
+            raise ValueError("surface charges must have n = {} rows and currents shape (n, 3[, npol])".format(n))
+        single = s.ndim == 1
+        s2 = s.reshape(n, -1)
+        h3 = h.reshape(n, 3, -1)
+        npol = s2.shape[1]
+        if h3.shape[2] != npol:
+            raise ValueError("charges and currents have different numbers of polarisations")
+        dev = geo.dev
+        kmat = geo.media([enei])[0][0].contiguous()
+        rowptr, rcol, rg, rf = geo.refined_csr()
+        g = GreenGeomC(n, geo.npart, geo.order, *[t.data_ptr() for t in (
+            geo.pos, geo.nvec, geo.area, geo.fpart, geo.con_in, geo.con_out, kmat, rowptr, rcol, rg, rf)])
+        sd = torch.tensor(np.ascontiguousarray(s2.T), dtype=torch.complex128, device=dev)
+        hd = torch.tensor(np.ascontiguousarray(np.transpose(h3, (2, 0, 1))), dtype=torch.complex128, device=dev)
+        out = torch.empty((npol, n, 14), dtype=torch.complex128, device=dev)
+        st = stream_handle()
+        for q in range(npol):
+            call("mnp_surface_eval", C_addr(g), inout - 1, 2 * np.pi / enei, sd[q].data_ptr(), hd[q].data_ptr(),
+                 out[q].data_ptr(), st)
+        o = np.transpose(out.cpu().numpy(), (1, 2, 0))     # (n, 14, npol)
+        del kmat
+        if single:
+            o = o[:, :, 0]
+        return o
+
+    def potential(self, sig, inout=1):
+        """Potentials and surface derivatives inside (inout=1) or outside (inout=2) of the
+        particle from a BEMRet solution (compgreen_ret.py:975-1034): CompStruct with
+        phi1, phi1p, a1, a1p (or phi2, ...)."""
+        o = self._surface_eval(sig, inout)
+        t = str(inout)
+        return CompStruct(self.p1, sig.enei, **{"phi" + t: o[:, 0], "phi" + t + "p": o[:, 1],
+                                                 "a" + t: o[:, 2:5], "a" + t + "p": o[:, 5:8]})
+
+    def field(self, sig, inout=1):
+        """Electric and magnetic fields inside (inout=1) or outside (inout=2) of the particle
+        surface from a BEMRet solution (compgreen_ret.py:911-973): CompStruct with e, h
+        (n, 3[, npol])."""
+        o = self._surface_eval(sig, inout)
+        return CompStruct(self.p1, sig.enei, e=o[:, 8:11], h=o[:, 11:14])
+
 
 class CompGreenStat:
     """Quasistatic Green function G = a_j / d and surface derivative

@@ -343,10 +343,17 @@ class Green:
         return np.where(a == b, a, 0)
 
     def block(self, k, key):
-        """G or F for all faces at wavenumber k (greenret_refined.py:654-810)."""
+        """G, F or the Cartesian derivative Gp (n, 3, n) for all faces at wavenumber k
+        (greenret_refined.py:654-866)."""
         s = self.s
         x = s.pos[:, None, :] - s.pos[None, :, :]
         d = np.maximum(np.sqrt(np.sum(x * x, axis=2)), EPSF)
+        if key == "Gp":
+            ikn = (1j * k) ** np.arange(self.order + 1)
+            M = x * ((1j * k - 1 / d) / d ** 2 * s.area[None, :])[:, :, None]
+            for (i, j), (g, f) in self.coef.items():
+                M[i, j] = f @ ikn
+            return np.transpose(M * np.exp(1j * k * d)[:, :, None], (0, 2, 1))
         if key == "G":
             M = s.area[None, :] / d + 0j
         else:
@@ -357,18 +364,30 @@ class Green:
         return M * np.exp(1j * k * d)
 
     def eval(self, r1, r2, key, ks):
-        """Region pair (r1, r2) matrix; ks = wavenumber per material (1-based index - 1)."""
+        """Region pair (r1, r2) matrix; ks = wavenumber per material (1-based index - 1).
+        Keys G, F, H1, H2 -> (n, n); Gp, H1p, H2p -> (n, 3, n) (H1p/H2p = Gp +- 2 pi n_i on
+        the self terms, greenret_refined.py:846-863)."""
         con = self.con(r1, r2)
-        out = np.zeros((self.s.n, self.s.n), complex)
+        vec = key in ("Gp", "H1p", "H2p")
+        n = self.s.n
+        out = np.zeros((n, 3, n) if vec else (n, n), complex)
         any_block = False
         for a in range(con.shape[0]):
             for b in range(con.shape[1]):
                 if con[a, b] == 0:
                     continue
                 any_block = True
-                base = "G" if key == "G" else "F"
+                base = "Gp" if vec else ("G" if key == "G" else "F")
                 M = self.block(ks[con[a, b] - 1], base)
                 ia, ib = np.nonzero(self.fp == a)[0], np.nonzero(self.fp == b)[0]
+                if vec:
+                    blk = M[np.ix_(ia, np.arange(3), ib)]
+                    if key in ("H1p", "H2p") and a == b:
+                        sg = TWO_PI if key == "H1p" else -TWO_PI
+                        for q, i in enumerate(ia):
+                            blk[q, :, q] += sg * self.s.nvec[i]
+                    out[np.ix_(ia, np.arange(3), ib)] = blk
+                    continue
                 blk = M[np.ix_(ia, ib)]
                 if key in ("H1", "H2") and a == b:
                     blk = blk + (TWO_PI if key == "H1" else -TWO_PI) * np.eye(len(ia))
@@ -608,3 +627,36 @@ def mie_sphere(enei, diameter, eps_in, eps_out, lmax=20):
     ext = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * np.real(an + bn))
     sca = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * (np.abs(an) ** 2 + np.abs(bn) ** 2))
     return float(ext), float(sca)
+
+
+def surface_potential_field(green, ks, enei, sig1, h1, sig2, h2, inout):
+    """Potentials and fields on the faces from a solution (CompGreenRet.potential and .field,
+    greenfun/compgreen_ret.py:912-1040): region inout (1 inside, 2 outside), sources of both
+    regions. Returns phi, phip (n), a, ap, e, h (n, 3)."""
+    r = inout - 1
+    k0 = TWO_PI / enei
+    hkey, pkey = ("H1", "H1p") if inout == 1 else ("H2", "H2p")
+
+    def mm(M, x):
+        if isinstance(M, int):
+            return 0
+        return np.tensordot(M, x, axes=([M.ndim - 1], [0]))
+
+    G1, G2 = green.eval(r, 0, "G", ks), green.eval(r, 1, "G", ks)
+    H1, H2 = green.eval(r, 0, hkey, ks), green.eval(r, 1, hkey, ks)
+    P1, P2 = green.eval(r, 0, pkey, ks), green.eval(r, 1, pkey, ks)
+    phi = mm(G1, sig1) + mm(G2, sig2)
+    phip = mm(H1, sig1) + mm(H2, sig2)
+    a = mm(G1, h1) + mm(G2, h2)
+    ap = mm(H1, h1) + mm(H2, h2)
+    e = 1j * k0 * a - mm(P1, sig1) - mm(P2, sig2)
+
+    def cross(P, h):
+        if isinstance(P, int):
+            return 0
+        return np.stack([mm(P[:, 1, :], h[:, 2]) - mm(P[:, 2, :], h[:, 1]),
+                         mm(P[:, 2, :], h[:, 0]) - mm(P[:, 0, :], h[:, 2]),
+                         mm(P[:, 0, :], h[:, 1]) - mm(P[:, 1, :], h[:, 0])], axis=1)
+
+    hf = cross(P1, h1) + cross(P2, h2)
+    return dict(phi=phi, phip=phip, a=a, ap=ap, e=e, h=hf)

@@ -1,0 +1,91 @@
+"""Surface potentials and fields of a BEMRet solution on the GPU (CompGreenRet.potential /
+.field, compgreen_ret.py:911-1034) against the oracle's dense evaluation."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, banner
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import bem_oracle as O                                       # noqa: E402
+import mnpbem_amd as M                                                   # noqa: E402
+
+DRUDE = (10.0, 9.065, 0.0708)
+GOLD = np.loadtxt(os.path.join(ROOT, "mnpbem_amd", "data", "gold.dat"), comments="%")
+
+
+def _setup(parts, inout, eps):
+    cp = M.ComParticle(eps, parts, inout, *range(1, len(parts) + 1), interp="curv")
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+    fp = cp.face_particle
+    closed = [(np.nonzero(fp == a)[0], np.nonzero(fp == a)[0], 1.0) for a in range(len(parts))]
+    return cp, O.Green(s, fp, inout, closed)
+
+
+def _check(cp, green, lam, pol, direc, tol=1e-10):
+    bem = M.BEMRet(cp)
+    exc = M.PlaneWaveRet(pol, direc)
+    sig, _ = bem.solve(exc.potential(cp, lam))
+    ks = [e(lam)[1] for e in cp.eps]
+    for inout in (1, 2):
+        ref = O.surface_potential_field(green, ks, lam, sig.sig1, sig.h1, sig.sig2, sig.h2, inout)
+        pot = bem.potential(sig, inout)
+        fld = bem.field(sig, inout)
+        t = str(inout)
+        got = dict(phi=pot["phi" + t], phip=pot["phi" + t + "p"], a=pot["a" + t], ap=pot["a" + t + "p"],
+                   e=fld.e, h=fld.h)
+        for key, want in ref.items():
+            err = np.max(np.abs(got[key] - want)) / np.max(np.abs(want))
+            assert err < tol, (inout, key, err)
+    return sig
+
+
+def test_fields_sphere_matches_oracle():
+    cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+    banner("surface fields sphere n={}".format(cp.nfaces))
+    for lam in (450.0, 520.0, 700.0):
+        _check(cp, green, lam, [1.0, 0, 0], [0, 0, 1.0])
+
+
+def test_fields_dimer_two_polarisations():
+    a = M.trisphere(60, 10)
+    b = M.trisphere(60, 10).shift([0, 0, 11.0])
+    cp, green = _setup([a, b], [[2, 1], [2, 1]], [M.EpsConst(1.33), M.EpsDrude(*DRUDE)])
+    banner("surface fields dimer n={} npol=2".format(cp.nfaces))
+    lam = 650.0
+    bem = M.BEMRet(cp)
+    exc = M.PlaneWaveRet(np.array([[1.0, 0, 0], [0, 0, 1.0]]), np.array([[0, 0, 1.0], [1.0, 0, 0]]))
+    sig, _ = bem.solve(exc.potential(cp, lam))
+    ks = [e(lam)[1] for e in cp.eps]
+    fld = bem.field(sig)                                   # outside by default
+    pot = bem.potential(sig, 1)
+    for ip in range(2):
+        ref2 = O.surface_potential_field(green, ks, lam, sig.sig1[:, ip], sig.h1[:, :, ip], sig.sig2[:, ip],
+                                         sig.h2[:, :, ip], 2)
+        ref1 = O.surface_potential_field(green, ks, lam, sig.sig1[:, ip], sig.h1[:, :, ip], sig.sig2[:, ip],
+                                         sig.h2[:, :, ip], 1)
+        for got, want in ((fld.e[:, :, ip], ref2["e"]), (fld.h[:, :, ip], ref2["h"]),
+                          (pot.phi1[:, ip], ref1["phi"]), (pot.a1p[:, :, ip], ref1["ap"])):
+            assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
+
+
+def test_fields_satisfy_boundary_conditions():
+    """Device fields alone: normal D is continuous to solver accuracy, tangential E to the
+    discretisation error."""
+    cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+    lam = 520.0
+    sig = _check(cp, green, lam, [1.0, 0, 0], [0, 0, 1.0])
+    bem = M.BEMRet(cp)
+    e1, e2 = bem.field(sig, 1).e, bem.field(sig, 2).e
+    n = cp.nvec
+    k = cp.eps[0](lam)[1]
+    e_inc = np.exp(1j * k * cp.pos[:, 2])[:, None] * np.array([1.0, 0, 0])
+    dn1 = cp.eps1(lam) * np.sum(n * e1, axis=1)
+    dn2 = cp.eps2(lam) * np.sum(n * (e2 + e_inc), axis=1)
+    assert np.linalg.norm(dn1 - dn2) < 1e-8 * np.linalg.norm(dn1)
+    assert np.linalg.norm(np.cross(n, e1 - e2 - e_inc)) < 0.01 * np.linalg.norm(e_inc)
