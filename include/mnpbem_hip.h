@@ -283,6 +283,19 @@ int mnp_get_tuning(int key, int* value);
 int mnp_surface_eval(const mnp_green_geom* g, int outside, double k0, const void* sig, const void* h, void* out,
                      void* stream);
 
+
+/* Entries of one region-pair Green function by key (0 G, 1 F, 2 H1, 3 H2, 4 Gp, 5 H1p,
+ * 6 H2p) from g on the fly, for the diagonal region pairs of non-nested particles
+ * (outside = 0: inside-inside, 1: outside-outside).
+ * mnp_green_block: columns [c0, c0 + ncol), row-major (n, ncol) complex, or (n, 3, ncol) for
+ *   keys 4-6 -- CompGreenRet.eval_block (greenfun/compgreen_ret.py:555-737) and eval of the
+ *   derivative keys (:507-553).
+ * mnp_green_pick: entries at row-major linear indices ind[q] = i * n + j (device int64),
+ *   out (count) or (count, 3) complex -- CompGreenRet.eval(..., ind) (:860-910). */
+int mnp_green_block(const mnp_green_geom* g, int outside, int key, int c0, int ncol, void* out, void* stream);
+int mnp_green_pick(const mnp_green_geom* g, int outside, int key, const long* ind, long count, void* out,
+                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,8 @@ _SIGS = {
     "mnp_probe_mfma_bf16": [P, P],
     "mnp_set_tuning": [I, I],
     "mnp_surface_eval": [P, I, C.c_double, P, P, P, P],
+    "mnp_green_block": [P, I, I, I, I, P, P],
+    "mnp_green_pick": [P, I, I, P, C.c_long, P, P],
     "mnp_get_tuning": [I, P],
 }
 

@@ -592,4 +592,27 @@ int mnp_surface_eval(const mnp_green_geom* g, int outside, double k0, const void
     return launch_status();
 }
 
+
+// ------------------------------------------------------------------------- entries by key
+int mnp_green_block(const mnp_green_geom* g, int outside, int key, int c0, int ncol, void* out, void* stream) {
+    REQUIRE(geom_ok(g), 1);
+    REQUIRE(outside == 0 || outside == 1, 2);
+    REQUIRE(key >= 0 && key <= 6, 3);
+    REQUIRE(c0 >= 0 && ncol >= 0 && (long)c0 + ncol <= g->n, 4);
+    REQUIRE(out || ncol == 0, 6);
+    green_block(to_geom(g), outside, key, c0, ncol, out, (hipStream_t)stream);
+    return launch_status();
+}
+
+int mnp_green_pick(const mnp_green_geom* g, int outside, int key, const long* ind, long count, void* out,
+                   void* stream) {
+    REQUIRE(geom_ok(g), 1);
+    REQUIRE(outside == 0 || outside == 1, 2);
+    REQUIRE(key >= 0 && key <= 6, 3);
+    REQUIRE(count >= 0, 5);
+    REQUIRE((ind && out) || count == 0, 4);
+    green_pick(to_geom(g), outside, key, ind, count, out, (hipStream_t)stream);
+    return launch_status();
+}
+
 }  // extern "C"

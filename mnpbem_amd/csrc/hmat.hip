@@ -535,6 +535,76 @@ void surface_eval(const GreenGeom& a, int outside, double k0, const void* s, con
                        (const cx<double>*)hv, (cx<double>*)out);
 }
 
+// ---------------------------------------------------------------------------- entries by key
+// Key of one region-pair entry: 0 G, 1 F, 2 H1 = F + 2 pi, 3 H2 = F - 2 pi (self terms), 4 Gp,
+// 5 H1p = Gp + 2 pi n_i, 6 H2p = Gp - 2 pi n_i (greenret_refined.py:654-866). Writes 1 value
+// (keys 0-3) or 3 values at stride `cstride` (keys 4-6).
+__device__ void key_entry(const GreenGeom& a, int outside, int key, int i, int j, cx<double>* o, long cstride) {
+    cx<double> G, Gp[3];
+    gp_entry(a, outside, i, j, G, Gp);
+    double self = 0.0;
+    if (i == j && (key == 2 || key == 3 || key == 5 || key == 6)) {
+        const int pa = a.fpart[i];
+        const int m = outside ? a.con_out[pa * a.npart + pa] : a.con_in[pa * a.npart + pa];
+        if (m > 0) self = (key == 2 || key == 5) ? HM_TWO_PI : -HM_TWO_PI;
+    }
+    const double nx = a.nvec[3 * i], ny = a.nvec[3 * i + 1], nz = a.nvec[3 * i + 2];
+    if (key == 0) {
+        o[0] = G;
+    } else if (key < 4) {
+        cx<double> F = nx * Gp[0] + ny * Gp[1] + nz * Gp[2];
+        F.re += self;
+        o[0] = F;
+    } else {
+        Gp[0].re += self * nx;
+        Gp[1].re += self * ny;
+        Gp[2].re += self * nz;
+        o[0] = Gp[0];
+        o[cstride] = Gp[1];
+        o[2 * cstride] = Gp[2];
+    }
+}
+
+// Columns [c0, c0 + ncol) of the n-row matrix of `key`, row-major: (n, ncol) for keys 0-3,
+// (n, 3, ncol) for keys 4-6 (CompGreenRet.eval / eval_block, compgreen_ret.py:507-737).
+__global__ __launch_bounds__(256) void k_green_block(GreenGeom a, int outside, int key, int c0, int ncol,
+                                                     cx<double>* out) {
+    const long e = (long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (long)a.n * ncol) return;
+    const int i = (int)(e / ncol), jj = (int)(e % ncol);
+    if (key < 4) key_entry(a, outside, key, i, c0 + jj, out + e, 0);
+    else key_entry(a, outside, key, i, c0 + jj, out + (long)i * 3 * ncol + jj, ncol);
+}
+
+// Entries at row-major linear indices ind[q] = i * n + j: out (count) or (count, 3).
+__global__ __launch_bounds__(256) void k_green_pick(GreenGeom a, int outside, int key, const long* ind, long count,
+                                                    cx<double>* out) {
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= count) return;
+    const long e = ind[q];
+    if (e < 0 || e >= (long)a.n * a.n) {   // out of range: NaN, never an access outside the geometry
+        const cx<double> nan = mk(__builtin_nan(""), __builtin_nan(""));
+        for (int c = 0; c < (key < 4 ? 1 : 3); ++c) out[(key < 4 ? 1 : 3) * q + c] = nan;
+        return;
+    }
+    const int i = (int)(e / a.n), j = (int)(e % a.n);
+    if (key < 4) key_entry(a, outside, key, i, j, out + q, 0);
+    else key_entry(a, outside, key, i, j, out + 3 * q, 1);
+}
+
+void green_block(const GreenGeom& a, int outside, int key, int c0, int ncol, void* out, hipStream_t st) {
+    const long tot = (long)a.n * ncol;
+    if (tot == 0) return;
+    hipLaunchKernelGGL(k_green_block, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, a, outside, key, c0, ncol,
+                       (cx<double>*)out);
+}
+
+void green_pick(const GreenGeom& a, int outside, int key, const long* ind, long count, void* out, hipStream_t st) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k_green_pick, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, a, outside, key, ind,
+                       count, (cx<double>*)out);
+}
+
 // ---------------------------------------------------------------------------- launchers
 void hm_fill_dense(const GreenGeom& a, const int* c2p, int nd, const int* blk, const long* off, void* D, long kstride,
                    hipStream_t s) {

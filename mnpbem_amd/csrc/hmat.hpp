@@ -57,5 +57,7 @@ void bem_operator_hmat(const HMat& h, const Face& f, const void* x, const void* 
                        double* norms, hipStream_t s);
 void surface_eval(const GreenGeom& a, int outside, double k0, const void* s, const void* hv, void* out,
                   hipStream_t st);
+void green_block(const GreenGeom& a, int outside, int key, int c0, int ncol, void* out, hipStream_t st);
+void green_pick(const GreenGeom& a, int outside, int key, const long* ind, long count, void* out, hipStream_t st);
 
 }  // namespace mnp

@@ -75,15 +75,38 @@ class CompGreenRet:
             self._geo = DeviceGeometry(self.p1, self.nearfield, device())
         return self._geo
 
-    def eval(self, i, j, key, enei):
-        """Region-pair matrix (n, n) complex128 (host numpy), or 0 when the regions do not
-        connect (compgreen_ret.py:507-553). Keys: 'G', 'F', 'H1', 'H2'."""
-        if key not in ("G", "F", "H1", "H2"):
+    _KEYS = ("G", "F", "H1", "H2", "Gp", "H1p", "H2p")
+
+    def _struct(self, enei):
+        """Geometry descriptor for entry-wise kernels at one wavelength: (struct, keep-alive)."""
+        from ._lib import GreenGeomC
+        geo = self.geo
+        kmat = geo.media([float(enei)])[0][0].contiguous()
+        rowptr, rcol, rg, rf = geo.refined_csr()
+        g = GreenGeomC(geo.n, geo.npart, geo.order, *[t.data_ptr() for t in (
+            geo.pos, geo.nvec, geo.area, geo.fpart, geo.con_in, geo.con_out, kmat, rowptr, rcol, rg, rf)])
+        return g, kmat
+
+    def _check(self, i, j, key):
+        if key not in self._KEYS:
             raise ValueError("Unknown key: {}".format(key))
         if not np.any(self.con[i][j]):
-            return 0
+            return False
         if i != j:
             raise NotImplementedError("cross-region Green functions are not provided")
+        return True
+
+    def eval(self, i, j, key, enei, ind=None):
+        """Region-pair matrix, or 0 when the regions do not connect (compgreen_ret.py:507-553).
+        Keys 'G', 'F', 'H1', 'H2' give (n, n) complex128 host arrays, 'Gp', 'H1p', 'H2p' give
+        (n, 3, n). With ``ind`` (row-major linear indices i * n + j) only those entries:
+        (len(ind),) or (len(ind), 3)."""
+        if ind is not None:
+            return self._pick(i, j, key, enei, ind)
+        if not self._check(i, j, key):
+            return 0
+        if key in ("Gp", "H1p", "H2p"):
+            return self.eval_block(i, j, key, enei, 0, self.geo.n)
         gb = GreenBatch(self.geo, 1, torch.complex128)
         kmat = self.geo.media([enei])[0]
         gb.assemble(kmat, 1)
@@ -99,13 +122,57 @@ class CompGreenRet:
             m = m + (4 * np.pi if key == "H1" else -4 * np.pi) * np.eye(m.shape[0])
         return m
 
+    def eval_block(self, i, j, key, enei, col_start, col_stop):
+        """Columns [col_start, col_stop) of ``eval(i, j, key, enei)`` without the full matrix
+        (compgreen_ret.py:555-737): (n, ncol), or (n, 3, ncol) for the derivative keys; zeros
+        when the regions do not connect."""
+        from ._lib import call
+        from .engine import stream_handle
+        from .hmatrix import C_addr
+        c0, c1 = int(col_start), int(col_stop)
+        n = self.geo.n
+        if not (0 <= c0 <= c1 <= n):
+            raise AssertionError("[error] eval_block col range ({}, {}) out of bounds for p2.n={}".format(c0, c1, n))
+        vec = key in ("Gp", "H1p", "H2p")
+        shape = (n, 3, c1 - c0) if vec else (n, c1 - c0)
+        if not self._check(i, j, key):
+            return np.zeros(shape, complex)
+        g, keep = self._struct(enei)
+        out = torch.empty(shape, dtype=torch.complex128, device=self.geo.dev)
+        call("mnp_green_block", C_addr(g), i, self._KEYS.index(key), c0, c1 - c0, out.data_ptr(), stream_handle())
+        res = out.cpu().numpy()
+        del keep
+        return res
+
+    def _pick(self, i, j, key, enei, ind):
+        """Entries at row-major linear indices (compgreen_ret.py:860-910)."""
+        from ._lib import call
+        from .engine import stream_handle
+        from .hmatrix import C_addr
+        ind = np.asarray(ind, dtype=np.int64).ravel()
+        n = self.geo.n
+        vec = key in ("Gp", "H1p", "H2p")
+        shape = (len(ind), 3) if vec else (len(ind),)
+        if len(ind) and (ind.min() < 0 or ind.max() >= n * n):
+            raise IndexError("linear index out of range for a {} x {} matrix".format(n, n))
+        if not self._check(i, j, key) or len(ind) == 0:
+            return np.zeros(shape, complex)
+        g, keep = self._struct(enei)
+        d_ind = torch.tensor(ind, device=self.geo.dev)
+        out = torch.empty(shape, dtype=torch.complex128, device=self.geo.dev)
+        call("mnp_green_pick", C_addr(g), i, self._KEYS.index(key), d_ind.data_ptr(), len(ind), out.data_ptr(),
+             stream_handle())
+        res = out.cpu().numpy()
+        del keep
+        return res
+
     # ------------------------------------------------------------ potentials and fields
     def _surface_eval(self, sig, inout):
         """Per polarisation, the 14 complex columns (phi, phip, a, ap, e, h) of region ``inout``
         on the faces, from the kernel that evaluates G, Gp and H entry-wise (no matrices). With
         non-nested particles only the sources of the same region contribute (the cross-region
         connectivities are empty), so inside uses (sig1, h1) and outside (sig2, h2)."""
-        from ._lib import call, GreenGeomC
+        from ._lib import call
         from .engine import stream_handle
         from .hmatrix import C_addr
         if inout not in (1, 2):
@@ -123,10 +190,7 @@ class CompGreenRet:
         if h3.shape[2] != npol:
             raise ValueError("charges and currents have different numbers of polarisations")
         dev = geo.dev
-        kmat = geo.media([enei])[0][0].contiguous()
-        rowptr, rcol, rg, rf = geo.refined_csr()
-        g = GreenGeomC(n, geo.npart, geo.order, *[t.data_ptr() for t in (
-            geo.pos, geo.nvec, geo.area, geo.fpart, geo.con_in, geo.con_out, kmat, rowptr, rcol, rg, rf)])
+        g, kmat = self._struct(enei)
         sd = torch.tensor(np.ascontiguousarray(s2.T), dtype=torch.complex128, device=dev)
         hd = torch.tensor(np.ascontiguousarray(np.transpose(h3, (2, 0, 1))), dtype=torch.complex128, device=dev)
         out = torch.empty((npol, n, 14), dtype=torch.complex128, device=dev)

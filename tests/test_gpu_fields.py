@@ -89,3 +89,42 @@ def test_fields_satisfy_boundary_conditions():
     dn2 = cp.eps2(lam) * np.sum(n * (e2 + e_inc), axis=1)
     assert np.linalg.norm(dn1 - dn2) < 1e-8 * np.linalg.norm(dn1)
     assert np.linalg.norm(np.cross(n, e1 - e2 - e_inc)) < 0.01 * np.linalg.norm(e_inc)
+
+
+@pytest.mark.parametrize("parts", ["sphere", "dimer"])
+def test_eval_keys_blocks_and_indices(parts):
+    """CompGreenRet.eval for all seven keys (G, F, H1, H2 from the assembly; Gp, H1p, H2p
+    entry-wise) against the oracle; eval_block = column slices of eval; eval(..., ind) =
+    picked entries; regions that do not connect give 0 / zeros."""
+    if parts == "sphere":
+        cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+    else:
+        a = M.trisphere(60, 10)
+        b = M.trisphere(60, 10).shift([0, 0, 11.0])
+        cp, green = _setup([a, b], [[2, 1], [2, 1]], [M.EpsConst(1.33), M.EpsDrude(*DRUDE)])
+    lam = 580.0
+    banner("eval keys {} n={}".format(parts, cp.nfaces))
+    g = M.CompGreenRet(cp, cp)
+    ks = [e(lam)[1] for e in cp.eps]
+    n = cp.nfaces
+    rng = np.random.default_rng(3)
+    ind = rng.integers(0, n * n, 500)
+    ind[:n // 4] = np.arange(n // 4) * (n + 1)                   # include self terms
+    for r in (0, 1):
+        for key in ("G", "F", "H1", "H2", "Gp", "H1p", "H2p"):
+            want = green.eval(r, r, key, ks)
+            got = g.eval(r, r, key, lam)
+            assert got.shape == want.shape
+            assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-9, (r, key)
+            blk = g.eval_block(r, r, key, lam, 37, 151)
+            assert np.max(np.abs(blk - got[..., 37:151])) <= 1e-12 * np.max(np.abs(got))
+            pick = g.eval(r, r, key, lam, ind=ind)
+            full = got if got.ndim == 2 else np.transpose(got, (0, 2, 1))
+            ref = full.reshape(n * n, -1)[ind].reshape(pick.shape)
+            assert np.max(np.abs(pick - ref)) <= 1e-12 * np.max(np.abs(got))
+    assert g.eval(0, 1, "G", lam) == 0
+    assert not np.any(g.eval_block(1, 0, "H1p", lam, 0, 10))
+    with pytest.raises(AssertionError):
+        g.eval_block(0, 0, "G", lam, 5, n + 1)
+    with pytest.raises(ValueError):
+        g.eval(0, 0, "Q", lam)
