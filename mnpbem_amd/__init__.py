@@ -11,9 +11,10 @@ from .comparticle import ComParticle
 from .greenfun import CompGreenRet, CompGreenStat, CompStruct
 from .bem import BEMRet
 from .bem_stat import BEMStat, PlaneWaveStat
+from .bem_stat_iter import ACACompGreenStat, BEMStatIter
 from .planewave import PlaneWaveRet, SpectrumRet
 
 __all__ = ["EpsConst", "EpsDrude", "EpsTable", "Particle", "trisphere", "trirod", "trispheresegment",
            "triellipsoid", "fvgrid", "ComParticle", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMStat",
-           "PlaneWaveStat", "PlaneWaveRet",
+           "BEMStatIter", "ACACompGreenStat", "PlaneWaveStat", "PlaneWaveRet",
            "SpectrumRet"]

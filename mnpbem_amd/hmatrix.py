@@ -176,14 +176,20 @@ class HMatrixRet:
     (``htol``, ``kmax``) in complex128 and stored as complex64 factors.
     ``stats`` reports block counts, ranks, compression and bytes per 8-column product."""
 
-    def __init__(self, geo, tree, enei, htol=1e-6, kmax=100, eta=2.5, fadmiss=None):
+    def __init__(self, geo, tree, enei, htol=1e-6, kmax=100, eta=2.5, fadmiss=None, static=False,
+                 kinds=(0, 1, 2, 3)):
+        """``static=True``: the quasistatic kernels (k = 0, every particle pair connected; G1 = G2
+        = G, H1 = F + 2 pi, H2 = F - 2 pi) as ACACompGreenStat (aca_compgreen_stat.py), with
+        ``enei`` ignored. ``kinds``: which of G1, G2, H1, H2 get low-rank factors (the others
+        keep rank 0 in the admissible blocks, so only their dense blocks are valid)."""
         import torch
         from . import engine
         from ._lib import call, GreenGeomC, HMatC
         self.geo, self.tree, self.enei = geo, tree, float(enei)
         self.htol, self.kmax = float(htol), int(kmax)
+        self.static, self.kinds = bool(static), tuple(int(q) for q in kinds)
         if fadmiss is None:
-            fadmiss = make_kaware_fadmiss(2 * np.pi / enei, eta)
+            fadmiss = make_kaware_fadmiss(0.0 if static else 2 * np.pi / enei, eta)
         blocks = block_partition(tree, fadmiss, eta, max_lowrank=ACA_MAXDIM)
         d = geo.dev
         n = geo.n
@@ -210,10 +216,14 @@ class HMatrixRet:
         self.rank = torch.zeros((max(self.nlr, 1), 4), dtype=torch.int32, device=d)
         kmat, self.L1, self.L2, self.kvac, _, _ = geo.media([enei])
         self.kmat = kmat[0].contiguous()
+        con_in, con_out = geo.con_in, geo.con_out
+        if self.static:
+            self.kmat = torch.zeros_like(self.kmat)
+            con_in = con_out = torch.ones_like(geo.con_in)
         rowptr, rcol, rg, rf = geo.refined_csr()
-        self._keep = (rowptr, rcol, rg, rf)
+        self._keep = (rowptr, rcol, rg, rf, con_in, con_out)
         g = GreenGeomC(n, geo.npart, geo.order, *[t.data_ptr() for t in (
-            geo.pos, geo.nvec, geo.area, geo.fpart, geo.con_in, geo.con_out, self.kmat, rowptr, rcol, rg, rf)])
+            geo.pos, geo.nvec, geo.area, geo.fpart, con_in, con_out, self.kmat, rowptr, rcol, rg, rf)])
         self._g = g
         st = engine.stream_handle()
         gp = C_addr(g)
@@ -229,6 +239,10 @@ class HMatrixRet:
             Vw = torch.empty(int(vws[-1]), dtype=torch.complex128, device=d)
             Us, Vs, ubase, vbase = [], [], [0], [0]
             for kind in range(4):
+                if kind not in self.kinds:
+                    uoff[kind] = ubase[-1]
+                    voff[kind] = vbase[-1]
+                    continue
                 # ACA of this kind, then its complex64 copy (the workspace is reused by the next kind)
                 call("mnp_hmat_aca", gp, self.c2p.data_ptr(), kind, self.nlr, self.lblk.data_ptr(),
                      uws_t.data_ptr(), vws_t.data_ptr(), Uw.data_ptr(), Vw.data_ptr(), self.rank.data_ptr(),

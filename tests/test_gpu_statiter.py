@@ -1,0 +1,96 @@
+"""BEMStatIter and ACACompGreenStat on the GPU: iterative = direct (reference test
+test_bem_iter.py:50-135 at the same tolerances), the MATLAB quasistatic table, H-matrix F
+against the dense F."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, banner
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import mnpbem_amd as M                                                   # noqa: E402
+
+DRUDE = (10.0, 9.065, 0.0708)
+
+
+def _sphere(nf=144, r=10.0):
+    return M.ComParticle([M.EpsConst(1.0), M.EpsDrude(*DRUDE)], [M.trisphere(nf, r)], [[2, 1]], 1)
+
+
+def test_dense_iterative_matches_direct():
+    cp = _sphere()
+    banner("bemstatiter dense sphere n={}".format(cp.nfaces))
+    exc = M.PlaneWaveStat(np.array([[0.0, 0.0, 1.0]]))(cp, 550.0)
+    sd, _ = M.BEMStat(cp).solve(exc)
+    it = M.BEMStatIter(cp, solver="gmres", tol=1e-8, maxit=300, precond="hmat")
+    si, _ = it.solve(exc)
+    assert np.abs(sd.sig - si.sig).max() / np.abs(sd.sig).max() < 1e-8
+    # unpreconditioned GMRES reaches the same solution
+    un = M.BEMStatIter(cp, tol=1e-10, maxit=300, precond=None)
+    su, _ = un.solve(exc)
+    assert np.abs(sd.sig - su.sig).max() / np.abs(sd.sig).max() < 1e-8
+    flag, relres, its = un.info()
+    assert len(flag) == 1 and flag[0] == 0 and relres[0] < 1e-10 and its[0][0] > 1
+
+
+def test_info_and_precond_only():
+    cp = _sphere()
+    exc = M.PlaneWaveStat(np.array([[0.0, 0.0, 1.0]]))(cp, 500.0)
+    it = M.BEMStatIter(cp, solver="gmres", tol=1e-6, maxit=200, precond="hmat")
+    it.solve(exc)
+    flag, relres, _ = it.info()
+    assert len(flag) == 1 and flag[0] == 0 and relres[0] < 1e-5
+    sd, _ = M.BEMStat(cp).solve(M.PlaneWaveStat(np.array([[0.0, 0.0, 1.0]]))(cp, 550.0))
+    p0 = M.BEMStatIter(cp, solver="gmres", maxit=0, precond="hmat")
+    s0, _ = p0.solve(M.PlaneWaveStat(np.array([[0.0, 0.0, 1.0]]))(cp, 550.0))
+    assert np.abs(sd.sig - s0.sig).max() / np.abs(sd.sig).max() < 1e-10
+
+
+def test_matlab_table_two_polarisations():
+    """validation/02_bemstat/sphere through the iterative solver (complex64 preconditioner,
+    GMRES to 1e-11)."""
+    ml = np.loadtxt(os.path.join(GOLDEN, "bemstat_sphere_matlab.csv"), delimiter=",", skiprows=1)
+    cp = M.ComParticle([M.EpsConst(1.0), M.EpsTable("gold.dat")], [M.trisphere(144, 20)], [[2, 1]], 1,
+                       interp="curv")
+    banner("bemstatiter matlab table")
+    it = M.BEMStatIter(cp, tol=1e-11, maxit=100, precond_precision="single")
+    exc = M.PlaneWaveStat([[1.0, 0, 0], [0, 1.0, 0]])
+    for row in ml[::10]:
+        sig, _ = it.solve(exc(cp, row[0]))
+        assert sig.sig.shape == (cp.nfaces, 2)
+        ext = exc.extinction(sig)
+        assert abs(ext[0] - row[1]) / row[1] < 1e-9
+        assert abs(ext[1] - row[1]) / row[1] < 2e-2           # y equals x up to the mesh's asymmetry
+    assert max(it.info()[0]) == 0
+
+
+def test_hmatrix_F_and_solution():
+    cp = _sphere(400, 10.0)
+    n = cp.nfaces
+    banner("bemstatiter hmatrix sphere n={}".format(n))
+    dense = M.CompGreenStat(cp, cp)
+    aca = M.ACACompGreenStat(cp, htol=1e-6, kmax=100, cleaf=32)
+    F, G = aca.eval("F", "G")
+    print("compression F {:.3f} G {:.3f}".format(F.compression(), G.compression()))
+    assert F.compression() < 1.0 and G.compression() < 1.0
+    Fd, Gd = dense.F, dense.G
+    assert np.linalg.norm(F.full() - Fd) / np.linalg.norm(Fd) < 1e-5
+    assert np.linalg.norm(G.full() - Gd) / np.linalg.norm(Gd) < 1e-5
+    x = np.random.default_rng(0).standard_normal((n, 3)) + 0j
+    assert np.linalg.norm(F @ x - Fd @ x) / np.linalg.norm(Fd @ x) < 1e-5
+    H2 = aca.eval("H2").full()
+    assert np.allclose(H2, Fd - 2 * np.pi * np.eye(n), rtol=0, atol=1e-5 * np.abs(Fd).max())
+    exc = M.PlaneWaveStat(np.array([[1.0, 0.0, 0.0]]))(cp, 520.0)
+    sd, _ = M.BEMStat(cp).solve(exc)
+    for pc in ("auto", "none"):
+        it = M.BEMStatIter(cp, hmatrix=True, htol=1e-6, cleaf=32, tol=1e-8, maxit=400, preconditioner=pc)
+        si, _ = it.solve(exc)
+        assert np.abs(sd.sig - si.sig).max() / np.abs(sd.sig).max() < 1e-4, pc
+        assert it.info()[0][-1] == 0
+    pot = it.potential(si, 1)
+    ref = Gd @ si.sig
+    assert np.abs(pot.phi1 - ref).max() / np.abs(ref).max() < 1e-5
