@@ -4,11 +4,14 @@
 Workload: gold-Drude capped rod (10 nm diameter x 40 nm, curved triangles, 2,496 faces) in
 eps = 1.77, plane wave polarised along the rod axis, 201 wavelengths over 500-1000 nm. One
 step = the full sweep (assembly, factorisation, solve, far field, extinction for every
-wavelength), batched `--batch` wavelengths per factorisation, inputs resident on the GPU.
+wavelength), `--batch` wavelengths per factorisation (default all 201 at once: about 150 GB
+of HBM for either precision), inputs resident on the GPU.
 
-value = wavelength solves/s of the mixed-precision path (complex64 LU chain on the matrix
-cores + FP64 iterative refinement on the exact operator); the complex128 LU path is timed
-in the same run as the reference point. Prints one JSON line.
+value = wavelength solves/s of the mixed-precision path (complex64 LU chain whose GEMMs run
+on the BF16 matrix cores with operands split into 3 bfloat16 terms, as accurate as FP32, +
+FP64 iterative refinement on the exact operator); the complex128 LU path is timed in the same
+run, at the same batch, as the reference point, and the 2-term variant is reported beside it.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -23,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s @2.5k faces mixed vs FP64 LU; 25k-face GMRES time-to-1e-6"
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP64_TFLOPS = 78.6
 DRUDE = (10.0, 9.065, 0.0708)
 ROD_N = [25, 11, 34]
@@ -271,7 +275,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=0, help="wavelengths per factorisation (0: all 201 at once)")
+    ap.add_argument("--gemm-split", type=int, default=3, choices=(0, 2, 3),
+                    help="bfloat16 terms per operand in the complex-float GEMMs (0: FP32 matrix cores)")
+    ap.add_argument("--no-mixed2", action="store_true", help="skip the extra 2-term mixed sweep")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="copy one 256 MiB buffer first (a kernel of known traffic for counter scaling)")
     ap.add_argument("--nb", type=int, default=32)
     ap.add_argument("--cpu-sample", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -287,9 +296,17 @@ def main():
 
     import torch
     from mnpbem_amd import _lib
+    if args.pmc_calibration:
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     M, cp, exc = build_workload()
     n = cp.nfaces
     lams = np.linspace(500, 1000, 201)
+    if args.batch <= 0:
+        args.batch = len(lams)
     torch.cuda.set_device(0)
     g = M.CompGreenRet(cp, cp)
 
@@ -297,7 +314,8 @@ def main():
         bem = solvers[prec]
         return bem.solve_spectrum(exc, lams, precision=prec)
 
-    solvers = {"mixed": M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g)}
+    solvers = {"mixed": M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g,
+                                 gemm_split=args.gemm_split)}
     if not args.skip_fp64:
         solvers["double"] = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="double", green=g)
 
@@ -314,7 +332,7 @@ def main():
         dt = time.perf_counter() - t0
         results[prec] = (dt, out)
         solvers_tuning[prec] = solvers[prec].tuning
-        solvers[prec]._solver = None if prec == "double" else solvers[prec]._solver
+        solvers[prec]._solver = None          # the next precision gets the memory (large batches)
         torch.cuda.empty_cache()
 
     # GEMM (dominant kernel) timing with HIP events over one extra mixed sweep
@@ -325,7 +343,12 @@ def main():
     _lib.call("mnp_prof_collect", prof.ctypes.data)
     _lib.call("mnp_prof_enable", 0)
     gemm_ms, gemm_flops, gemm_launches, gemm_bytes = prof
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    # algorithmic (complex, FP32-equivalent) flops of the GEMM launches over their kernel time
+    effective = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    # the split-BF16 kernel executes split * (split + 1) / 2 BF16 products per real product
+    products = {0: 1, 2: 3, 3: 6}[args.gemm_split]
+    achieved = effective * products
+    peak = PEAK_BF16_TFLOPS if args.gemm_split else PEAK_FP32_TFLOPS
 
     dt_mixed, out_mixed = results["mixed"]
     value = args.steps * len(lams) / dt_mixed
@@ -336,6 +359,23 @@ def main():
         rel = np.max(np.abs(out_mixed["ext"] - out_d["ext"]) / np.abs(out_d["ext"]))
         extra = dict(fp64_solves_per_s=round(v_d, 3), speedup_vs_fp64=round(value / v_d, 3),
                      ext_rel_err_vs_fp64=float(rel))
+    if not args.no_mixed2 and args.gemm_split != 2 and "double" in results:
+        # the same sweep with 2-term operands (about 16 significant bits in the GEMMs)
+        b2 = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g, gemm_split=2)
+        b2.solve_spectrum(exc, lams, precision="mixed")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out2 = b2.solve_spectrum(exc, lams, precision="mixed")
+        torch.cuda.synchronize()
+        dt2 = time.perf_counter() - t0
+        extra["mixed_2term_solves_per_s"] = round(len(lams) / dt2, 3)
+        extra["mixed_2term_speedup_vs_fp64"] = round(len(lams) / dt2 / extra["fp64_solves_per_s"], 3)
+        extra["mixed_2term_ext_rel_err_vs_fp64"] = float(np.max(np.abs(out2["ext"] - results["double"][1]["ext"]) /
+                                                                np.abs(results["double"][1]["ext"])))
+        extra["mixed_2term_refine_iterations_max"] = int(max(out2.get("refine_iterations", [0]) or [0]))
+        b2._solver = None
+        del b2
+        torch.cuda.empty_cache()
     flops_per_solve = 34.67 * n ** 3
     extra["path_tflops_mixed"] = round(value * flops_per_solve / 1e12, 3)
     extra["refine_iterations_max"] = int(max(out_mixed.get("refine_iterations", [0]) or [0]))
@@ -356,14 +396,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64+c128-refine",
+        "dtype": "c64 LU ({}) + c128 refinement".format(
+            {0: "FP32 MFMA", 2: "2-term split-BF16 MFMA", 3: "3-term split-BF16 MFMA, FP32-equivalent"}[args.gemm_split]),
         "data": "synthetic",
         "config": {"workload": "BEMRet gold-Drude capped rod 10x40 nm, n={} curved triangles, eps_out=1.77, "
                                "axial plane wave, 201 wavelengths 500-1000 nm, batch {}".format(n, args.batch),
                    "faces": n, "wavelengths": len(lams), "batch": args.batch,
                    "lu_tuning_mixed": solvers_tuning.get("mixed"), "lu_tuning_fp64": solvers_tuning.get("double")},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm_sub_c64v (complex-float GEMM)", "achieved": round(achieved, 3),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "roofline": {"bound": "mfma",
+                     "kernel": ("k_gemm_sub_c64p<{},1,1> (complex GEMM on BF16 matrix cores, executed BF16 "
+                                "flops)".format(args.gemm_split) if args.gemm_split else
+                                "k_gemm_sub_c64v (complex-float GEMM)"),
+                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "effective_complex_tflops": round(effective, 3),
+                     "effective_frac_of_fp32_peak": round(effective / PEAK_FP32_TFLOPS, 4),
                      "traffic": GEMM_TRAFFIC_BYTES, "traffic_source": GEMM_TRAFFIC_SOURCE,
                      "algorithmic_bytes_per_launch": round(gemm_bytes / max(gemm_launches, 1)),
                      "gemm_ms_per_sweep": round(gemm_ms, 3), "gemm_launches_per_sweep": int(gemm_launches)},

@@ -269,7 +269,12 @@ int mnp_probe_mfma_bf16(float* out, void* stream);
  * LU tuning knobs (process wide, host values; no counterpart in the reference). key 0: outer
  * block width of the blocked LU and right-side solves (16..1024 columns, default 128); 1:
  * lookahead (trailing update split over two streams) on/off; 2: register-resident panel for
- * complex float on/off; 3: panel height from which the multi-workgroup panel is used (>= 256).
+ * complex float on/off; 3: panel height from which the multi-workgroup panel is used (>= 256);
+ * 4: complex-float GEMM on the FP32 matrix cores (0, default) or on the BF16 matrix cores with
+ * operands split into 2 or 3 bfloat16 terms, per tile (2, 3) or once into planes (12, 13; 2
+ * terms: about 16 significant bits, 3: about 24, as FP32); 5: workgroup tile of the plane GEMM
+ * (11: 64 x 64, 21: 128 x 64, 12: 64 x 128, 22: 128 x 128); 6: row tiles per group of the
+ * plane GEMM's XCD-aware tile order (0: plain grid order).
  * mnpbem_amd.autotune picks them per problem shape by timing. */
 int mnp_set_tuning(int key, int value);
 int mnp_get_tuning(int key, int* value);

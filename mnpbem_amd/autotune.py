@@ -17,7 +17,15 @@ import torch
 from ._lib import call
 
 _CACHE = {}
-_KEYS = {"nbo": 0, "lookahead": 1}
+_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4}
+
+
+def _value(key, v):
+    """Library value of a knob: gemm_split 2 or 3 (bfloat16 terms per operand) selects the
+    pre-split plane GEMM (library modes 12, 13), anything else the FP32 matrix-core GEMM."""
+    if key == "gemm_split":
+        return 10 + int(v) if int(v) in (2, 3) else 0
+    return int(v)
 
 
 def _get(key):
@@ -28,7 +36,8 @@ def _get(key):
 
 @contextlib.contextmanager
 def knobs(cfg):
-    """Set the library's LU knobs from cfg (keys 'nbo', 'lookahead') for the duration."""
+    """Set the library's LU knobs from cfg (keys 'nbo', 'lookahead', 'gemm_split') for the
+    duration."""
     if not cfg:
         yield
         return
@@ -36,7 +45,7 @@ def knobs(cfg):
     try:
         for k, i in _KEYS.items():
             if k in cfg:
-                call("mnp_set_tuning", i, int(cfg[k]))
+                call("mnp_set_tuning", i, _value(k, cfg[k]))
         yield
     finally:
         for k, i in _KEYS.items():
@@ -48,10 +57,11 @@ def candidates():
             if nbo >= nb]
 
 
-def tune_lu(n, batch, dtype, dev, reps=1, seed=0):
+def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):
     """Fastest (nb, nbo, lookahead) for `batch` LUs of size n in `dtype` (complex64 or
-    complex128); cached per (n, batch, dtype). Returns the config with its time in ms."""
-    key = (int(n), int(batch), str(dtype))
+    complex128) with the complex-float GEMM mode ``gemm_split``; cached per (n, batch, dtype,
+    gemm_split). Returns the config (with gemm_split) and its time in ms."""
+    key = (int(n), int(batch), str(dtype), int(gemm_split))
     if key in _CACHE:
         return _CACHE[key]
     prec = 0 if dtype == torch.complex128 else 1
@@ -63,6 +73,7 @@ def tune_lu(n, batch, dtype, dev, reps=1, seed=0):
     st = torch.cuda.current_stream(dev).cuda_stream
     best = None
     for cfg in candidates():
+        cfg = dict(cfg, gemm_split=gemm_split)
         with knobs(cfg):
             times = []
             for rep in range(reps + 1):

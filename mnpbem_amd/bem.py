@@ -71,6 +71,12 @@ class BEMRet:
         self.max_refine = int(op.get("max_refine", 12))
         # LU shape autotuning (autotune.py): default on for batched sweeps (batch >= 8)
         self.autotune = op.get("autotune", None)
+        # complex-float GEMMs of the mixed chain: operands split into 3 bfloat16 terms (as
+        # accurate as FP32, on the BF16 matrix cores; default), 2 terms (about 16 bits; faster,
+        # the FP64 refinement takes the accuracy back), or 0 (FP32 matrix cores)
+        self.gemm_split = int(op.get("gemm_split", 3))
+        if self.gemm_split not in (0, 2, 3):
+            raise ValueError("BEMRet: gemm_split must be 0, 2 or 3")
         self.tuning = None
         self.g = op.get("green") or CompGreenRet(p, p, **{k: v for k, v in op.items()
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
@@ -88,11 +94,13 @@ class BEMRet:
             # the previous solver's buffers go back to torch's caching allocator and are reused
             # (no empty_cache: re-mapping tens of GB per solver costs more than the solves)
             self._solver = None
-            nb, tuning = self.nb, None
+            nb = self.nb
+            split = self.gemm_split if precision != "double" else 0
+            tuning = dict(gemm_split=split)
             if (self.autotune if self.autotune is not None else batch >= 8):
                 from . import autotune
                 fdt = torch.complex128 if precision == "double" else torch.complex64
-                tuning = autotune.tune_lu(self.g.geo.n, batch, fdt, self.g.geo.dev)
+                tuning = autotune.tune_lu(self.g.geo.n, batch, fdt, self.g.geo.dev, gemm_split=split)
                 nb = tuning["nb"]
             self.tuning = tuning
             s = engine.RetSolverBatch(self.g.geo, batch, precision, nb, self.refine_tol, self.max_refine, tuning)

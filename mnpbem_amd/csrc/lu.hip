@@ -638,9 +638,21 @@ static int coop_min_rows() {
 }
 
 // Tuning knobs (process wide): 0 = outer block width (columns, >= nb), 1 = lookahead on/off,
-// 2 = register-resident panel on/off, 3 = rows from which panels use the cooperative kernel.
+// 2 = register-resident panel on/off, 3 = rows from which panels use the cooperative kernel,
+// 4 = complex-float GEMM operand terms (0: FP32 matrix cores, 2 or 3: split BF16 per tile,
+// 12 or 13: split once into planes), 5 = workgroup tile of the plane GEMM (11, 21, 12, 22),
+// 6 = row tiles per group of its XCD-aware tile order (0: grid order).
+int set_gemm_split(int v);
+int get_gemm_split();
+int set_plane_tile(int v);
+int get_plane_tile();
+int set_plane_group(int v);
+int get_plane_group();
 int set_tuning(int key, int value) {
     switch (key) {
+        case 4: return set_gemm_split(value);
+        case 5: return set_plane_tile(value);
+        case 6: return set_plane_group(value);
         case 0: if (value < 16 || value > 1024) return -2; g_nbo = value; return 0;
         case 1: g_lookahead = value != 0; return 0;
         case 2: g_panel_reg = value != 0; return 0;
@@ -654,6 +666,9 @@ int get_tuning(int key) {
         case 1: return g_lookahead ? 1 : 0;
         case 2: return g_panel_reg ? 1 : 0;
         case 3: return coop_min_rows();
+        case 4: return get_gemm_split();
+        case 5: return get_plane_tile();
+        case 6: return get_plane_group();
         default: return -1;
     }
 }

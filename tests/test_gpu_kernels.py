@@ -1,5 +1,7 @@
 """Kernel-level parity on the GPU: matrix-core GEMM, LU, permutations, solves, GEMV.
 Reference arithmetic: numpy / LAPACK (scipy) on the host."""
+import ctypes
+
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -79,6 +81,46 @@ def test_gemm_sub(prec, m, n, k, lda, ldb):
     for b in range(B_):
         ref = Cm[b] - A[b][:m] @ Bm[b][:k]
         assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < tol
+
+
+@pytest.mark.parametrize("split,tile", [(2, 11), (3, 11), (12, 11), (13, 11), (12, 111), (13, 111), (13, 121),
+                                        (13, 112), (13, 21), (12, 12), (13, 22)])
+@pytest.mark.parametrize("m,n,k,lda,ldb", [(150, 97, 45, 150, 46), (300, 260, 128, 300, 128), (64, 64, 16, 64, 16),
+                                           (131, 270, 37, 132, 38), (131, 70, 37, 131, 37)])
+def test_gemm_sub_split_bf16(split, tile, m, n, k, lda, ldb):
+    """Complex-float GEMM on the BF16 matrix cores with 2 or 3 operand terms, split per tile
+    (2, 3) or once into planes (12, 13) with the LDS-staged (11, 21, 12, 22) or direct (111,
+    121, 112) plane kernels and their workgroup tiles; ragged edges, K not a multiple of 16, padded leading
+    dimensions: error against the exact product within the terms kept (2^-15 and 2^-22
+    relative to |A||B|, with FP32 accumulation)."""
+    banner("gemm_sub split={} tile={} m={} n={} k={}".format(split, tile, m, n, k))
+    rng = np.random.default_rng(2)
+    B_ = 2
+    A = [_rand(rng, lda, k) for _ in range(B_)]
+    Bm = [_rand(rng, ldb, n) for _ in range(B_)]
+    Cm = [_rand(rng, m, n) for _ in range(B_)]
+    tA, tB, tC = _cm(A, torch.complex64), _cm(Bm, torch.complex64), _cm(Cm, torch.complex64)
+    old, old_t = ctypes.c_int(0), ctypes.c_int(0)
+    _lib().call("mnp_get_tuning", 4, ctypes.addressof(old))
+    _lib().call("mnp_get_tuning", 5, ctypes.addressof(old_t))
+    _lib().call("mnp_set_tuning", 4, split)
+    _lib().call("mnp_set_tuning", 5, tile)
+    try:
+        _lib().call("mnp_gemm_sub_batched", 1, m, n, k, tA.data_ptr(), lda, lda * k, tB.data_ptr(), ldb, ldb * n,
+                    tC.data_ptr(), m, m * n, B_, _sh())
+        torch.cuda.synchronize()
+    finally:
+        _lib().call("mnp_set_tuning", 4, old.value)
+        _lib().call("mnp_set_tuning", 5, old_t.value)
+    for b in range(B_):
+        A64 = A[b][:m].astype(np.complex64).astype(complex)
+        B64 = Bm[b][:k].astype(np.complex64).astype(complex)
+        ref = Cm[b].astype(np.complex64).astype(complex) - A64 @ B64
+        scale = np.abs(A64) @ np.abs(B64)
+        err = np.max(np.abs(_np(tC, b) - ref) / (scale + np.abs(ref)))
+        assert err < (4e-5 if split % 10 == 2 else 1e-6), err
+        if split % 10 == 3:           # as accurate as the FP32 kernel (within 2x)
+            assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < 4e-6
 
 
 @pytest.mark.parametrize("n,nb", [(200, 32), (97, 32), (64, 16)])
