@@ -852,7 +852,12 @@ void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void*
     const bool vec = sizeof(T) == 4 && ((lda | ldb | sA | sB) & 1) == 0 && ((uintptr_t)A & 15) == 0 &&
                      ((uintptr_t)B & 15) == 0;
     if (sizeof(T) == 4 && g_gemm_split >= 12) {
-        if (g_prof_on) g_prof_pending = rec;
+        if (g_prof_on) {
+            // the kernel reads A and B as bfloat16 planes (2 bytes x 2 components x terms)
+            const double terms = g_gemm_split - 10;
+            rec.bytes = (4.0 * terms * ((double)m * k + (double)k * n) + 16.0 * m * n) * batch;
+            g_prof_pending = rec;
+        }
         const bool ok = g_gemm_split == 12
                             ? gemm_planes<2>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s)
                             : gemm_planes<3>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
