@@ -141,14 +141,16 @@ def test_drude_sphere_sweep_vs_oracle(sphere):
     assert np.all(np.isfinite(ext))
 
 
-def test_mixed_precision_refinement(sphere):
-    """complex64 factorisation + FP64 refinement reaches the complex128 answer (1e-8 on ext)."""
+@pytest.mark.parametrize("split", [0, 2, 3])
+def test_mixed_precision_refinement(sphere, split):
+    """complex64 factorisation (GEMMs on the FP32 matrix cores, or on the BF16 ones with 2 or 3
+    operand terms) + FP64 refinement reaches the complex128 answer (1e-8 on ext)."""
     cp, s, fp, closed = sphere
     lams = np.linspace(400, 800, 9)
-    banner("mixed precision sphere")
+    banner("mixed precision sphere gemm_split={}".format(split))
     exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
     ref = M.BEMRet(cp, batch=9).solve_spectrum(exc, lams)["ext"][:, 0]
-    out = M.BEMRet(cp, batch=9, precision="mixed").solve_spectrum(exc, lams)
+    out = M.BEMRet(cp, batch=9, precision="mixed", gemm_split=split, autotune=False).solve_spectrum(exc, lams)
     ext = out["ext"][:, 0]
     assert np.max(np.abs(ext - ref) / np.abs(ref)) < 1e-8
     assert np.max(out["relres"][0]) <= 1e-12
