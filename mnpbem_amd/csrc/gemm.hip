@@ -444,12 +444,14 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64s(int m, int n, int k, cons
 // tile of C re-splits the same rows of A); here A and B are split once into bfloat16 planes,
 // and the GEMM only copies 16-byte chunks. Plane layout (per batch entry): element (qc, r, k)
 // of plane qc = 2 term + component (0 re, 1 im), row r (A: row of A, B: column of B), k at
-// ((qc * KS + k / 16) * R + r) * 16 + k % 16, KS = ceil(K / 16), R = rows padded to 128, so
-// the 128 x 16 tile of one k slice is 4 KB contiguous. Padding rows and k are zero. A is
+// (((qc * KS + k / 16) * 2 + (k / 8) % 2) * R + r) * 8 + k % 8, KS = ceil(K / 16), R = rows
+// padded to 128: one k slice of 128 rows is 4 KB contiguous, as two halves of 8 k in which
+// consecutive rows are 16 bytes apart (consecutive lanes read consecutive rows coalesced and
+// store them to LDS rows 48 bytes apart, conflict-free). Padding rows and k are zero. A is
 // stored negated (C -= A B).
 // One thread per plane row and k slice of 16: reads 16 complex values (TRANS: 128 contiguous
 // bytes of a column of B; else one element per k, coalesced across the threads' rows of A) and
-// writes 32 contiguous bytes per term and component.
+// writes 16 bytes per half slice, term and component (consecutive threads: consecutive 16 bytes).
 template <int NS, bool TRANS>
 __global__ __launch_bounds__(256) void k_split_planes(int rows, int K, const cx<float>* __restrict__ X, long ldx,
                                                       long sX, __bf16* __restrict__ P, int R, long sP, float sgn,
@@ -499,9 +501,8 @@ __global__ __launch_bounds__(256) void k_split_planes(int rows, int K, const cx<
                 x[8 + e] -= (float)h1[e];
             }
             const long qc = 2 * q + c;
-            bf8v* d = (bf8v*)(P + ((qc * KS + s) * R + r) * 16);
-            d[0] = h0;
-            d[1] = h1;
+            *(bf8v*)(P + (((qc * KS + s) * 2) * R + r) * 8) = h0;
+            *(bf8v*)(P + (((qc * KS + s) * 2 + 1) * R + r) * 8) = h1;
         }
     }
 }
@@ -528,12 +529,17 @@ __device__ inline void xcd_tile(int gm, int& bx, int& by, int& bz) {
     by = (int)(w / rows);
 }
 
-// Workgroup tile (64 MI) x (64 NJ) of C, 4 waves in 2 x 2, each wave MI x NJ MFMA tiles of 32 x 32.
-template <int NS, int MI, int NJ>
-__global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
-                                                       long sPA, const __bf16* __restrict__ PB, int RB, long sPB,
-                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
-    constexpr int TM = 64 * MI, TN = 64 * NJ;
+// Workgroup of WR x WC waves (64 WR WC lanes), each wave MI x NJ MFMA tiles of 32 x 32: a
+// (32 MI WR) x (32 NJ WC) tile of C.
+template <int NS, int MI, int NJ, int WR = 2, int WC = 2>
+__global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, int K, const __bf16* __restrict__ PA,
+                                                                int RA, long sPA, const __bf16* __restrict__ PB,
+                                                                int RB, long sPB, cx<float>* __restrict__ C, long ldc,
+                                                                long sC, int gm) {
+    constexpr int NT = 64 * WR * WC;
+    constexpr int TM = 32 * MI * WR, TN = 32 * NJ * WC;
+    constexpr int CA = 4 * NS * TM, CB = 4 * NS * TN;          // 16-byte chunks per k slice
+    constexpr int UA = (CA + NT - 1) / NT, UB = (CB + NT - 1) / NT;
     __shared__ __attribute__((aligned(16))) __bf16 As[NS][2][TM][SKP];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][2][TN][SKP];
     typedef unsigned u4v __attribute__((ext_vector_type(4)));
@@ -545,7 +551,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
     const int KS = (K + 15) / 16;
     const int row0 = tx * TM, col0 = ty * TN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave & 1) * 32 * MI, wn = (wave >> 1) * 32 * NJ;
+    const int wm = (wave % WR) * 32 * MI, wn = (wave / WR) * 32 * NJ;
     f16acc cre[MI][NJ], cim[MI][NJ];
 #pragma unroll
     for (int a = 0; a < MI; ++a)
@@ -553,31 +559,33 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
         for (int b = 0; b < NJ; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
-    // chunk e = t + 256 u of the A (B) tile: plane e / (2 TM), row (e % (2 TM)) / 2, half e % 2
-    u4v ra[NS * MI], rb[NS * NJ];
+    // chunk e = t + NT u of the A (B) tile: plane e / (2 TM), half (e % (2 TM)) / TM, row e % TM
+    u4v ra[UA], rb[UB];
     auto load = [&](int s) {
 #pragma unroll
-        for (int u = 0; u < NS * MI; ++u) {
-            const int e = t + 256 * u, qc = e / (2 * TM), w = e % (2 * TM);
-            ra[u] = *(const u4v*)(PA + (((long)qc * KS + s) * RA + row0 + (w >> 1)) * 16 + 8 * (w & 1));
+        for (int u = 0; u < UA; ++u) {
+            const int e = t + NT * u, qc = e / (2 * TM), w = e % (2 * TM);
+            if (CA % NT == 0 || e < CA)
+                ra[u] = *(const u4v*)(PA + ((((long)qc * KS + s) * 2 + w / TM) * RA + row0 + w % TM) * 8);
         }
 #pragma unroll
-        for (int u = 0; u < NS * NJ; ++u) {
-            const int e = t + 256 * u, qc = e / (2 * TN), w = e % (2 * TN);
-            rb[u] = *(const u4v*)(PB + (((long)qc * KS + s) * RB + col0 + (w >> 1)) * 16 + 8 * (w & 1));
+        for (int u = 0; u < UB; ++u) {
+            const int e = t + NT * u, qc = e / (2 * TN), w = e % (2 * TN);
+            if (CB % NT == 0 || e < CB)
+                rb[u] = *(const u4v*)(PB + ((((long)qc * KS + s) * 2 + w / TN) * RB + col0 + w % TN) * 8);
         }
     };
     load(0);
     for (int s = 0; s < KS; ++s) {
 #pragma unroll
-        for (int u = 0; u < NS * MI; ++u) {
-            const int e = t + 256 * u, qc = e / (2 * TM), w = e % (2 * TM);
-            *(u4v*)&As[qc >> 1][qc & 1][w >> 1][8 * (w & 1)] = ra[u];
+        for (int u = 0; u < UA; ++u) {
+            const int e = t + NT * u, qc = e / (2 * TM), w = e % (2 * TM);
+            if (CA % NT == 0 || e < CA) *(u4v*)&As[qc >> 1][qc & 1][w % TM][8 * (w / TM)] = ra[u];
         }
 #pragma unroll
-        for (int u = 0; u < NS * NJ; ++u) {
-            const int e = t + 256 * u, qc = e / (2 * TN), w = e % (2 * TN);
-            *(u4v*)&Bs[qc >> 1][qc & 1][w >> 1][8 * (w & 1)] = rb[u];
+        for (int u = 0; u < UB; ++u) {
+            const int e = t + NT * u, qc = e / (2 * TN), w = e % (2 * TN);
+            if (CB % NT == 0 || e < CB) *(u4v*)&Bs[qc >> 1][qc & 1][w % TN][8 * (w / TN)] = rb[u];
         }
         __syncthreads();
         if (s + 1 < KS) load(s + 1);
@@ -641,10 +649,9 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
             }
 }
 
-// Variant without LDS: in the plane layout the 32 rows x 16 k of one MFMA operand are 1 KB
-// contiguous, and lane l's 16 bytes (row l % 32, k 8 (l / 32) .. +7) sit at byte 32 (l % 32) +
-// 16 (l / 32) of it, so each wave loads its operands straight from memory with one coalesced
-// 16-byte load per operand fragment (the two waves sharing a row or column tile meet in the L1),
+// Variant without LDS: in the plane layout lane l's 16 bytes of an MFMA operand (row l % 32,
+// k 8 (l / 32) .. +7) lie in two 512-byte runs (one per half slice), so each wave loads its
+// operands straight from memory with one coalesced 16-byte load per operand fragment (the two waves sharing a row or column tile meet in the L1),
 // and the next k slice is loaded while the current one is multiplied. No barriers.
 template <int NS, int MI, int NJ>
 __global__ __launch_bounds__(256) void k_gemm_sub_c64d(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
@@ -659,7 +666,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64d(int m, int n, int K, cons
     const int KS = (K + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = tx * 64 * MI + (wave & 1) * 32 * MI, wn = ty * 64 * NJ + (wave >> 1) * 32 * NJ;
-    const int lo = (lane & 31) * 16 + 8 * (lane >> 5);
+    const int lr = lane & 31, lh = lane >> 5;
     f16acc cre[MI][NJ], cim[MI][NJ];
 #pragma unroll
     for (int a = 0; a < MI; ++a)
@@ -674,10 +681,10 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64d(int m, int n, int K, cons
         for (int qc = 0; qc < 2 * NS; ++qc) {
 #pragma unroll
             for (int ti = 0; ti < MI; ++ti)
-                fa[buf][ti][qc] = *(const u4v*)(PA + (((long)qc * KS + s) * RA + wm + 32 * ti) * 16 + lo);
+                fa[buf][ti][qc] = *(const u4v*)(PA + ((((long)qc * KS + s) * 2 + lh) * RA + wm + 32 * ti + lr) * 8);
 #pragma unroll
             for (int tj = 0; tj < NJ; ++tj)
-                fb[buf][tj][qc] = *(const u4v*)(PB + (((long)qc * KS + s) * RB + wn + 32 * tj) * 16 + lo);
+                fb[buf][tj][qc] = *(const u4v*)(PB + ((((long)qc * KS + s) * 2 + lh) * RB + wn + 32 * tj + lr) * 8);
         }
     };
     auto step = [&](int buf) {
@@ -792,6 +799,31 @@ static bool gemm_planes(int m, int n, int k, const void* A, long lda, long sA, c
             hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 2>), dim3((m + 63) / 64, (n + 127) / 128, batch), dim3(256), 0,
                                s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
             break;
+        case 1142:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 2>), dim3((m + 127) / 128, (n + 63) / 64, batch),
+                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
+        case 1124:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 2, 4>), dim3((m + 63) / 64, (n + 127) / 128, batch),
+                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
+        case 1144:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 4>), dim3((m + 127) / 128, (n + 127) / 128, batch),
+                               dim3(1024), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
+        case 1242:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 2, 4, 2>), dim3((m + 127) / 128, (n + 127) / 128, batch),
+                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
+        case 2124:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 2, 1, 2, 4>), dim3((m + 127) / 128, (n + 127) / 128, batch),
+                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
         case 111:
             hipLaunchKernelGGL((k_gemm_sub_c64d<NS, 1, 1>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s,
                                m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
@@ -821,7 +853,9 @@ int set_gemm_split(int v) {
 }
 int get_gemm_split() { return g_gemm_split; }
 int set_plane_tile(int v) {
-    if (v != 11 && v != 21 && v != 12 && v != 22 && v != 111 && v != 121 && v != 112) return -2;
+    if (v != 11 && v != 21 && v != 12 && v != 22 && v != 111 && v != 121 && v != 112 && v != 1142 && v != 1124 &&
+        v != 1144 && v != 1242 && v != 2124)
+        return -2;
     g_plane_tile = v;
     return 0;
 }
