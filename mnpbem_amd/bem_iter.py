@@ -117,7 +117,8 @@ class DeviceGMRES:
 class BEMRetIter:
     """Iterative BEMRet. Options (reference names): ``solver='gmres'``, ``tol=1e-4``,
     ``maxit=200``, ``restart=None`` (= min(n, maxit)), ``precond='hmat'|'bemret'|None``,
-    ``preconditioner='auto'|'none'``, ``precond_precision='single'|'double'``, ``hmatrix``,
+    ``preconditioner='auto'|'none'``, ``precond_precision='single'|'double'``,
+    ``precond_gemm_split=2|3|0`` (not in the reference), ``hmatrix``,
     ``htol=1e-6``, ``kmax=[4, 100]``, ``cleaf=200``, ``eta=2.5``, ``fadmiss``; not in the
     reference: ``precond_wavelength`` (factor the preconditioner at that wavelength once and
     reuse it for a sweep) and ``solve(exc, x0=...)``."""
@@ -145,6 +146,10 @@ class BEMRetIter:
         # preconditioner='none' (bem_ret_iter.py:1688-1692, :1745-1774)
         self.use_precond = pc != "none" if self.hmatrix else not (pc == "none" or pr is None)
         self.precond_precision = op.get("precond_precision", "single")
+        # complex64 preconditioner GEMMs: operands split into 2 bfloat16 terms by default (about
+        # 16 significant bits: ample for an approximate inverse, GMRES judges every iterate);
+        # 3 = as accurate as FP32, 0 = the FP32 matrix cores
+        self.precond_gemm_split = int(op.get("precond_gemm_split", 2))
         # None: the preconditioner is the BEMRet chain of each solve's wavelength (reference);
         # a wavelength: the chain is factored there once and reused for every later solve
         self.precond_wavelength = op.get("precond_wavelength", None)
@@ -186,7 +191,15 @@ class BEMRetIter:
         if self.use_precond and not keep:
             torch.cuda.empty_cache()
             # complex64 (default) or complex128 chain; refinement is done by GMRES itself
-            self._pre = engine.RetSolverBatch(geo, 1, "single" if self.precond_precision == "single" else "double")
+            single = self.precond_precision == "single"
+            tuning = dict(gemm_split=self.precond_gemm_split if single else 0)
+            # one large matrix: wider outer blocks give the trailing GEMMs a deeper K (2-term GEMMs
+            # at n = 24,544: outer block 128 -> 4.5 s, 256 -> 3.7 s, 512 -> 3.5 s for the chain)
+            if geo.n >= 12000:
+                tuning["nbo"] = 512
+            elif geo.n >= 4000:
+                tuning["nbo"] = 256
+            self._pre = engine.RetSolverBatch(geo, 1, "single" if single else "double", tuning=tuning)
             self._pre_enei = enei if self.precond_wavelength is None else float(self.precond_wavelength)
             t0 = _now(geo.dev)
             self._pre.factor([self._pre_enei])
