@@ -37,7 +37,9 @@ int mnp_probe_mfma_layout(double* out, void* stream);
  * each matrix-core GEMM launch is bracketed by HIP events on its stream; collect returns
  * out[0] = GEMM kernel ms, out[1] = algorithmic flops (8 m n k per complex GEMM),
  * out[2] = launches since the previous collect, out[3] = algorithmic bytes (A and B read once,
- * C read and written once) (host array of 4; waits for the events). */
+ * C read and written once) (host array of 4; waits for the events). When some launches ran the
+ * split-BF16 plane kernel, the sums cover those launches only (the dominant kernel); the rest
+ * are dropped. */
 int mnp_prof_enable(int on);
 int mnp_prof_collect(double* out);
 

@@ -23,6 +23,7 @@ namespace mnp {
 struct ProfRec {
     hipEvent_t a, b;
     double flops, bytes;
+    int plane;                              // 1: the split-BF16 plane kernel
 };
 static bool g_prof_on = false;
 static std::vector<ProfRec> g_prof;
@@ -42,23 +43,30 @@ static hipEvent_t pool_get() {
 
 void prof_enable(bool on) { g_prof_on = on; }
 
-// Sum of GEMM kernel time (ms), algorithmic flops and bytes since the last call; waits for the
+// Sum of GEMM kernel time (ms), algorithmic flops and bytes since the last call, over the
+// launches of the split-BF16 plane kernel when there were any, else over all; waits for the
 // events.
 void prof_collect(double* ms, double* flops, double* launches, double* bytes) {
-    double t = 0, f = 0, by = 0;
+    double t = 0, f = 0, by = 0, nl = 0;
+    bool any_plane = false;
+    for (auto& r : g_prof) any_plane |= r.plane != 0;
     for (auto& r : g_prof) {
-        hipEventSynchronize(r.b);
-        float e = 0;
-        hipEventElapsedTime(&e, r.a, r.b);
-        t += e;
-        f += r.flops;
-        by += r.bytes;
+        if (!any_plane || r.plane) {
+            hipEventSynchronize(r.b);
+            float e = 0;
+            hipEventElapsedTime(&e, r.a, r.b);
+            t += e;
+            f += r.flops;
+            by += r.bytes;
+            nl += 1;
+        }
         g_pool.push_back(r.a);
         g_pool.push_back(r.b);
     }
+    hipDeviceSynchronize();
     *ms = t;
     *flops = f;
-    *launches = (double)g_prof.size();
+    *launches = nl;
     *bytes = by;
     g_prof.clear();
 }
@@ -531,7 +539,7 @@ __device__ inline void xcd_tile(int gm, int& bx, int& by, int& bz) {
 
 // Workgroup of WR x WC waves (64 WR WC lanes), each wave MI x NJ MFMA tiles of 32 x 32: a
 // (32 MI WR) x (32 NJ WC) tile of C.
-template <int NS, int MI, int NJ, int WR = 2, int WC = 2>
+template <int NS, int MI, int NJ, int WR = 2, int WC = 2, int NB = 1>
 __global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, int K, const __bf16* __restrict__ PA,
                                                                 int RA, long sPA, const __bf16* __restrict__ PB,
                                                                 int RB, long sPB, cx<float>* __restrict__ C, long ldc,
@@ -540,8 +548,10 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, in
     constexpr int TM = 32 * MI * WR, TN = 32 * NJ * WC;
     constexpr int CA = 4 * NS * TM, CB = 4 * NS * TN;          // 16-byte chunks per k slice
     constexpr int UA = (CA + NT - 1) / NT, UB = (CB + NT - 1) / NT;
-    __shared__ __attribute__((aligned(16))) __bf16 As[NS][2][TM][SKP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][2][TN][SKP];
+    // NB = 2: two LDS buffers, the next k slice is stored while the current one is read (one
+    // barrier per slice)
+    __shared__ __attribute__((aligned(16))) __bf16 As[NB][NS][2][TM][SKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NB][NS][2][TN][SKP];
     typedef unsigned u4v __attribute__((ext_vector_type(4)));
     int tx, ty, bz;
     xcd_tile(gm, tx, ty, bz);
@@ -575,35 +585,34 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, in
                 rb[u] = *(const u4v*)(PB + ((((long)qc * KS + s) * 2 + w / TN) * RB + col0 + w % TN) * 8);
         }
     };
-    load(0);
-    for (int s = 0; s < KS; ++s) {
+    auto store = [&](int buf) {
 #pragma unroll
         for (int u = 0; u < UA; ++u) {
             const int e = t + NT * u, qc = e / (2 * TM), w = e % (2 * TM);
-            if (CA % NT == 0 || e < CA) *(u4v*)&As[qc >> 1][qc & 1][w % TM][8 * (w / TM)] = ra[u];
+            if (CA % NT == 0 || e < CA) *(u4v*)&As[buf][qc >> 1][qc & 1][w % TM][8 * (w / TM)] = ra[u];
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
             const int e = t + NT * u, qc = e / (2 * TN), w = e % (2 * TN);
-            if (CB % NT == 0 || e < CB) *(u4v*)&Bs[qc >> 1][qc & 1][w % TN][8 * (w / TN)] = rb[u];
+            if (CB % NT == 0 || e < CB) *(u4v*)&Bs[buf][qc >> 1][qc & 1][w % TN][8 * (w / TN)] = rb[u];
         }
-        __syncthreads();
-        if (s + 1 < KS) load(s + 1);
+    };
+    auto compute = [&](int buf) {
         const int fr = lane & 31, fk = 8 * (lane >> 5);
         bf8v pr[NJ][NS], pi[NJ][NS], qr[MI][NS], qi[MI][NS], qp[MI][NS];
 #pragma unroll
         for (int tj = 0; tj < NJ; ++tj)
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
-                pr[tj][q] = *(const bf8v*)&Bs[q][0][wn + 32 * tj + fr][fk];
-                pi[tj][q] = *(const bf8v*)&Bs[q][1][wn + 32 * tj + fr][fk];
+                pr[tj][q] = *(const bf8v*)&Bs[buf][q][0][wn + 32 * tj + fr][fk];
+                pi[tj][q] = *(const bf8v*)&Bs[buf][q][1][wn + 32 * tj + fr][fk];
             }
 #pragma unroll
         for (int ti = 0; ti < MI; ++ti)
 #pragma unroll
             for (int q = 0; q < NS; ++q) {
-                qr[ti][q] = *(const bf8v*)&As[q][0][wm + 32 * ti + fr][fk];
-                qi[ti][q] = *(const bf8v*)&As[q][1][wm + 32 * ti + fr][fk];
+                qr[ti][q] = *(const bf8v*)&As[buf][q][0][wm + 32 * ti + fr][fk];
+                qi[ti][q] = *(const bf8v*)&As[buf][q][1][wm + 32 * ti + fr][fk];
                 u4v w = __builtin_bit_cast(u4v, qi[ti][q]);
                 w ^= 0x80008000u;
                 qp[ti][q] = __builtin_bit_cast(bf8v, w);
@@ -629,7 +638,25 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, in
                         cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[tj][p], qi[ti][q], cim[ti][tj], 0, 0, 0);
                     }
             }
+    };
+    load(0);
+    if (NB == 1) {
+        for (int s = 0; s < KS; ++s) {
+            store(0);
+            __syncthreads();
+            if (s + 1 < KS) load(s + 1);
+            compute(0);
+            __syncthreads();
+        }
+    } else {
+        store(0);
         __syncthreads();
+        for (int s = 0; s < KS; ++s) {
+            if (s + 1 < KS) load(s + 1);
+            compute(s & 1);
+            if (s + 1 < KS) store((s + 1) & 1);
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int ti = 0; ti < MI; ++ti)
@@ -799,6 +826,16 @@ static bool gemm_planes(int m, int n, int k, const void* A, long lda, long sA, c
             hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 2>), dim3((m + 63) / 64, (n + 127) / 128, batch), dim3(256), 0,
                                s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
             break;
+        case 3011:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 2, 2, 2>), dim3((m + 63) / 64, (n + 63) / 64, batch),
+                               dim3(256), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
+        case 3142:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 2, 2>), dim3((m + 127) / 128, (n + 63) / 64, batch),
+                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                               g_plane_group);
+            break;
         case 1142:
             hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 2>), dim3((m + 127) / 128, (n + 63) / 64, batch),
                                dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
@@ -854,7 +891,7 @@ int set_gemm_split(int v) {
 int get_gemm_split() { return g_gemm_split; }
 int set_plane_tile(int v) {
     if (v != 11 && v != 21 && v != 12 && v != 22 && v != 111 && v != 121 && v != 112 && v != 1142 && v != 1124 &&
-        v != 1144 && v != 1242 && v != 2124)
+        v != 1144 && v != 1242 && v != 2124 && v != 3011 && v != 3142)
         return -2;
     g_plane_tile = v;
     return 0;
@@ -885,11 +922,16 @@ void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void*
     // 16-byte loads need even leading dimensions and batch strides and 16-byte aligned bases
     const bool vec = sizeof(T) == 4 && ((lda | ldb | sA | sB) & 1) == 0 && ((uintptr_t)A & 15) == 0 &&
                      ((uintptr_t)B & 15) == 0;
-    if (sizeof(T) == 4 && g_gemm_split >= 12) {
+    // The plane GEMM splits A (m x k) and B (k x n) per launch: with a narrow C the split passes
+    // outweigh the faster products (measured at m = 2496, k = 256..2048: 3 terms lose to the
+    // FP32 matrix cores below ~160 columns, 2 terms below ~96), so narrow launches keep FP32.
+    const int narrow = (m < n ? m : n);
+    if (sizeof(T) == 4 && g_gemm_split >= 12 && narrow >= (g_gemm_split == 13 ? 160 : 96)) {
         if (g_prof_on) {
             // the kernel reads A and B as bfloat16 planes (2 bytes x 2 components x terms)
             const double terms = g_gemm_split - 10;
             rec.bytes = (4.0 * terms * ((double)m * k + (double)k * n) + 16.0 * m * n) * batch;
+            rec.plane = 1;
             g_prof_pending = rec;
         }
         const bool ok = g_gemm_split == 12
