@@ -368,7 +368,10 @@ def main():
         extra = dict(fp64_solves_per_s=round(v_d, 3), speedup_vs_fp64=round(value / v_d, 3),
                      ext_rel_err_vs_fp64=float(rel))
     if not args.no_mixed2 and args.gemm_split != 2 and "double" in results:
-        # the same sweep with 2-term operands (about 16 significant bits in the GEMMs)
+        # the same sweep with 2-term operands (about 16 significant bits in the GEMMs); the
+        # 3-term solver's buffers go first (both would not fit at batch 201)
+        solvers["mixed"]._solver = None
+        torch.cuda.empty_cache()
         b2 = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g, gemm_split=2)
         b2.solve_spectrum(exc, lams, precision="mixed")
         torch.cuda.synchronize()

@@ -52,7 +52,7 @@ def knobs(cfg):
             call("mnp_set_tuning", i, old[k])
 
 
-def candidates():
+def candidates(n=0):
     return [dict(nb=nb, nbo=nbo, lookahead=la) for nb in (16, 32) for nbo in (64, 128, 256) for la in (0, 1)
             if nbo >= nb]
 
@@ -87,7 +87,7 @@ def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):
         return 1e3 * min(times)
 
     best = None
-    for cfg in candidates():
+    for cfg in candidates(n):
         cfg = dict(cfg, gemm_split=gemm_split)
         ms = timed(cfg)
         if best is None or ms < best["ms"]:
