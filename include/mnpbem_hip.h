@@ -171,6 +171,27 @@ typedef struct mnp_green_geom {
     const double* rf;
 } mnp_green_geom;
 
+/* Observer points and the faces of a particle for fields at points (all device pointers):
+ * ppos npt x 3, pmed npt media of the points (1-based material index); fpos nface x 3 face
+ * centroids, farea, fpart (particle of each face), inout npart x 2 (inside, outside medium,
+ * 1-based); kmat nmat complex wavenumbers; refined point-face pairs as a CSR over the points
+ * (rowptr npt + 1, rcol ascending face indices, rg nref x (order + 1), rf nref x 3 x
+ * (order + 1) Taylor coefficients). */
+typedef struct mnp_point_geom {
+    int npt, nface, npart, order;
+    const double* ppos;
+    const int* pmed;
+    const double* fpos;
+    const double* farea;
+    const int* fpart;
+    const int* inout;
+    const double* kmat;
+    const int* rowptr;
+    const int* rcol;
+    const double* rg;
+    const double* rf;
+} mnp_point_geom;
+
 /* A built set of four H-matrices on one tree (all device pointers). D: complex double dense
  * blocks, kind q at D + q dstride, block b at doff[b]; U, V: complex float low-rank factors
  * (m x rank, n x rank) of kind q and block b at uoff[q nlr + b], voff[q nlr + b];
@@ -302,6 +323,15 @@ int mnp_surface_eval(const mnp_green_geom* g, int outside, double k0, const void
 int mnp_green_block(const mnp_green_geom* g, int outside, int key, int c0, int ncol, void* out, void* stream);
 int mnp_green_pick(const mnp_green_geom* g, int outside, int key, const long* ind, long count, void* out,
                    void* stream);
+
+
+/* Potentials and fields at points from a solution: CompGreenRet(p1 = ComPoint, p2 =
+ * ComParticle).potential / .field (greenfun/compgreen_ret.py:911-1034). A point in medium m
+ * sees the faces whose inside medium is m through sig1, h1 and those whose outside medium is m
+ * through sig2, h2 (n complex, n x 3 complex row-major). out: npt x 14 complex per point
+ * (phi, phip = 0, a[3], ap[3] = 0, e[3], h[3]; bare points carry no normal). */
+int mnp_point_eval(const mnp_point_geom* g, double k0, const void* sig1, const void* h1, const void* sig2,
+                   const void* h2, void* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@ from .materials import EpsConst, EpsDrude, EpsTable
 from .particle import Particle
 from .mesh import trisphere, trirod, trispheresegment, triellipsoid, fvgrid
 from .comparticle import ComParticle
+from .compoint import ComPoint
 from .greenfun import CompGreenRet, CompGreenStat, CompStruct
 from .bem import BEMRet
 from .bem_stat import BEMStat, PlaneWaveStat
@@ -15,6 +16,6 @@ from .bem_stat_iter import ACACompGreenStat, BEMStatIter
 from .planewave import PlaneWaveRet, SpectrumRet
 
 __all__ = ["EpsConst", "EpsDrude", "EpsTable", "Particle", "trisphere", "trirod", "trispheresegment",
-           "triellipsoid", "fvgrid", "ComParticle", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMStat",
+           "triellipsoid", "fvgrid", "ComParticle", "ComPoint", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMStat",
            "BEMStatIter", "ACACompGreenStat", "PlaneWaveStat", "PlaneWaveRet",
            "SpectrumRet"]

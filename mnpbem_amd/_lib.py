@@ -52,6 +52,7 @@ _SIGS = {
     "mnp_set_tuning": [I, I],
     "mnp_surface_eval": [P, I, C.c_double, P, P, P, P],
     "mnp_green_block": [P, I, I, I, I, P, P],
+    "mnp_point_eval": [P, C.c_double, P, P, P, P, P, P],
     "mnp_green_pick": [P, I, I, P, C.c_long, P, P],
     "mnp_get_tuning": [I, P],
 }
@@ -61,6 +62,12 @@ class GreenGeomC(C.Structure):
     """mnp_green_geom of include/mnpbem_hip.h (device pointers)."""
     _fields_ = [("n", I), ("npart", I), ("order", I)] + [
         (f, P) for f in ("pos", "nvec", "area", "fpart", "con_in", "con_out", "kmat", "rowptr", "rcol", "rg", "rf")]
+
+
+class PointGeomC(C.Structure):
+    """mnp_point_geom of include/mnpbem_hip.h (device pointers)."""
+    _fields_ = [("npt", I), ("nface", I), ("npart", I), ("order", I)] + [
+        (f, P) for f in ("ppos", "pmed", "fpos", "farea", "fpart", "inout", "kmat", "rowptr", "rcol", "rg", "rf")]
 
 
 class HMatC(C.Structure):

@@ -615,4 +615,19 @@ int mnp_green_pick(const mnp_green_geom* g, int outside, int key, const long* in
     return launch_status();
 }
 
+
+// ------------------------------------------------------------------------- fields at points
+int mnp_point_eval(const mnp_point_geom* g, double k0, const void* sig1, const void* h1, const void* sig2,
+                   const void* h2, void* out, void* stream) {
+    REQUIRE(g && g->npt >= 0 && g->nface > 0 && g->npart > 0 && g->order >= 0 && g->order < 8, 1);
+    REQUIRE(g->ppos && g->pmed && g->fpos && g->farea && g->fpart && g->inout && g->kmat && g->rowptr, 1);
+    REQUIRE(k0 > 0, 2);
+    REQUIRE(sig1 && h1 && sig2 && h2, 3);
+    REQUIRE(out || g->npt == 0, 7);
+    PointGeom a{g->npt, g->nface, g->npart, g->order, g->ppos, g->pmed, g->fpos, g->farea, g->fpart, g->inout,
+                g->kmat, g->rowptr, g->rcol, g->rg, g->rf};
+    point_eval(a, k0, sig1, h1, sig2, h2, out, (hipStream_t)stream);
+    return launch_status();
+}
+
 }  // extern "C"

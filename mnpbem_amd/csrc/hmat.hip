@@ -535,6 +535,103 @@ void surface_eval(const GreenGeom& a, int outside, double k0, const void* s, con
                        (const cx<double>*)hv, (cx<double>*)out);
 }
 
+// ---------------------------------------------------------------------------- fields at points
+// One wave per observer point i in medium m: lanes over the faces j that touch m (inside
+// medium m: sources sig1, h1; outside medium m: sig2, h2), G and Gp as for face pairs (refined
+// Taylor pairs from the CSR over the points, far pairs in closed form). out: 14 complex per
+// point as k_surface_eval (phi, 0, a, 0, e = i k0 a - Gp sigma, h = Gp x h).
+__global__ __launch_bounds__(256) void k_point_eval(PointGeom a, double k0, const cx<double>* s1, const cx<double>* h1,
+                                                    const cx<double>* s2, const cx<double>* h2, cx<double>* out) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= a.npt) return;
+    const int m = a.pmed[i];
+    cx<double> acc[10];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) acc[q] = mk(0.0, 0.0);
+    if (m > 0) {
+        const cx<double> k = reinterpret_cast<const cx<double>*>(a.kmat)[m - 1];
+        const cx<double> ik = mk(-k.im, k.re);
+        const double px = a.ppos[3 * i], py = a.ppos[3 * i + 1], pz = a.ppos[3 * i + 2];
+        const int r0 = a.rowptr[i], r1 = a.rowptr[i + 1];
+        const int no = a.order + 1;
+        for (int j = lane; j < a.nface; j += 64) {
+            const int pj = a.fpart[j];
+            const bool in = a.inout[2 * pj] == m, out_ = a.inout[2 * pj + 1] == m;
+            if (!in && !out_) continue;
+            const cx<double> s = in ? s1[j] : s2[j];
+            const cx<double>* hv = in ? h1 + 3 * j : h2 + 3 * j;
+            const double rx = px - a.fpos[3 * j], ry = py - a.fpos[3 * j + 1], rz = pz - a.fpos[3 * j + 2];
+            const double d = sqrt(rx * rx + ry * ry + rz * rz);
+            int lo = r0, hi = r1 - 1, p = -1;
+            while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int c = a.rcol[mid];
+                if (c == j) {
+                    p = mid;
+                    break;
+                }
+                if (c < j) lo = mid + 1;
+                else hi = mid - 1;
+            }
+            const cx<double> ph = expikd(k, d);
+            cx<double> G, Gp[3];
+            if (p >= 0) {
+                cx<double> sg = mk(0.0, 0.0), sf[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+                for (int q = no - 1; q >= 0; --q) {
+                    sg = ik * sg + mk(a.rg[(long)p * no + q], 0.0);
+                    for (int c = 0; c < 3; ++c) sf[c] = ik * sf[c] + mk(a.rf[((long)p * 3 + c) * no + q], 0.0);
+                }
+                G = sg * ph;
+                for (int c = 0; c < 3; ++c) Gp[c] = sf[c] * ph;
+            } else if (d > 0) {
+                const double inv_d = 1.0 / d, aj = a.farea[j];
+                G = (aj * inv_d) * ph;
+                const cx<double> f = (aj * inv_d * inv_d) * (mk(-k.im - inv_d, k.re) * ph);
+                Gp[0] = rx * f;
+                Gp[1] = ry * f;
+                Gp[2] = rz * f;
+            } else {
+                continue;      // a point on a centroid: excluded (ComPoint's mindist)
+            }
+            const cx<double> h[3] = {hv[0], hv[1], hv[2]};
+            acc[0] = acc[0] + G * s;
+            for (int c = 0; c < 3; ++c) {
+                acc[1 + c] = acc[1 + c] + G * h[c];
+                acc[4 + c] = acc[4 + c] + Gp[c] * s;
+            }
+            acc[7] = acc[7] + (Gp[1] * h[2] - Gp[2] * h[1]);
+            acc[8] = acc[8] + (Gp[2] * h[0] - Gp[0] * h[2]);
+            acc[9] = acc[9] + (Gp[0] * h[1] - Gp[1] * h[0]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+        for (int o = 32; o > 0; o >>= 1) {
+            acc[q].re += __shfl_down(acc[q].re, o, 64);
+            acc[q].im += __shfl_down(acc[q].im, o, 64);
+        }
+    if (lane == 0) {
+        cx<double>* o = out + 14L * i;
+        const cx<double> zero = mk(0.0, 0.0), ik0 = mk(0.0, k0);
+        o[0] = acc[0];
+        o[1] = zero;
+        for (int c = 0; c < 3; ++c) {
+            o[2 + c] = acc[1 + c];
+            o[5 + c] = zero;
+            o[8 + c] = ik0 * acc[1 + c] - acc[4 + c];
+            o[11 + c] = acc[7 + c];
+        }
+    }
+}
+
+void point_eval(const PointGeom& a, double k0, const void* s1, const void* h1, const void* s2, const void* h2,
+                void* out, hipStream_t st) {
+    if (a.npt <= 0) return;
+    hipLaunchKernelGGL(k_point_eval, dim3((a.npt + 3) / 4), dim3(256), 0, st, a, k0, (const cx<double>*)s1,
+                       (const cx<double>*)h1, (const cx<double>*)s2, (const cx<double>*)h2, (cx<double>*)out);
+}
+
 // ---------------------------------------------------------------------------- entries by key
 // Key of one region-pair entry: 0 G, 1 F, 2 H1 = F + 2 pi, 3 H2 = F - 2 pi (self terms), 4 Gp,
 // 5 H1p = Gp + 2 pi n_i, 6 H2p = Gp - 2 pi n_i (greenret_refined.py:654-866). Writes 1 value

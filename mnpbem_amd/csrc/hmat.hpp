@@ -24,6 +24,22 @@ struct GreenGeom {
     const double* rf;       // nref x 3 x (order + 1)
 };
 
+// Observer points against the faces of a particle (fields at points).
+struct PointGeom {
+    int npt, nface, npart, order;
+    const double* ppos;
+    const int* pmed;        // medium of each point (1-based)
+    const double* fpos;
+    const double* farea;
+    const int* fpart;
+    const int* inout;       // npart x 2 (inside, outside), 1-based
+    const double* kmat;
+    const int* rowptr;      // npt + 1, CSR of refined point-face pairs
+    const int* rcol;
+    const double* rg;
+    const double* rf;
+};
+
 // Built H-matrices of the four kinds (G1, G2, H1, H2) on one cluster tree.
 struct HMat {
     int n, nd, nlr;
@@ -57,6 +73,8 @@ void bem_operator_hmat(const HMat& h, const Face& f, const void* x, const void* 
                        double* norms, hipStream_t s);
 void surface_eval(const GreenGeom& a, int outside, double k0, const void* s, const void* hv, void* out,
                   hipStream_t st);
+void point_eval(const PointGeom& a, double k0, const void* s1, const void* h1, const void* s2, const void* h2,
+                void* out, hipStream_t st);
 void green_block(const GreenGeom& a, int outside, int key, int c0, int ncol, void* out, hipStream_t st);
 void green_pick(const GreenGeom& a, int outside, int key, const long* ind, long count, void* out, hipStream_t st);
 

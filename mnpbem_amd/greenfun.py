@@ -58,10 +58,22 @@ class CompGreenRet:
     ``deriv='cart'``."""
 
     def __init__(self, p1, p2, **op):
-        if p1 is not p2:
-            raise NotImplementedError("CompGreenRet on the device covers p1 is p2 (the BEM solve path)")
         if op.get("deriv", "cart") != "cart":
             raise NotImplementedError("only the Cartesian surface derivative (deriv='cart') is provided")
+        from .compoint import ComPoint
+        self._points = None
+        if isinstance(p1, ComPoint):
+            # observer points against a particle: potentials and fields at points
+            from .compoint import PointGreen
+            from .nearfield import ORDER
+            self.p1, self.p2, self.deriv = p1, p2, "cart"
+            self._points = PointGreen(p1, p2, device(), op.get("RelCutoff", 3.0), op.get("AbsCutoff", 0.0),
+                                      op.get("order", ORDER))
+            self.nearfield, self._geo = None, None
+            return
+        if p1 is not p2:
+            raise NotImplementedError("CompGreenRet on the device covers p1 is p2 (the BEM solve path) and "
+                                      "p1 = ComPoint with p2 a ComParticle (fields at points)")
         self.p1 = self.p2 = p1
         self.deriv = "cart"
         self.nearfield = NearField(p1, rel_cutoff=op.get("RelCutoff", 3.0), abs_cutoff=op.get("AbsCutoff", 0.0),
@@ -71,6 +83,8 @@ class CompGreenRet:
 
     @property
     def geo(self):
+        if self._points is not None:
+            raise NotImplementedError("for observer points the device path provides potential and field")
         if self._geo is None:
             self._geo = DeviceGeometry(self.p1, self.nearfield, device())
         return self._geo
@@ -88,6 +102,8 @@ class CompGreenRet:
         return g, kmat
 
     def _check(self, i, j, key):
+        if self._points is not None:
+            raise NotImplementedError("for observer points the device path provides potential and field")
         if key not in self._KEYS:
             raise ValueError("Unknown key: {}".format(key))
         if not np.any(self.con[i][j]):
@@ -207,8 +223,9 @@ class CompGreenRet:
     def potential(self, sig, inout=1):
         """Potentials and surface derivatives inside (inout=1) or outside (inout=2) of the
         particle from a BEMRet solution (compgreen_ret.py:975-1034): CompStruct with
-        phi1, phi1p, a1, a1p (or phi2, ...)."""
-        o = self._surface_eval(sig, inout)
+        phi1, phi1p, a1, a1p (or phi2, ...). For p1 = ComPoint the points have one region each
+        (their own medium), so ``inout`` only names the fields (phi1 ... for 1, phi2 ... else)."""
+        o = self._points.eval(sig, sig.enei) if self._points is not None else self._surface_eval(sig, inout)
         t = str(inout)
         return CompStruct(self.p1, sig.enei, **{"phi" + t: o[:, 0], "phi" + t + "p": o[:, 1],
                                                  "a" + t: o[:, 2:5], "a" + t + "p": o[:, 5:8]})
@@ -216,8 +233,9 @@ class CompGreenRet:
     def field(self, sig, inout=1):
         """Electric and magnetic fields inside (inout=1) or outside (inout=2) of the particle
         surface from a BEMRet solution (compgreen_ret.py:911-973): CompStruct with e, h
-        (n, 3[, npol])."""
-        o = self._surface_eval(sig, inout)
+        (n, 3[, npol]). For p1 = ComPoint: the scattered fields at the points (``inout`` is not
+        used: each point sees the side of the surface that faces its medium)."""
+        o = self._points.eval(sig, sig.enei) if self._points is not None else self._surface_eval(sig, inout)
         return CompStruct(self.p1, sig.enei, e=o[:, 8:11], h=o[:, 11:14])
 
 

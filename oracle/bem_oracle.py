@@ -21,12 +21,18 @@ the reference algorithm of MNPBEM's retarded direct solver:
     plane wave            simulation/planewave_ret.py:257-361, :391-504
     far field             spectrum/spectrum_ret.py:231-459
     8N operator           bem/bem_ret_iter.py:1196-1232, :1362-1501
+    surface fields        greenfun/compgreen_ret.py:911-1040 (Gp, H1p, H2p: greenret_refined.py:812-866)
+    fields at points      geometry/compoint.py:262-297 (media), greenfun/compgreen_ret.py:911-1034,
+                          refine_utils.py:89-159 and greenret_refined.py:287-420 (point-face pairs)
 
 Parity pins (see tests/test_oracle.py): the reference's MATLAB validation
 tables for trisphere(144, 20) in gold (ext/sca/abs over 41 wavelengths), the
 Mie series for a sphere (pinned itself by the reference's Mie table), quadrature exactness, the Fuchs-Liu
 closed-surface identity, brute-force integration of the refined
-coefficients on fine rules, LU backward error, and ext = sca + abs >= 0.
+coefficients on fine rules, LU backward error, and ext = sca + abs >= 0; surface
+fields by n.Hp = H and the boundary conditions of a BEM solution; fields at points
+by the far-field limit of the pinned far-field amplitude and by continuity with the
+surface fields.
 
 Inputs are plain arrays: corner ``verts``/``faces`` (NaN in column 3 for
 triangles) and curved ``verts2``/``faces2`` (9 columns), e.g. from
@@ -660,3 +666,77 @@ def surface_potential_field(green, ks, enei, sig1, h1, sig2, h2, inout):
 
     hf = cross(P1, h1) + cross(P2, h2)
     return dict(phi=phi, phip=phip, a=a, ap=ap, e=e, h=hf)
+
+
+def point_media(s, fp, inout, pts):
+    """Medium of each point: the side of the nearest face (outside when (x - x_face) . n >= 0),
+    compoint.py:262-297 with the nearest face over all particles."""
+    d = np.sqrt(np.sum((pts[:, None, :] - s.pos[None, :, :]) ** 2, axis=2))
+    j = np.argmin(d, axis=1)
+    side = np.sum((pts - s.pos[j]) * s.nvec[j], axis=1) >= 0
+    io = np.asarray(inout)
+    return np.where(side, io[fp[j], 1], io[fp[j], 0])
+
+
+def point_face_coefficients(s, x, j, order=5):
+    """Taylor coefficients in ik of G and Gp between observer x and face j from the face's
+    quadrature points around the centroid distance r0 (greenret_refined.py:287-420):
+    g_n = sum_q w_q (r_q - r0)^n / (n! r_q), f_n the gradient of the same sum in x."""
+    qp, qw = s.quad_points(j)
+    v = x - qp
+    r = np.maximum(np.sqrt(np.sum(v * v, axis=1)), EPSF)
+    dl = r - np.linalg.norm(x - s.pos[j])
+    g = np.zeros(order + 1)
+    f = np.zeros((3, order + 1))
+    for n in range(order + 1):
+        g[n] = np.sum(qw * dl ** n / r) / factorial(n)
+        sc = -1.0 / r ** 3 if n == 0 else -dl ** n / (r ** 3 * factorial(n)) + dl ** (n - 1) / (r ** 2 * factorial(n - 1))
+        f[:, n] = (qw * sc) @ v
+    return g, f
+
+
+def point_potential_field(s, fp, inout, pts, ks, enei, sig1, h1, sig2, h2, order=5, rel=3.0, absc=0.0):
+    """Potentials and fields at points from a solution: CompGreenRet(ComPoint, ComParticle)
+    .potential/.field (compgreen_ret.py:911-1034). A point in medium m sees faces whose inside
+    medium is m through sig1, h1 and faces whose outside medium is m through sig2, h2. Pairs
+    with (d - rad_j) / rad_j < rel (refine_utils.py:89-159, radius of the source face) are
+    Taylor-integrated over the source face's quadrature points (greenret_refined.py:287-420).
+    Returns phi (np,), a, e, h (np, 3)."""
+    med = point_media(s, fp, inout, pts)
+    io = np.asarray(inout)
+    npt = len(pts)
+    k0 = TWO_PI / enei
+    phi = np.zeros(npt, complex)
+    a = np.zeros((npt, 3), complex)
+    e = np.zeros((npt, 3), complex)
+    hf = np.zeros((npt, 3), complex)
+    ikn = lambda k: (1j * k) ** np.arange(order + 1)
+    for i in range(npt):
+        m = med[i]
+        k = ks[m - 1]
+        for j in range(s.n):
+            if io[fp[j], 0] == m:
+                sv, hv = sig1[j], h1[j]
+            elif io[fp[j], 1] == m:
+                sv, hv = sig2[j], h2[j]
+            else:
+                continue
+            x = pts[i] - s.pos[j]
+            d = np.linalg.norm(x)
+            if d == 0:
+                continue
+            ph = np.exp(1j * k * d)
+            dr = d - s.rad[j]
+            if dr < absc or dr / s.rad[j] < rel:
+                g, f = point_face_coefficients(s, pts[i], j, order)
+                G = (g @ ikn(k)) * ph
+                Gp = (f @ ikn(k)) * ph
+            else:
+                G = s.area[j] / d * ph
+                Gp = x * ((1j * k - 1 / d) / d ** 2 * s.area[j]) * ph
+            phi[i] += G * sv
+            a[i] += G * hv
+            e[i] -= Gp * sv
+            hf[i] += np.cross(Gp, hv)
+    e += 1j * k0 * a
+    return dict(phi=phi, a=a, e=e, h=hf, medium=med)

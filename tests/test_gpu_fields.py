@@ -128,3 +128,45 @@ def test_eval_keys_blocks_and_indices(parts):
         g.eval_block(0, 0, "G", lam, 5, n + 1)
     with pytest.raises(ValueError):
         g.eval(0, 0, "Q", lam)
+
+
+@pytest.mark.parametrize("case", ["sphere", "dimer"])
+def test_point_fields_match_oracle(case):
+    """Fields and potentials at points (ComPoint, CompGreenRet(pt, p)) against the oracle:
+    points inside and outside, some within the refinement cutoff of a face, two polarisations
+    for the dimer; media of the points as the oracle's."""
+    rng = np.random.default_rng(7)
+    if case == "sphere":
+        cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+        d = rng.standard_normal((40, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        rr = np.concatenate([rng.uniform(10.6, 14.0, 20), rng.uniform(15.0, 60.0, 12), rng.uniform(1.0, 8.0, 8)])
+        pts = d * rr[:, None]
+        exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+        inout = [[2, 1]]
+    else:
+        a = M.trisphere(60, 10)
+        b = M.trisphere(60, 10).shift([0, 0, 11.0])
+        cp, green = _setup([a, b], [[2, 1], [2, 1]], [M.EpsConst(1.33), M.EpsDrude(*DRUDE)])
+        pts = np.column_stack([rng.uniform(-3, 3, 30), rng.uniform(-3, 3, 30), rng.uniform(4.0, 7.0, 30)])
+        pts = np.vstack([pts, [[0, 0, -12.0], [8.0, 0, 5.5], [0, 0, 0.5], [0, 0, 11.0]]])
+        exc = M.PlaneWaveRet(np.array([[1.0, 0, 0], [0, 0, 1.0]]), np.array([[0, 0, 1.0], [1.0, 0, 0]]))
+        inout = [[2, 1], [2, 1]]
+    lam = 600.0
+    banner("point fields {} npt={}".format(case, len(pts)))
+    sig, _ = M.BEMRet(cp).solve(exc.potential(cp, lam))
+    pt = M.ComPoint(cp, pts)
+    g = M.CompGreenRet(pt, cp)
+    assert g._points.n_ref > 0                       # some pairs are refined
+    fld, pot = g.field(sig), g.potential(sig)
+    ks = [e(lam)[1] for e in cp.eps]
+    s = green.s
+    assert np.array_equal(pt.medium, O.point_media(s, green.fp, inout, pt.pos))
+    npol = 1 if np.ndim(sig.sig1) == 1 else sig.sig1.shape[1]
+    for ip in range(npol):
+        col = (lambda v: v) if npol == 1 else (lambda v: v[..., ip])
+        ref = O.point_potential_field(s, green.fp, inout, pt.pos, ks, lam, col(sig.sig1), col(sig.h1),
+                                      col(sig.sig2), col(sig.h2))
+        for got, want in ((col(fld.e), ref["e"]), (col(fld.h), ref["h"]), (col(pot.phi1), ref["phi"]),
+                          (col(pot.a1), ref["a"])):
+            assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10

@@ -246,3 +246,46 @@ def test_surface_fields_pinned_by_boundary_conditions(gold_sphere):
     # the surface derivatives of the potentials are the normal parts of their gradients
     assert np.allclose(F1["phip"], np.einsum("ic,icj->ij", n, g.eval(0, 0, "H1p", ks)) @ s1, rtol=0, atol=1e-12 *
                        np.abs(F1["phip"]).max())
+
+
+def test_point_fields_pinned(gold_sphere):
+    """Pins of point_potential_field (fields at points, compgreen_ret.py:911-1034 for
+    p1 = ComPoint): (1) far away the field tends to e^{ikR}/R times the far-field amplitude of
+    spectrum_ret.py:231-392 (itself pinned by the MATLAB cross sections); (2) the near-pair
+    coefficients of a point 0.7 element radii above a face are the integrals of 1/r and -r/r^3
+    over that face (subdivided rule) to the accuracy of the 28-point rule; (3) media: points
+    inside the sphere get medium 2, outside medium 1."""
+    ml, out = gold_sphere
+    g, f = out["green"], out["fields"][5]
+    s = g.s
+    lam = ml[5, 0]
+    ks = [O.eps_const(lam, 1.0)[1], O.eps_table(lam, GOLD)[1]]
+    s1, h1, s2, h2 = f["sol"]
+    fp = np.zeros(s.n, int)
+    dirs = np.array([[0, 0, 1.0], [1.0, 0, 0], [0.6, 0.0, 0.8], [0.3, -0.5, 0.812404]])
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    k = ks[0]
+    ef, hfar = O.farfield(s.pos, s.area, dirs, lam, k, s1, h1, s2, h2, np.zeros(0, int), np.arange(s.n))
+    errs = []
+    for R in (1e5, 1e6):
+        F = O.point_potential_field(s, fp, [[2, 1]], R * dirs, ks, lam, s1, h1, s2, h2)
+        sc = R * np.exp(-1j * k * R)
+        err_e = np.linalg.norm(F["e"] * sc - ef) / np.linalg.norm(ef)
+        err_h = np.linalg.norm(F["h"] * sc - hfar) / np.linalg.norm(hfar)
+        # the near-zone terms fall off as 1 / (k R) relative to the radiating ones
+        assert err_e < 3.0 / (k.real * R) and err_h < 3.0 / (k.real * R), (R, err_e, err_h)
+        errs.append(err_e)
+    assert 0.05 < errs[1] / errs[0] < 0.2
+    for j in (0, 17, 140):
+        x = s.pos[j] + 0.7 * s.rad[j] * s.nvec[j]
+        gq, fq = O.point_face_coefficients(s, x, j)
+        sv = s.pos[j]
+        s.pos[j] = x                                   # _brute integrates over face j seen from s.pos[i]
+        try:
+            gb, fb = _brute(s, j, j)
+        finally:
+            s.pos[j] = sv
+        assert abs(gq[0] - gb) / abs(gb) < 5e-3
+        assert np.linalg.norm(fq[:, 0] - fb) / np.linalg.norm(fb) < 5e-2
+    pts = np.array([[0, 0, 0.0], [3.0, 2.0, -1.0], [0, 0, 15.0], [25.0, 0, 0]])
+    assert list(O.point_media(s, fp, [[2, 1]], pts)) == [2, 2, 1, 1]
