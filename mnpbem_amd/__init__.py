@@ -8,7 +8,7 @@ from .materials import EpsConst, EpsDrude, EpsTable
 from .particle import Particle
 from .mesh import trisphere, trirod, trispheresegment, triellipsoid, fvgrid
 from .comparticle import ComParticle
-from .compoint import ComPoint
+from .compoint import ComPoint, MeshField
 from .greenfun import CompGreenRet, CompGreenStat, CompStruct
 from .bem import BEMRet
 from .bem_stat import BEMStat, PlaneWaveStat
@@ -16,6 +16,6 @@ from .bem_stat_iter import ACACompGreenStat, BEMStatIter
 from .planewave import PlaneWaveRet, SpectrumRet
 
 __all__ = ["EpsConst", "EpsDrude", "EpsTable", "Particle", "trisphere", "trirod", "trispheresegment",
-           "triellipsoid", "fvgrid", "ComParticle", "ComPoint", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMStat",
+           "triellipsoid", "fvgrid", "ComParticle", "ComPoint", "MeshField", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMStat",
            "BEMStatIter", "ACACompGreenStat", "PlaneWaveStat", "PlaneWaveRet",
            "SpectrumRet"]

@@ -145,3 +145,64 @@ class PointGreen:
         o = np.transpose(out.cpu().numpy(), (1, 2, 0))
         del kmat
         return o[:, :, 0] if single else o
+
+
+def _expand_grid(x, y, z):
+    """Grid arrays of one shape from x, y, z as the reference's meshfield (meshfield.py:
+    expand): equal shapes stay; two equal shapes and a scalar third broadcast; two equal
+    shapes and a 1-D third add a trailing axis over the third."""
+    arrs = [np.asarray(v, dtype=np.float64) for v in (x, y, z)]
+    shapes = [a.shape for a in arrs]
+    if len(set(shapes)) == 1:
+        return arrs
+    for i, j, k in ((0, 1, 2), (0, 2, 1), (1, 2, 0)):
+        if shapes[i] == shapes[j]:
+            base, third = arrs[i], arrs[k]
+            out = [None, None, None]
+            if third.size == 1:
+                out[i], out[j] = arrs[i], arrs[j]
+                out[k] = np.zeros_like(base) + float(third.ravel()[0])
+            else:
+                n2 = third.size
+                tile = lambda a: np.repeat(a[..., None], n2, axis=-1)
+                out[i], out[j] = tile(arrs[i]), tile(arrs[j])
+                out[k] = np.broadcast_to(third.ravel(), base.shape + (n2,)).copy()
+            return out
+    return np.broadcast_arrays(*arrs)
+
+
+class MeshField:
+    """Scattered fields of a retarded BEM solution on a grid (simulation/meshfield.py:18-520,
+    ``sim='ret'``): grid points become a ComPoint (``mindist`` drops points near the surface;
+    their values are NaN), fields come from ``CompGreenRet(ComPoint, p).field`` on the device,
+    shaped as the grid + (3,). The incident field is the excitation's and is not added."""
+
+    def __init__(self, p, x, y, z=None, nmax=None, mindist=None, sim="stat", **options):
+        if sim != "ret":
+            raise NotImplementedError("MeshField on the device covers sim='ret' (retarded fields)")
+        from .greenfun import CompGreenRet
+        self.x, self.y, self.z = _expand_grid(x, y, 0.0 if z is None else z)
+        self.p, self.nmax = p, nmax
+        pos = np.column_stack([self.x.ravel(), self.y.ravel(), self.z.ravel()])
+        self.pt = ComPoint(p, pos, mindist=mindist)
+        self.g = CompGreenRet(self.pt, p, **options)
+        self._npos = len(pos)
+
+    def _grid(self, v):
+        v = np.asarray(v)
+        out = np.full((self._npos,) + v.shape[1:], np.nan + 0j, dtype=complex)
+        out[self.pt.index] = v
+        return out.reshape(self.x.shape + v.shape[1:])
+
+    def field(self, sig, inout=2):
+        """(e, h): scattered fields on the grid, shape grid + (3,) [+ (npol,)]."""
+        f = self.g.field(sig, inout)
+        return self._grid(f.e), self._grid(f.h)
+
+    __call__ = field
+
+    def potential(self, sig, inout=2):
+        """phi and a on the grid (phi: grid [+ (npol,)], a: grid + (3,) [+ (npol,)])."""
+        pot = self.g.potential(sig, inout)
+        t = str(inout)
+        return self._grid(pot["phi" + t]), self._grid(pot["a" + t])

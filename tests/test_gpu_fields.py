@@ -170,3 +170,27 @@ def test_point_fields_match_oracle(case):
         for got, want in ((col(fld.e), ref["e"]), (col(fld.h), ref["h"]), (col(pot.phi1), ref["phi"]),
                           (col(pot.a1), ref["a"])):
             assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
+
+
+def test_meshfield_grid():
+    """MeshField (sim='ret'): grid + (3,) fields equal CompGreenRet(ComPoint).field at the kept
+    points, NaN where mindist drops points; grid expansion of an (x, z) plane at y = 0."""
+    cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+    lam = 520.0
+    sig, _ = M.BEMRet(cp).solve(M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0]).potential(cp, lam))
+    x, z = np.meshgrid(np.linspace(-20, 20, 17), np.linspace(-20, 20, 13))
+    banner("meshfield {}".format(x.shape))
+    mf = M.MeshField(cp, x, 0.0, z, mindist=0.5, sim="ret")
+    e, h = mf(sig)
+    assert e.shape == x.shape + (3,) and h.shape == x.shape + (3,)
+    pos = np.column_stack([x.ravel(), np.zeros(x.size), z.ravel()])
+    pt = M.ComPoint(cp, pos, mindist=0.5)
+    ref = M.CompGreenRet(pt, cp).field(sig)
+    ef = e.reshape(-1, 3)
+    assert np.allclose(ef[pt.index], ref.e, rtol=0, atol=1e-12 * np.abs(ref.e).max())
+    dropped = np.setdiff1d(np.arange(x.size), pt.index)
+    assert np.all(np.isnan(ef[dropped]))
+    r = np.linalg.norm(pos, axis=1)
+    assert np.all(np.abs(r[dropped] - 10.0) < 1.0)       # only points next to the surface are dropped
+    with pytest.raises(NotImplementedError):
+        M.MeshField(cp, x, 0.0, z)                         # the quasistatic default is not provided
