@@ -117,19 +117,31 @@ class PointGreen:
         self.rg, self.rf = T(g, torch.float64), T(f, torch.float64)
         self.n_ref = len(rows)
 
-    def eval(self, sig, enei):
-        """(npt, 14, npol) or (npt, 14) complex: phi, phip, a, ap, e, h at the points."""
+    def eval(self, sig, enei, static=False):
+        """(npt, 14, npol) or (npt, 14) complex: phi, phip, a, ap, e, h at the points.
+        ``static``: the quasistatic Green function 1/r of every face with the one charge
+        ``sig.sig`` (CompGreenStat(ComPoint, p), compgreen_stat.py:974-1050): k = 0, every
+        point connected to every face, no currents; then e = -Gp sig."""
         from ._lib import call, PointGeomC
         from .engine import stream_handle
         from .hmatrix import C_addr
         p, n = self.p, self.p.nfaces
         enei = float(enei)
-        kmat = torch.tensor(np.array([complex(e(enei)[1]) for e in p.eps]), dtype=torch.complex128, device=self.dev)
+        if static:
+            kmat = torch.zeros(1, dtype=torch.complex128, device=self.dev)
+            pmed = torch.ones_like(self.pmed)
+            inout = torch.ones_like(self.inout)
+            s1 = s2 = np.asarray(sig.sig, complex)
+            h1 = h2 = np.zeros(s1.shape[:1] + (3,) + s1.shape[1:], complex)
+        else:
+            kmat = torch.tensor(np.array([complex(e(enei)[1]) for e in p.eps]), dtype=torch.complex128,
+                                device=self.dev)
+            pmed, inout = self.pmed, self.inout
+            s1, h1 = np.asarray(sig.sig1, complex), np.asarray(sig.h1, complex)
+            s2, h2 = np.asarray(sig.sig2, complex), np.asarray(sig.h2, complex)
         g = PointGeomC(self.pt.n, n, len(p.inout), self.order, *[t.data_ptr() for t in (
-            self.ppos, self.pmed, self.fpos, self.farea, self.fpart, self.inout, kmat, self.rowptr, self.rcol,
+            self.ppos, pmed, self.fpos, self.farea, self.fpart, inout, kmat, self.rowptr, self.rcol,
             self.rg, self.rf)])
-        s1, h1 = np.asarray(sig.sig1, complex), np.asarray(sig.h1, complex)
-        s2, h2 = np.asarray(sig.sig2, complex), np.asarray(sig.h2, complex)
         single = s1.ndim == 1
         s1, s2 = s1.reshape(n, -1), s2.reshape(n, -1)
         h1, h2 = h1.reshape(n, 3, -1), h2.reshape(n, 3, -1)
@@ -143,7 +155,7 @@ class PointGreen:
             call("mnp_point_eval", C_addr(g), 2 * np.pi / enei, ts1[q].data_ptr(), th1[q].data_ptr(),
                  ts2[q].data_ptr(), th2[q].data_ptr(), out[q].data_ptr(), st)
         o = np.transpose(out.cpu().numpy(), (1, 2, 0))
-        del kmat
+        del kmat, pmed, inout
         return o[:, :, 0] if single else o
 
 
@@ -172,20 +184,21 @@ def _expand_grid(x, y, z):
 
 
 class MeshField:
-    """Scattered fields of a retarded BEM solution on a grid (simulation/meshfield.py:18-520,
-    ``sim='ret'``): grid points become a ComPoint (``mindist`` drops points near the surface;
-    their values are NaN), fields come from ``CompGreenRet(ComPoint, p).field`` on the device,
-    shaped as the grid + (3,). The incident field is the excitation's and is not added."""
+    """Scattered fields of a BEM solution on a grid (simulation/meshfield.py:18-520): grid
+    points become a ComPoint (``mindist`` drops points near the surface; their values are NaN),
+    fields come from ``CompGreenRet(ComPoint, p).field`` (``sim='ret'``) or
+    ``CompGreenStat(ComPoint, p).field`` (``sim='stat'``, the reference's default) on the
+    device, shaped as the grid + (3,). The incident field is the excitation's and is not added."""
 
     def __init__(self, p, x, y, z=None, nmax=None, mindist=None, sim="stat", **options):
-        if sim != "ret":
-            raise NotImplementedError("MeshField on the device covers sim='ret' (retarded fields)")
-        from .greenfun import CompGreenRet
+        if sim not in ("stat", "ret"):
+            raise ValueError("MeshField: sim must be 'stat' or 'ret'")
+        from .greenfun import CompGreenRet, CompGreenStat
         self.x, self.y, self.z = _expand_grid(x, y, 0.0 if z is None else z)
-        self.p, self.nmax = p, nmax
+        self.p, self.nmax, self.sim = p, nmax, sim
         pos = np.column_stack([self.x.ravel(), self.y.ravel(), self.z.ravel()])
         self.pt = ComPoint(p, pos, mindist=mindist)
-        self.g = CompGreenRet(self.pt, p, **options)
+        self.g = (CompGreenRet if sim == "ret" else CompGreenStat)(self.pt, p, **options)
         self._npos = len(pos)
 
     def _grid(self, v):
@@ -195,14 +208,16 @@ class MeshField:
         return out.reshape(self.x.shape + v.shape[1:])
 
     def field(self, sig, inout=2):
-        """(e, h): scattered fields on the grid, shape grid + (3,) [+ (npol,)]."""
+        """(e, h): scattered fields on the grid, shape grid + (3,) [+ (npol,)]; h is None for
+        sim='stat'."""
         f = self.g.field(sig, inout)
-        return self._grid(f.e), self._grid(f.h)
+        return self._grid(f.e), (self._grid(f.h) if "h" in f else None)
 
     __call__ = field
 
     def potential(self, sig, inout=2):
-        """phi and a on the grid (phi: grid [+ (npol,)], a: grid + (3,) [+ (npol,)])."""
+        """phi and a on the grid (phi: grid [+ (npol,)], a: grid + (3,) [+ (npol,)]; a is None
+        for sim='stat')."""
         pot = self.g.potential(sig, inout)
         t = str(inout)
-        return self._grid(pot["phi" + t]), self._grid(pot["a" + t])
+        return self._grid(pot["phi" + t]), (self._grid(pot["a" + t]) if ("a" + t) in pot else None)

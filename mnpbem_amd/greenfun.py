@@ -253,8 +253,19 @@ class CompGreenStat:
     otherwise it removes the correction that the retarded set-up folds in by default."""
 
     def __init__(self, p1, p2, **op):
+        from .compoint import ComPoint
+        self._points = None
+        if isinstance(p1, ComPoint):
+            # observer points: potentials and fields at points (k = 0 Green function)
+            from .compoint import PointGreen
+            from .nearfield import ORDER
+            self.p1, self.p2 = p1, p2
+            self._points = PointGreen(p1, p2, device(), op.get("RelCutoff", 3.0), op.get("AbsCutoff", 0.0),
+                                      op.get("order", ORDER))
+            return
         if p1 is not p2:
-            raise NotImplementedError("CompGreenStat on the device covers p1 is p2 (the BEM solve path)")
+            raise NotImplementedError("CompGreenStat on the device covers p1 is p2 (the BEM solve path) and "
+                                      "p1 = ComPoint with p2 a ComParticle (fields at points)")
         self.p1 = self.p2 = p1
         self.green = op.get("green") or CompGreenRet(p1, p1, **{k: v for k, v in op.items()
                                                                 if k in ("RelCutoff", "AbsCutoff", "order")})
@@ -290,3 +301,23 @@ class CompGreenStat:
     def F(self):
         m = GreenBatch.to_numpy(self._assemble().H1, 0).real.copy()
         return m + np.diag(self.diag_shift - 2 * np.pi)
+
+    def potential(self, sig, inout=1):
+        """phi = G sig, phip = H sig (compgreen_stat.py:1011-1050); for p1 = ComPoint on the
+        device (phip = 0 for bare points), else from the host matrices."""
+        t = "1" if inout == 1 else "2"
+        if self._points is not None:
+            o = self._points.eval(sig, sig.enei, static=True)
+            return CompStruct(self.p1, sig.enei, **{"phi" + t: o[:, 0], "phi" + t + "p": o[:, 1]})
+        s = np.asarray(sig.sig)
+        H = self.F + (2 * np.pi if inout == 1 else -2 * np.pi) * np.eye(s.shape[0])
+        return CompStruct(self.p1, sig.enei, **{"phi" + t: self.G @ s, "phi" + t + "p": H @ s})
+
+    def field(self, sig, inout=1):
+        """e = -Hp sig (compgreen_stat.py:974-1009); provided for p1 = ComPoint (fields at
+        points, where Hp = Gp)."""
+        if self._points is None:
+            raise NotImplementedError("CompGreenStat.field on the device covers p1 = ComPoint")
+        o = self._points.eval(sig, sig.enei, static=True)
+        return CompStruct(self.p1, sig.enei, e=o[:, 8:11])
+

@@ -192,5 +192,39 @@ def test_meshfield_grid():
     assert np.all(np.isnan(ef[dropped]))
     r = np.linalg.norm(pos, axis=1)
     assert np.all(np.abs(r[dropped] - 10.0) < 1.0)       # only points next to the surface are dropped
-    with pytest.raises(NotImplementedError):
-        M.MeshField(cp, x, 0.0, z)                         # the quasistatic default is not provided
+    with pytest.raises(ValueError):
+        M.MeshField(cp, x, 0.0, z, sim="layer")
+
+
+def test_stat_point_fields():
+    """Quasistatic fields at points (CompGreenStat(ComPoint, p).field = -Gp sig with G = 1/r,
+    compgreen_stat.py:974-1050): against the oracle at k = 0 (1e-10), and far away against
+    the field of the induced dipole p = sum a_j r_j sig_j, (3 (p.n) n - p) / R^3, to the
+    quadrupole order (size / R)."""
+    cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+    lam = 520.0
+    sig, _ = M.BEMStat(cp).solve(M.PlaneWaveStat([1.0, 0, 0])(cp, lam))
+    rng = np.random.default_rng(11)
+    d = rng.standard_normal((30, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    pts = d * np.concatenate([rng.uniform(10.5, 14.0, 15), rng.uniform(1.0, 8.0, 10), rng.uniform(20, 40, 5)])[:, None]
+    banner("stat point fields npt={}".format(len(pts)))
+    pt = M.ComPoint(cp, pts)
+    g = M.CompGreenStat(pt, cp)
+    e = g.field(sig).e
+    phi = g.potential(sig).phi1
+    s = green.s
+    ref = O.point_potential_field(s, green.fp, [[2, 1]], pt.pos, [0.0, 0.0], lam, sig.sig, np.zeros((s.n, 3)),
+                                  sig.sig, np.zeros((s.n, 3)))
+    assert np.max(np.abs(e - ref["e"])) / np.max(np.abs(ref["e"])) < 1e-10
+    assert np.max(np.abs(phi - ref["phi"])) / np.max(np.abs(ref["phi"])) < 1e-10
+    R = 1e4
+    far = M.ComPoint(cp, R * d[:8])
+    ef = M.CompGreenStat(far, cp).field(sig).e
+    pdip = (cp.area[:, None] * cp.pos).T @ sig.sig
+    nn = far.pos / R
+    edip = (3 * nn * (nn @ pdip)[:, None] - pdip[None, :]) / R ** 3
+    assert np.linalg.norm(ef - edip) / np.linalg.norm(edip) < 30.0 / R
+    x, z = np.meshgrid(np.linspace(-20, 20, 9), np.linspace(-20, 20, 7))
+    em, hm = M.MeshField(cp, x, 0.0, z, mindist=0.5)(sig)
+    assert em.shape == x.shape + (3,) and hm is None
