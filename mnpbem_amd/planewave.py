@@ -104,6 +104,31 @@ class PlaneWaveRet:
 
     __call__ = potential
 
+    def field(self, p, enei, inout=1):
+        """Incident fields e = pol e^{ik dir.r}, h = n_b dir x e (planewave_ret.py:157-240) on the
+        faces of a ComParticle whose side ``inout`` (1 inside, 2 outside) is the excitation
+        medium, or at the points of a ComPoint that lie in it; zero elsewhere. (n, 3) for one
+        polarisation, (n, 3, npol) otherwise."""
+        from .compoint import ComPoint
+        if not np.allclose(np.sum(self.pol * self.dir, axis=1), 0):
+            raise ValueError("Polarization and propagation direction must be orthogonal")
+        eps, _ = p.eps[self.medium - 1](enei)
+        nb = np.sqrt(complex(np.asarray(eps).ravel()[0]))
+        k = 2 * np.pi / enei * nb
+        if isinstance(p, ComPoint):
+            n, ind = p.n, np.nonzero(p.medium == self.medium)[0]
+        else:
+            n, ind = p.nfaces, np.nonzero(p.inout_faces[:, inout - 1] == self.medium)[0]
+        npol = self.pol.shape[0]
+        e = np.zeros((n, 3, npol), complex)
+        h = np.zeros((n, 3, npol), complex)
+        for i in range(npol):
+            e[ind, :, i] = np.exp(1j * k * (p.pos[ind] @ self.dir[i]))[:, None] * self.pol[i]
+            h[:, :, i] = nb * np.cross(self.dir[i][None, :], e[:, :, i])
+        if npol == 1:
+            e, h = e[:, :, 0], h[:, :, 0]
+        return CompStruct(p, enei, e=e, h=h)
+
     def _farfield(self, sig, dirs, ipol):
         p = sig.p
         geo = _geo_for(p)

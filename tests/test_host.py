@@ -100,3 +100,23 @@ def test_cabi_rejects_bad_arguments_without_gpu_work():
     with pytest.raises(ValueError):
         _lib.call("mnp_bemret_factor", 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 32, None)
     assert _lib.lib.mnp_rsolve_batched(0, 4, 4, 1, 1, 4, 16, 1, 4, 5, 4, 16, 5, 4, 16, 32, None) == -13  # B == X
+
+
+def test_planewave_field_at_points_and_faces():
+    """PlaneWaveRet.field (planewave_ret.py:157-240): e = pol e^{ik dir.r}, h = n_b dir x e in
+    the excitation medium, zero elsewhere; for faces the chosen side selects them."""
+    import mnpbem_amd as M
+    cp = M.ComParticle([M.EpsConst(1.77), M.EpsDrude(10, 9.065, 0.0708)], [M.trisphere(60, 10)], [[2, 1]], 1)
+    pts = np.array([[0, 0, 0.0], [0, 0, 9.0], [7.0, 1.0, -2.0]])
+    pt = M.ComPoint(cp, pts)
+    pw = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    f = pw.field(pt, 600.0)
+    nb = np.sqrt(1.77)
+    k = 2 * np.pi / 600.0 * nb
+    out = pt.medium == 1
+    assert np.allclose(f.e[out], np.exp(1j * k * pt.pos[out, 2])[:, None] * np.array([1.0, 0, 0]))
+    assert np.allclose(f.h[out], nb * np.cross([0, 0, 1.0], f.e[out]))
+    assert np.all(f.e[~out] == 0) and np.any(~out)
+    fo = pw.field(cp, 600.0, inout=2)
+    assert np.allclose(fo.e[:, 0], np.exp(1j * k * cp.pos[:, 2]))
+    assert np.all(pw.field(cp, 600.0, inout=1).e == 0)
