@@ -32,10 +32,10 @@ DRUDE = (10.0, 9.065, 0.0708)
 ROD_N = [25, 11, 34]
 
 
-# HBM bytes per k_gemm_sub_c64p<3,1,1> launch of this sweep (batch 201, 1022 launches) from
+# HBM bytes per k_gemm_sub_c64p<3,1,1> launch of this sweep (batch 201, 478 launches) from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a 256 MiB
 # copy in the same run (FETCH x2.0, WRITE x1.0); the commands and raw averages are in the file named.
-GEMM_TRAFFIC_BYTES = int((717486.74 * 2.0 + 917543.37 * 1.0) * 1024)
+GEMM_TRAFFIC_BYTES = int((2639095.59 * 2.0 + 3445227.46 * 1.0) * 1024)
 GEMM_TRAFFIC_SOURCE = "profiles/gemm_pmc_rod2496.md"
 
 
