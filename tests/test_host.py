@@ -120,3 +120,36 @@ def test_planewave_field_at_points_and_faces():
     fo = pw.field(cp, 600.0, inout=2)
     assert np.allclose(fo.e[:, 0], np.exp(1j * k * cp.pos[:, 2]))
     assert np.all(pw.field(cp, 600.0, inout=1).e == 0)
+
+
+def test_compoint_media_and_grid_expansion():
+    """ComPoint media against the oracle's rule (nearest face over all particles, outside when
+    (x - x_face) . n >= 0), mindist drops points near the surface, medium filter, ordering by
+    medium with the original indices; MeshField grid expansion as meshfield.py:expand."""
+    import mnpbem_amd as M
+    from mnpbem_amd.compoint import _expand_grid
+    from oracle import bem_oracle as O
+    a = M.trisphere(60, 10)
+    b = M.trisphere(60, 10).shift([0, 0, 11.0])
+    cp = M.ComParticle([M.EpsConst(1.33), M.EpsDrude(10, 9.065, 0.0708), M.EpsConst(2.0)], [a, b],
+                       [[2, 1], [3, 1]], 1, 2)
+    rng = np.random.default_rng(5)
+    pts = np.column_stack([rng.uniform(-8, 8, 200), rng.uniform(-8, 8, 200), rng.uniform(-8, 19, 200)])
+    pt = M.ComPoint(cp, pts)
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+    want = O.point_media(s, cp.face_particle, [[2, 1], [3, 1]], pts)
+    assert np.array_equal(pt.medium, want[pt.index])
+    assert sorted(pt.index.tolist()) == list(range(200))
+    assert np.all(np.diff(pt.medium) >= 0) and set(pt.inout) == {1, 2, 3}
+    assert np.array_equal(pt.pos, pts[pt.index])
+    near = M.ComPoint(cp, pts, mindist=1.5)
+    dmin = np.min(np.linalg.norm(pts[:, None, :] - cp.pos[None, :, :], axis=2), axis=1)
+    assert set(near.index.tolist()) == set(np.nonzero(dmin >= 1.5)[0].tolist())
+    only3 = M.ComPoint(cp, pts, medium=3)
+    assert np.all(only3.medium == 3) and only3.n == np.sum(want == 3)
+    x, z = np.meshgrid(np.linspace(-1, 1, 5), np.linspace(-2, 2, 4))
+    gx, gy, gz = _expand_grid(x, 0.0, z)
+    assert gx.shape == gy.shape == gz.shape == x.shape and np.all(gy == 0)
+    gx, gy, gz = _expand_grid(x, z, np.array([0.0, 1.0, 2.0]))
+    assert gx.shape == (4, 5, 3) and np.array_equal(gz[1, 2], [0.0, 1.0, 2.0])
