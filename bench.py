@@ -298,8 +298,8 @@ def main():
     ap.add_argument("--no-warmstart", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this pool has one MI355X and the wavelength sweep "
-              "is sharded across GPUs only by design (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the wavelength sweep would shard by giving each GPU its own "
+              "wavelength batches (one process per GPU), which is not built; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
