@@ -304,8 +304,9 @@ __global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>*
 }
 
 // Panel factorisation spread over the whole device (cooperative launch, grid G x batch).
-// Each thread holds one row of the (n - j0) x w panel (w <= 32) in registers, 256 rows per
-// workgroup. Per column: every workgroup publishes its best pivot candidate (value, row index
+// Each thread holds one row of the (n - j0) x w panel (w <= 32) in registers, 1024 rows per
+// workgroup for complex float, 256 for complex double (few workgroups per matrix: the
+// per-column barrier's atomics on one counter serialise, so its cost grows with their number). Per column: every workgroup publishes its best pivot candidate (value, row index
 // and the candidate row itself) and the workgroup holding the diagonal row publishes that row;
 // after one barrier among the matrix's workgroups every workgroup picks the same pivot
 // (largest |re|+|im|, smallest row on ties), the two rows trade places and all rows below the
@@ -316,7 +317,17 @@ __global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>*
 // starts): each workgroup adds 1 with release semantics and waits (acquire) until all G have
 // arrived for the column. The cooperative launch guarantees that all workgroups are resident.
 // The wait is bounded: after CP_SPIN_LIMIT polls it gives up and flags info[b] = -1.
-constexpr int CP_T = 256, CP_W = 32;
+constexpr int CP_W = 32;
+// rows (threads) per workgroup: 1024 for complex float; 256 for complex double, whose 32-column
+// row (128 registers) leaves no room for more threads
+template <typename T>
+struct CoopRows {
+    static constexpr int value = 1024;
+};
+template <>
+struct CoopRows<double> {
+    static constexpr int value = 256;
+};
 constexpr long CP_SPIN_LIMIT = 1L << 22;
 static int g_coop_min_rows = -1;   // tall-panel threshold (rows below the diagonal block)
 
@@ -338,9 +349,10 @@ __device__ __forceinline__ void matrix_barrier(unsigned* cnt, unsigned target, i
 }
 
 template <typename T>
-__global__ __launch_bounds__(CP_T) void k_panel_coop(int n, int j0, int w, cx<T>* A, long lda, long sA, int* ipiv,
-                                                     long sP, int* info, T* cv, int* ci, cx<T>* crow, cx<T>* drow,
-                                                     unsigned* cnt) {
+__global__ __launch_bounds__(CoopRows<T>::value) void k_panel_coop(int n, int j0, int w, cx<T>* A, long lda, long sA,
+                                                                   int* ipiv, long sP, int* info, T* cv, int* ci,
+                                                                   cx<T>* crow, cx<T>* drow, unsigned* cnt) {
+    constexpr int CP_T = CoopRows<T>::value;
     __shared__ T red_v[CP_T / 64];
     __shared__ int red_i[CP_T / 64];
     __shared__ T gv[CP_T];
@@ -471,6 +483,7 @@ static bool coop_prepare(int n, int batch, hipStream_t s, CoopWS& ws) {
         g_coop_min_rows = e ? atoi(e) : 4096;
     }
     if (n < g_coop_min_rows) return false;
+    constexpr int CP_T = CoopRows<T>::value;
     const int gmax = (n + CP_T - 1) / CP_T;
     const size_t ncand = 2 * (size_t)batch * gmax;
     const size_t bytes = ncand * CP_W * sizeof(cx<T>) + 2 * (size_t)batch * CP_W * sizeof(cx<T>) +
@@ -506,6 +519,7 @@ static bool coop_prepare(int n, int batch, hipStream_t s, CoopWS& ws) {
 template <typename T>
 static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* ipiv, long sP, int* info, int batch,
                        const CoopWS& ws, hipStream_t s) {
+    constexpr int CP_T = CoopRows<T>::value;
     static int capacity = -1;
     if (capacity < 0) {
         int dev = 0, per_cu = 0, cus = 0;
