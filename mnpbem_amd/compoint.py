@@ -127,6 +127,9 @@ class PointGreen:
         from .hmatrix import C_addr
         p, n = self.p, self.p.nfaces
         enei = float(enei)
+        if self.pt.n == 0:                     # no points kept (all dropped by mindist or medium)
+            npol = 1 if np.ndim(sig.sig if static else sig.sig1) == 1 else np.shape(sig.sig if static else sig.sig1)[1]
+            return np.zeros((0, 14) if npol == 1 else (0, 14, npol), complex)
         if static:
             kmat = torch.zeros(1, dtype=torch.complex128, device=self.dev)
             pmed = torch.ones_like(self.pmed)

@@ -228,3 +228,21 @@ def test_stat_point_fields():
     x, z = np.meshgrid(np.linspace(-20, 20, 9), np.linspace(-20, 20, 7))
     em, hm = M.MeshField(cp, x, 0.0, z, mindist=0.5)(sig)
     assert em.shape == x.shape + (3,) and hm is None
+
+
+def test_points_empty_and_single():
+    """Edge cases: a ComPoint whose points are all dropped (mindist) gives empty fields, and a
+    single point far away works for both Green functions."""
+    cp, green = _setup([M.trisphere(60, 10)], [[2, 1]], [M.EpsConst(1.0), M.EpsDrude(*DRUDE)])
+    lam = 600.0
+    sig, _ = M.BEMRet(cp).solve(M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0]).potential(cp, lam))
+    banner("points empty / single")
+    empty = M.ComPoint(cp, cp.pos[:5] * 1.0001, mindist=1.0)
+    assert empty.n == 0
+    f = M.CompGreenRet(empty, cp).field(sig)
+    assert f.e.shape == (0, 3) and f.h.shape == (0, 3)
+    one = M.ComPoint(cp, [[0.0, 0.0, 50.0]])
+    f1 = M.CompGreenRet(one, cp).field(sig)
+    assert f1.e.shape == (1, 3) and np.all(np.isfinite(f1.e)) and np.abs(f1.e).max() > 0
+    sst, _ = M.BEMStat(cp).solve(M.PlaneWaveStat([1.0, 0, 0])(cp, lam))
+    assert M.CompGreenStat(empty, cp).field(sst).e.shape == (0, 3)
