@@ -5,7 +5,8 @@ The unknown is the packed 8N vector x = [sig1 h1 sig2 h2]; the operator is the e
 the GPU (``mnp_bem_operator``). The Krylov solver is restarted GMRES with right
 preconditioning, classical Gram-Schmidt with one re-orthogonalisation, Givens rotations; the
 basis lives in HBM (``mnp_mdot`` / ``mnp_mupdate``), only the (restart+1) x restart
-Hessenberg matrix is on the host. Preconditioner 'bemret' is the BEMRet LU chain of the
+Hessenberg matrix is on the host. ``solver='bicgstab'`` and ``'cgs'`` run right-preconditioned
+BiCGSTAB and CGS on the same device vectors. Preconditioner 'bemret' is the BEMRet LU chain of the
 same wavelength (as the reference's default, bem_ret_iter.py:801-1086), in complex64
 (``precond_precision='single'``) or complex128; 'none' runs unpreconditioned.
 
@@ -114,8 +115,122 @@ class DeviceGMRES:
         return x, flag, relres, it, hist
 
 
+class DeviceBiCGSTAB:
+    """Right-preconditioned BiCGSTAB (van der Vorst) on device vectors, the reference's
+    solver='bicgstab' (bem_iter.py:64-68, scipy's bicgstab): solves A M y = b, x = M y.
+    Returns (x, flag, relres, iterations, history) as DeviceGMRES; relres is recomputed."""
+
+    def __init__(self, L, dev):
+        self.L, self.dev = L, dev
+
+    def solve(self, apply_A, apply_M, b, x0, tol, maxit):
+        x = x0.clone() if x0 is not None else torch.zeros_like(b)
+        bnorm = float(torch.linalg.vector_norm(b).item())
+        if bnorm == 0:
+            return torch.zeros_like(b), 0, 0.0, 0, [0.0]
+        r = torch.empty_like(b)
+        apply_A(x, r, rhs=b)
+        rh = r.clone()
+        p, v = torch.zeros_like(b), torch.zeros_like(b)
+        ph, sh, t = torch.empty_like(b), torch.empty_like(b), torch.empty_like(b)
+        rho = alpha = omega = 1.0 + 0j
+        hist = [float(torch.linalg.vector_norm(r).item()) / bnorm]
+        flag, it = 1, 0
+        while it < maxit and hist[-1] > tol:
+            rho_new = complex(torch.vdot(rh, r).item())
+            if rho_new == 0:
+                break
+            beta = (rho_new / rho) * (alpha / omega)
+            p.sub_(v, alpha=omega).mul_(beta).add_(r)
+            apply_M(p, ph)
+            apply_A(ph, v, rhs=None)
+            alpha = rho_new / complex(torch.vdot(rh, v).item())
+            s = r - alpha * v
+            it += 1
+            sn = float(torch.linalg.vector_norm(s).item()) / bnorm
+            if sn <= tol:
+                x.add_(ph, alpha=alpha)
+                hist.append(sn)
+                break
+            apply_M(s, sh)
+            apply_A(sh, t, rhs=None)
+            tt = float(torch.linalg.vector_norm(t).item()) ** 2
+            omega = complex(torch.vdot(t, s).item()) / tt if tt > 0 else 0j
+            x.add_(ph, alpha=alpha).add_(sh, alpha=omega)
+            r = s - omega * t
+            hist.append(float(torch.linalg.vector_norm(r).item()) / bnorm)
+            rho = rho_new
+            if omega == 0:
+                break
+        apply_A(x, r, rhs=b)
+        relres = float(torch.linalg.vector_norm(r).item()) / bnorm
+        if relres <= tol:
+            flag = 0
+        return x, flag, relres, it, hist
+
+
+class DeviceCGS:
+    """Right-preconditioned conjugate gradients squared (Sonneveld), the reference's
+    solver='cgs' (bem_iter.py:60-63, scipy's cgs). Returns as DeviceGMRES."""
+
+    def __init__(self, L, dev):
+        self.L, self.dev = L, dev
+
+    def solve(self, apply_A, apply_M, b, x0, tol, maxit):
+        x = x0.clone() if x0 is not None else torch.zeros_like(b)
+        bnorm = float(torch.linalg.vector_norm(b).item())
+        if bnorm == 0:
+            return torch.zeros_like(b), 0, 0.0, 0, [0.0]
+        r = torch.empty_like(b)
+        apply_A(x, r, rhs=b)
+        rh = r.clone()
+        p, q = torch.zeros_like(b), torch.zeros_like(b)
+        ph, vh, uh, au = (torch.empty_like(b) for _ in range(4))
+        rho_prev = 1.0 + 0j
+        hist = [float(torch.linalg.vector_norm(r).item()) / bnorm]
+        flag, it = 1, 0
+        while it < maxit and hist[-1] > tol:
+            rho = complex(torch.vdot(rh, r).item())
+            if rho == 0:
+                break
+            if it == 0:
+                u = r.clone()
+                p.copy_(u)
+            else:
+                beta = rho / rho_prev
+                u = r + beta * q
+                p.mul_(beta).add_(q).mul_(beta).add_(u)
+            apply_M(p, ph)
+            apply_A(ph, vh, rhs=None)
+            alpha = rho / complex(torch.vdot(rh, vh).item())
+            q = u - alpha * vh
+            apply_M(u + q, uh)
+            x.add_(uh, alpha=alpha)
+            apply_A(uh, au, rhs=None)
+            r.sub_(au, alpha=alpha)
+            it += 1
+            hist.append(float(torch.linalg.vector_norm(r).item()) / bnorm)
+            rho_prev = rho
+        apply_A(x, r, rhs=b)
+        relres = float(torch.linalg.vector_norm(r).item()) / bnorm
+        if relres <= tol:
+            flag = 0
+        return x, flag, relres, it, hist
+
+
+def make_solver(name, L, restart, dev):
+    """Device Krylov solver by the reference's name (bem_iter.py:11-17)."""
+    if name == "gmres":
+        return DeviceGMRES(L, restart, dev)
+    if name == "bicgstab":
+        return DeviceBiCGSTAB(L, dev)
+    if name == "cgs":
+        return DeviceCGS(L, dev)
+    raise ValueError("[error] iterative solver not known: <{}>".format(name))
+
+
 class BEMRetIter:
-    """Iterative BEMRet. Options (reference names): ``solver='gmres'``, ``tol=1e-4``,
+    """Iterative BEMRet. Options (reference names): ``solver='gmres'|'bicgstab'|'cgs'``, ``tol=1e-4``,
     ``maxit=200``, ``restart=None`` (= min(n, maxit)), ``precond='hmat'|'bemret'|None``,
     ``preconditioner='auto'|'none'``, ``precond_precision='single'|'double'``,
     ``precond_gemm_split=2|3|0`` (not in the reference), ``hmatrix``,
@@ -135,8 +250,8 @@ class BEMRetIter:
         self.fadmiss = op.get("fadmiss", None)
         self.p = p
         self.solver = op.get("solver", "gmres")
-        if self.solver != "gmres":
-            raise NotImplementedError("only solver='gmres' is provided on the device")
+        if self.solver not in ("gmres", "bicgstab", "cgs"):
+            raise ValueError("[error] iterative solver not known: <{}>".format(self.solver))
         self.tol = float(op.get("tol", 1e-4))
         self.maxit = int(op.get("maxit", 200))
         self.restart = op.get("restart", None)
@@ -259,7 +374,7 @@ class BEMRetIter:
         L = 8 * n
         restart = int(self.restart) if self.restart is not None else min(L, self.maxit)
         restart = max(1, min(restart, 128))
-        gm = DeviceGMRES(L, restart, self.g.geo.dev)
+        gm = make_solver(self.solver, L, restart, self.g.geo.dev)
         out = np.zeros((npol, 8, n), complex)
         infos = []
         for ip in range(npol):

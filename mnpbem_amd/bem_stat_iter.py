@@ -2,15 +2,15 @@
 H-matrix quasistatic Green function (greenfun/aca_compgreen_stat.py:11-327).
 
 The system is that of BEMStat, -(F + diag Lambda) sigma = phi', Lambda = 2 pi (eps1 + eps2) /
-(eps1 - eps2) per face (bem_stat_iter.py:855-868), solved by the device GMRES of
-``bem_iter.DeviceGMRES`` (restarted, right-preconditioned, CGS2). F is wavelength independent:
+(eps1 - eps2) per face (bem_stat_iter.py:855-868), solved by the device Krylov
+solvers of ``bem_iter`` (restarted GMRES with CGS2, BiCGSTAB or CGS, right-preconditioned). F is wavelength independent:
 dense on the device (``CompGreenStat``), or, with ``hmatrix=True``, an H-matrix whose entries,
 ACA and products run on the device (``hmatrix.HMatrixRet`` with the k = 0 kernels). The
 preconditioner is the LU of the dense -(F + diag Lambda) of the solve's wavelength
 (precond 'hmat' / 'full', bem_stat_iter.py:528-600; also built for ``hmatrix=True`` unless
 ``preconditioner='none'``, :973-999), in complex128 (default) or complex64.
 
-Differences from the reference: only GMRES; ``solve(exc, x0=...)`` accepts an initial guess;
+Differences from the reference: ``solve(exc, x0=...)`` accepts an initial guess;
 ``info()`` reports the true relative residual recomputed after the last restart.
 """
 import numpy as np
@@ -18,7 +18,7 @@ import torch
 
 from . import engine
 from ._lib import call
-from .bem_iter import DeviceGMRES, _now
+from .bem_iter import _now, make_solver
 from .greenfun import CompGreenStat, CompStruct
 
 
@@ -131,7 +131,7 @@ class ACACompGreenStat:
 
 
 class BEMStatIter:
-    """Iterative BEMStat. Options (reference names): ``solver='gmres'``, ``tol=1e-4``,
+    """Iterative BEMStat. Options (reference names): ``solver='gmres'|'bicgstab'|'cgs'``, ``tol=1e-4``,
     ``maxit=200``, ``restart=None``, ``precond='hmat'|'full'|None``,
     ``preconditioner='auto'|'none'``, ``hmatrix``, ``htol=1e-6``, ``kmax=[4, 100]``,
     ``cleaf=200``, ``eta=2.5``, ``fadmiss``; not in the reference: ``precond_precision``
@@ -149,8 +149,8 @@ class BEMStatIter:
         self.eta = float(op.get("eta", 2.5))
         self.fadmiss = op.get("fadmiss", None)
         self.solver = op.get("solver", "gmres")
-        if self.solver != "gmres":
-            raise NotImplementedError("only solver='gmres' is provided on the device")
+        if self.solver not in ("gmres", "bicgstab", "cgs"):
+            raise ValueError("[error] iterative solver not known: <{}>".format(self.solver))
         self.tol = float(op.get("tol", 1e-4))
         self.maxit = int(op.get("maxit", 200))
         self.restart = op.get("restart", None)
@@ -261,7 +261,7 @@ class BEMStatIter:
         cols = phip.reshape(n, -1)
         restart = int(self.restart) if self.restart is not None else min(n, self.maxit)
         restart = max(1, min(restart, 128))
-        gm = DeviceGMRES(n, restart, geo.dev) if self.maxit > 0 else None
+        gm = make_solver(self.solver, n, restart, geo.dev) if self.maxit > 0 else None
         out = np.zeros_like(cols)
         x0c = None if x0 is None else np.asarray(x0, complex).reshape(n, -1)
         t0 = _now(geo.dev)

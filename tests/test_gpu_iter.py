@@ -73,3 +73,25 @@ def test_warm_start_from_solution(sphere):
     sig, _ = it.solve(exc(cp, lam), x0=direct)
     flag, relres, niter = it.info()
     assert flag == 0 and niter == 0
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "cgs"])
+def test_bicgstab_and_cgs_equal_direct(sphere, solver):
+    """solver='bicgstab' / 'cgs' (bem_iter.py:60-68): with the BEMRet preconditioner they reach
+    the direct solution (reference test_bem_iter.py:75-85 asks 1e-6); unpreconditioned BiCGSTAB
+    converges on the same system."""
+    cp, exc = sphere
+    lam = 600.0
+    banner("bemretiter solver={}".format(solver))
+    direct, _ = M.BEMRet(cp).solve(exc(cp, lam))
+    it = BEMRetIter(cp, solver=solver, tol=1e-10, maxit=100)
+    sig, _ = it.solve(exc(cp, lam))
+    flag, relres, niter = it.info()
+    assert flag == 0 and relres <= 1e-10 and niter <= 20
+    for f in ("sig1", "h1", "sig2", "h2"):
+        assert _rel(getattr(sig, f), getattr(direct, f)) < 1e-7
+    if solver == "bicgstab":
+        un = BEMRetIter(cp, solver=solver, precond=None, tol=1e-8, maxit=2000)
+        s2, _ = un.solve(exc(cp, lam))
+        assert un.info()[0] == 0
+        assert _rel(s2.sig2, direct.sig2) < 1e-5

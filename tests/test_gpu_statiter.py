@@ -94,3 +94,19 @@ def test_hmatrix_F_and_solution():
     pot = it.potential(si, 1)
     ref = Gd @ si.sig
     assert np.abs(pot.phi1 - ref).max() / np.abs(ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("solver", ["bicgstab", "cgs"])
+def test_bicgstab_and_cgs_match_direct(solver):
+    """Reference test_bem_iter.py:75-85: BEMStatIter(solver='bicgstab', tol=1e-8) matches
+    BEMStat to 1e-6; the same for 'cgs', with and without the preconditioner."""
+    cp = _sphere()
+    exc = M.PlaneWaveStat(np.array([[0.0, 0.0, 1.0]]))(cp, 550.0)
+    sd, _ = M.BEMStat(cp).solve(exc)
+    for pre in ("hmat", None):
+        it = M.BEMStatIter(cp, solver=solver, tol=1e-8, maxit=300, precond=pre)
+        si, _ = it.solve(exc)
+        assert it.info()[0][-1] == 0
+        assert np.abs(sd.sig - si.sig).max() / np.abs(sd.sig).max() < 1e-6
+    with pytest.raises(ValueError):
+        M.BEMStatIter(cp, solver="qmr")
