@@ -90,6 +90,16 @@ class BEMStat:
 
     __truediv__ = solve
 
+    def potential(self, sig, inout=2):
+        """phi = G sig, phip = H sig outside (inout=2, default) or inside (1) of the surface
+        (bem_stat.py:936-960, via CompGreenStat.potential)."""
+        return self.g.potential(sig, inout)
+
+    def field(self, sig, inout=2):
+        """e = -Hp sig outside (inout=2, default) or inside (1) of the surface (bem_stat.py:
+        863-935, deriv='cart', via CompGreenStat.field)."""
+        return self.g.field(sig, inout)
+
     def clear(self):
         self._fac = None
         self.enei = None

@@ -310,6 +310,10 @@ class BEMStatIter:
         t = "1" if inout == 1 else "2"
         return CompStruct(self.p, sig.enei, **{"phi" + t: self.g.G @ s, "phi" + t + "p": H @ s})
 
+    def field(self, sig, inout=2):
+        """e = -Hp sig on the surface (bem_stat_iter.py:1019-1072, via CompGreenStat.field)."""
+        return self._gs.field(sig, inout)
+
     def clear(self):
         self._pre = None
         self.enei = None

@@ -314,10 +314,38 @@ class CompGreenStat:
         return CompStruct(self.p1, sig.enei, **{"phi" + t: self.G @ s, "phi" + t + "p": H @ s})
 
     def field(self, sig, inout=1):
-        """e = -Hp sig (compgreen_stat.py:974-1009); provided for p1 = ComPoint (fields at
-        points, where Hp = Gp)."""
-        if self._points is None:
-            raise NotImplementedError("CompGreenStat.field on the device covers p1 = ComPoint")
-        o = self._points.eval(sig, sig.enei, static=True)
-        return CompStruct(self.p1, sig.enei, e=o[:, 8:11])
+        """e = -Hp sig (compgreen_stat.py:974-1009): at points (p1 = ComPoint, Hp = Gp), or on the
+        faces inside (inout=1, H1p = Gp + 2 pi n on the self terms) or outside (inout=2, H2p =
+        Gp - 2 pi n), from the k = 0 entries on the device (``mnp_surface_eval`` without
+        currents)."""
+        if self._points is not None:
+            o = self._points.eval(sig, sig.enei, static=True)
+            return CompStruct(self.p1, sig.enei, e=o[:, 8:11])
+        if np.any(self.diag_shift != 0):
+            raise NotImplementedError("surface fields need the closed-surface correction of the particle "
+                                      "(give ComParticle its closed arguments)")
+        from ._lib import call, GreenGeomC
+        from .engine import stream_handle
+        from .hmatrix import C_addr
+        if inout not in (1, 2):
+            raise ValueError("inout must be 1 (inside) or 2 (outside)")
+        geo = self.green.geo
+        n = geo.n
+        s = np.asarray(sig.sig, complex)
+        single = s.ndim == 1
+        s2 = s.reshape(n, -1)
+        kmat = torch.zeros(geo.nmat, dtype=torch.complex128, device=geo.dev)
+        ones = torch.ones_like(geo.con_in)
+        rowptr, rcol, rg, rf = geo.refined_csr()
+        g = GreenGeomC(n, geo.npart, geo.order, *[t.data_ptr() for t in (
+            geo.pos, geo.nvec, geo.area, geo.fpart, ones, ones, kmat, rowptr, rcol, rg, rf)])
+        sd = torch.tensor(np.ascontiguousarray(s2.T), dtype=torch.complex128, device=geo.dev)
+        hz = torch.zeros((n, 3), dtype=torch.complex128, device=geo.dev)
+        out = torch.empty((s2.shape[1], n, 14), dtype=torch.complex128, device=geo.dev)
+        for q in range(s2.shape[1]):
+            call("mnp_surface_eval", C_addr(g), inout - 1, 2 * np.pi / float(sig.enei), sd[q].data_ptr(),
+                 hz.data_ptr(), out[q].data_ptr(), stream_handle())
+        e = np.transpose(out.cpu().numpy()[:, :, 8:11], (1, 2, 0))
+        del kmat, ones
+        return CompStruct(self.p1, sig.enei, e=e[:, :, 0] if single else e)
 

@@ -246,3 +246,28 @@ def test_points_empty_and_single():
     assert f1.e.shape == (1, 3) and np.all(np.isfinite(f1.e)) and np.abs(f1.e).max() > 0
     sst, _ = M.BEMStat(cp).solve(M.PlaneWaveStat([1.0, 0, 0])(cp, lam))
     assert M.CompGreenStat(empty, cp).field(sst).e.shape == (0, 3)
+
+
+def test_bemstat_surface_field():
+    """BEMStat.field (bem_stat.py:863-935, e = -H1p/H2p sig) against the oracle at k = 0, and
+    normal D continuity across the surface with the incident field on both sides (the
+    quasistatic BEM equation -(Lambda + F) sig = phi' is eps1 n.(e1 + e_inc) = eps2 n.(e2 +
+    e_inc) with the external potential present everywhere; exact)."""
+    cp, green = _setup([M.trisphere(144, 20)], [[2, 1]], [M.EpsConst(1.0), M.EpsTable("gold.dat")])
+    lam = 520.0
+    bem = M.BEMStat(cp)
+    pol = np.array([1.0, 0, 0])
+    sig, _ = bem.solve(M.PlaneWaveStat(pol)(cp, lam))
+    banner("bemstat surface field")
+    z = np.zeros((cp.nfaces, 3))
+    for inout in (1, 2):
+        e = bem.field(sig, inout).e
+        ref = O.surface_potential_field(green, [0.0, 0.0], lam, sig.sig, z, sig.sig, z, inout)
+        assert np.max(np.abs(e - ref["e"])) / np.max(np.abs(ref["e"])) < 1e-10
+    e1, e2 = bem.field(sig, 1).e, bem.field(sig, 2).e
+    n = cp.nvec
+    d1 = cp.eps1(lam) * np.sum(n * (e1 + pol[None, :]), axis=1)
+    d2 = cp.eps2(lam) * np.sum(n * (e2 + pol[None, :]), axis=1)
+    assert np.linalg.norm(d1 - d2) < 1e-8 * np.linalg.norm(d1)
+    it = M.BEMStatIter(cp)
+    assert np.allclose(it.field(sig, 2).e, e2, rtol=0, atol=1e-12 * np.abs(e2).max())
