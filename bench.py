@@ -47,39 +47,80 @@ def build_workload():
     return M, cp, exc
 
 
-def cpu_baseline(cp, nsample):
-    """The oracle (FP64 numpy/LAPACK restatement of the reference path) on the host cores:
-    per-wavelength assembly + LU chain + solve + extinction, timed over `nsample`
-    wavelengths after the wavelength-independent set-up."""
+def _oracle_sweep(cp, lams, pol, direc, eps_out):
+    """The oracle's per-wavelength loop (assembly + LU chain + solve + extinction) after its
+    wavelength-independent set-up; returns seconds."""
     from oracle import bem_oracle as O
     p = cp.pc
     s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
-    fp = np.zeros(s.n, int)
-    green = O.Green(s, fp, [[2, 1]], [(np.arange(s.n), np.arange(s.n), 1.0)])
-    lams = np.linspace(500, 1000, 201)[:: max(1, 201 // max(nsample, 1))][:nsample]
-    pol, dr = np.array([0.0, 0, 1.0]), np.array([1.0, 0, 0])
+    green = O.Green(s, np.zeros(s.n, int), [[2, 1]], [(np.arange(s.n), np.arange(s.n), 1.0)])
+    pol, direc = np.asarray(pol, float), np.asarray(direc, float)
     t0 = time.perf_counter()
     for lam in lams:
-        eps_k = [O.eps_const(lam, 1.77), O.eps_drude(lam, *DRUDE)]
+        eps_k = [O.eps_const(lam, eps_out), O.eps_drude(lam, *DRUDE)]
         ks = [e[1] for e in eps_k]
         G1, G2, H1, H2 = O.matrices(green, ks)
         e1 = np.full(s.n, eps_k[1][0])
         e2 = np.full(s.n, eps_k[0][0])
         k0 = 2 * np.pi / lam
         bem = O.BEMRetOracle(G1, G2, H1, H2, s.nvec, e1, e2, k0)
-        a2, a2p = O.planewave(s.pos, s.nvec, pol, dr, lam, 1.77, np.arange(s.n))
+        a2, a2p = O.planewave(s.pos, s.nvec, pol, direc, lam, eps_out, np.arange(s.n))
         z = np.zeros_like(a2)
         sol = bem.solve(*O.excitation(s.nvec, k0, e1, e2, z, z, a2, a2p))
-        O.extinction(pol, dr, lam, ks[0], 1.77, s.pos, s.area, sol, np.zeros(0, int), np.arange(s.n))
-    dt = time.perf_counter() - t0
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([int(i.get("num_threads", 1)) for i in threadpool_info()] or [1])
-    except Exception:
-        threads = len(os.sched_getaffinity(0))
-    return dict(value=len(lams) / dt, unit="solves/s", cores=threads, kind="port",
-                sample="{} of the 201 wavelengths at n={} (assembly+LU chain+solve+extinction per wavelength, "
-                       "{:.1f} s; geometry set-up excluded)".format(len(lams), s.n, dt))
+        O.extinction(pol, direc, lam, ks[0], eps_out, s.pos, s.area, sol, np.zeros(0, int), np.arange(s.n))
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(cp, nsample, nsample1=1):
+    """The oracle (FP64 numpy/LAPACK restatement of the reference path) on the host cores:
+    per-wavelength assembly + LU chain + solve + extinction, timed over `nsample` wavelengths
+    with every BLAS thread and over `nsample1` wavelengths on one core, after the
+    wavelength-independent set-up."""
+    from threadpoolctl import threadpool_info, threadpool_limits
+    lams = np.linspace(500, 1000, 201)[:: max(1, 201 // max(nsample, 1))][:nsample]
+    affinity = len(os.sched_getaffinity(0))
+    blas = max([int(i.get("num_threads", 1)) for i in threadpool_info()] or [1])
+    dt = _oracle_sweep(cp, lams, [0, 0, 1.0], [1.0, 0, 0], 1.77)
+    with threadpool_limits(limits=1):
+        dt1 = _oracle_sweep(cp, lams[:nsample1], [0, 0, 1.0], [1.0, 0, 0], 1.77)
+    return dict(value=round(len(lams) / dt, 4), unit="solves/s", cores=blas, kind="port",
+                host_cpus=affinity, blas_threads=blas,
+                single_core_value=round(nsample1 / dt1, 4),
+                sample="{} of the 201 wavelengths at n={} with {} BLAS threads on {} host CPUs ({:.1f} s), and {} "
+                       "on one core ({:.1f} s): assembly + LU chain + solve + extinction per wavelength, geometry "
+                       "set-up excluded".format(len(lams), cp.nfaces, blas, affinity, dt, nsample1, dt1))
+
+
+def config1(skip_cpu=False):
+    """The reference's own CPU-runnable case: trisphere(144, 20) gold-Drude sphere in vacuum,
+    x-polarised plane wave along z, 41 wavelengths 400-800 nm. GPU: the complex128 sweep (one
+    batch of 41) timed after a warm-up; CPU: the oracle's loop over all 41 wavelengths with every
+    BLAS thread (the scope of validation/03_bemret/sphere/run_python.py:37-45)."""
+    import torch
+    import mnpbem_amd as M
+    cp = M.ComParticle([M.EpsConst(1.0), M.EpsDrude(*DRUDE)], [M.trisphere(144, 20)], [[2, 1]], 1, interp="curv")
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    lams = np.linspace(400, 800, 41)
+    out = {"workload": "BEMRet gold-Drude sphere trisphere(144, 20), n={}, vacuum, 41 wavelengths 400-800 nm"
+                       .format(cp.nfaces)}
+    for prec in ("double", "mixed"):
+        bem = M.BEMRet(cp, batch=41, precision=prec)
+        bem.solve_spectrum(exc, lams)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = bem.solve_spectrum(exc, lams)
+        torch.cuda.synchronize()
+        out["gpu_{}_s".format(prec)] = round(time.perf_counter() - t0, 4)
+        out["gpu_{}_solves_per_s".format(prec)] = round(41 / (time.perf_counter() - t0), 2)
+        if prec == "double":
+            ext_d = r["ext"][:, 0]
+    if not skip_cpu:
+        dt = _oracle_sweep(cp, lams, [1.0, 0, 0], [0, 0, 1.0], 1.0)
+        out["cpu_s"] = round(dt, 3)
+        out["cpu_solves_per_s"] = round(41 / dt, 3)
+        out["cpu_kind"] = "port"
+    out["ext_max"] = float(np.max(ext_d))
+    return out
 
 
 def gmres25k():
@@ -123,7 +164,8 @@ def _dimer25k():
     import mnpbem_amd as M
     p1 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, -41.0])
     p2 = M.trirod(20, 80, [60, 14, 80], triangles=True).shift([0, 0, 41.0])
-    return M.ComParticle([M.EpsConst(1.0), M.EpsDrude(10, 9.065, 0.0708)], [p1, p2], [[2, 1], [2, 1]], 1, 2)
+    return M.ComParticle([M.EpsConst(1.0), M.EpsDrude(10, 9.065, 0.0708)], [p1, p2], [[2, 1], [2, 1]], 1, 2,
+                         interp="curv")
 
 
 def warmstart25k(nlam=11, seed=7):
@@ -149,7 +191,7 @@ def warmstart25k(nlam=11, seed=7):
         else:
             a = float(rng.uniform(5, 10))
             part = M.triellipsoid(int(rng.choice([400, 576, 784, 1024])), [a, a, a * float(rng.uniform(1, 4))])
-        cps.append(M.ComParticle([M.EpsConst(1.0), drude], [part], [[2, 1]], 1))
+        cps.append(M.ComParticle([M.EpsConst(1.0), drude], [part], [[2, 1]], 1, interp="curv"))
     t0 = time.perf_counter()
     X, T = W.training_set(cps, rng.uniform(450, 950, 6), pol, direc)
     t_data = time.perf_counter() - t0
@@ -206,28 +248,50 @@ def warmstart25k(nlam=11, seed=7):
                                                   max(res["previous+learned"]["time_s"], 1e-9), 3))
 
 
+def sphere_mesh(nfaces, diameter):
+    """Curved-triangle sphere whose face count is the even number closest to ``nfaces``: a
+    latitude-longitude grid of m longitudes and t latitude bands (trispheresegment over the full
+    sphere, poles merged) has exactly 2 m t faces; m / t is kept between 1.5 and 2.5."""
+    import mnpbem_amd as M
+    best = None
+    for t in range(4, 80):
+        for m in (int(np.floor(nfaces / (2 * t))), int(np.ceil(nfaces / (2 * t)))):
+            if m < 6 or not 1.5 <= m / t <= 2.5:
+                continue
+            key = (abs(2 * m * t - nfaces), abs(m / t - 2.0))
+            if best is None or key < best[0]:
+                best = (key, m, t)
+    _, m, t = best
+    return M.trispheresegment(np.linspace(0, 2 * np.pi, m + 1), np.linspace(0, np.pi, t + 2), diameter,
+                              triangles=True)
+
+
 def particles64(seed=20261203):
-    """64 seeded particles of uneven size: spheres, prolate ellipsoids (aspect 1-4) and sphere
-    gap dimers (gap 1-5 nm), target face counts uniform in 300-3000 (nearest tabulated sphere
-    point set; a dimer splits the count over its two spheres), gold Drude in eps 1.33."""
+    """64 seeded particles of uneven size, as (kind, ComParticle): spheres (20 nm), prolate
+    ellipsoids (semi-axes 8, 8, 8 x aspect U[1, 4]) and gap dimers of two 20 nm spheres (gap
+    U[1, 5] nm), face counts drawn uniform in 300-3000 and realised by the mesh to within a few
+    faces (a dimer splits its count over its two spheres), gold Drude in eps 1.33."""
     import mnpbem_amd as M
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(64):
-        kind = int(rng.integers(3))
+        kind = ("sphere", "ellipsoid", "dimer")[int(rng.integers(3))]
         nf = int(rng.integers(300, 3001))
         drude = M.EpsDrude(10, 9.065, 0.0708)
-        if kind == 0:
-            cp = M.ComParticle([M.EpsConst(1.33), drude], [M.trisphere(nf // 2 + 2, 20)], [[2, 1]], 1)
-        elif kind == 1:
+        if kind == "sphere":
+            cp = M.ComParticle([M.EpsConst(1.33), drude], [sphere_mesh(nf, 20.0)], [[2, 1]], 1,
+                               interp="curv")
+        elif kind == "ellipsoid":
             asp = float(rng.uniform(1, 4))
-            cp = M.ComParticle([M.EpsConst(1.33), drude], [M.triellipsoid(nf // 2 + 2, [8, 8, 8 * asp])], [[2, 1]], 1)
+            cp = M.ComParticle([M.EpsConst(1.33), drude], [sphere_mesh(nf, 2.0).scale([8, 8, 8 * asp])],
+                               [[2, 1]], 1, interp="curv")
         else:
             gap = float(rng.uniform(1, 5))
-            s1 = M.trisphere(nf // 4 + 2, 20).shift([0, 0, -(10 + gap / 2)])
-            s2 = M.trisphere(nf // 4 + 2, 20).shift([0, 0, 10 + gap / 2])
-            cp = M.ComParticle([M.EpsConst(1.33), drude], [s1, s2], [[2, 1], [2, 1]], 1, 2)
-        out.append(cp)
+            s1 = sphere_mesh(nf // 2, 20.0).shift([0, 0, -(10 + gap / 2)])
+            s2 = sphere_mesh(nf // 2, 20.0).shift([0, 0, 10 + gap / 2])
+            cp = M.ComParticle([M.EpsConst(1.33), drude], [s1, s2], [[2, 1], [2, 1]], 1, 2,
+                               interp="curv")
+        out.append((kind, cp))
     return out
 
 
@@ -239,7 +303,7 @@ def batched64():
     import torch
     import mnpbem_amd as M
     t0 = time.perf_counter()
-    parts = particles64()
+    parts = [cp for _, cp in particles64()]
     greens = [M.CompGreenRet(cp, cp) for cp in parts]
     for g in greens:
         g.geo
@@ -295,6 +359,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-gmres25k", action="store_true")
     ap.add_argument("--no-batched64", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-warmstart", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -351,11 +416,12 @@ def main():
     _lib.call("mnp_prof_collect", prof.ctypes.data)
     _lib.call("mnp_prof_enable", 0)
     gemm_ms, gemm_flops, gemm_launches, gemm_bytes = prof
-    # algorithmic (complex, FP32-equivalent) flops of the GEMM launches over their kernel time
-    effective = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
-    # the split-BF16 kernel executes split * (split + 1) / 2 BF16 products per real product
+    # algorithmic work of the GEMM launches (8 m n k real flops per complex product, the
+    # complex FP32-equivalent count) over their kernel time, against the peak of the matrix
+    # cores the kernel runs on (BF16 for the split operands, FP32 otherwise)
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    # the split-BF16 kernel executes terms (terms + 1) / 2 BF16 products per real product
     products = {0: 1, 2: 3, 3: 6}[args.gemm_split]
-    achieved = effective * products
     peak = PEAK_BF16_TFLOPS if args.gemm_split else PEAK_FP32_TFLOPS
 
     dt_mixed, out_mixed = results["mixed"]
@@ -415,20 +481,27 @@ def main():
                    "faces": n, "wavelengths": len(lams), "batch": args.batch,
                    "lu_tuning_mixed": solvers_tuning.get("mixed"), "lu_tuning_fp64": solvers_tuning.get("double")},
         "roofline": {"bound": "mfma",
-                     "kernel": ("k_gemm_sub_c64p<{},1,1> (complex GEMM on BF16 matrix cores, executed BF16 "
-                                "flops)".format(args.gemm_split) if args.gemm_split else
-                                "k_gemm_sub_c64v (complex-float GEMM)"),
+                     "kernel": ("k_gemm_sub_c64p<{}> (complex GEMM on the BF16 matrix cores, {} operand terms)"
+                                .format(args.gemm_split, args.gemm_split) if args.gemm_split else
+                                "k_gemm_sub_c64v (complex-float GEMM, FP32 matrix cores)"),
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "effective_complex_tflops": round(effective, 3),
-                     "effective_frac_of_fp32_peak": round(effective / PEAK_FP32_TFLOPS, 4),
+                     "frac": round(achieved / peak, 4),
+                     "work": "algorithmic 8 m n k real flops per launch (complex FP32-equivalent)",
+                     "executed_bf16_tflops": round(achieved * products, 3) if args.gemm_split else None,
+                     "executed_frac": round(achieved * products / peak, 4) if args.gemm_split else None,
+                     "frac_of_fp32_peak": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": GEMM_TRAFFIC_BYTES, "traffic_source": GEMM_TRAFFIC_SOURCE,
                      "algorithmic_bytes_per_launch": round(gemm_bytes / max(gemm_launches, 1)),
+                     "algorithmic_flops_per_launch": round(gemm_flops / max(gemm_launches, 1)),
+                     "kernel_us_per_launch": round(1e3 * gemm_ms / max(gemm_launches, 1), 2),
                      "gemm_ms_per_sweep": round(gemm_ms, 3), "gemm_launches_per_sweep": int(gemm_launches)},
         "cpu_baseline": cpu,
     }
     line.update(extra)
     solvers.clear()
     torch.cuda.empty_cache()
+    if not args.no_config1:
+        line["config1"] = config1(skip_cpu=args.no_cpu)
     if not args.no_batched64:
         line["batched64"] = batched64()
     if not args.no_gmres25k:

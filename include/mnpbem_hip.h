@@ -16,6 +16,10 @@
  *     1000 + hipError_t when a launch failed.
  *   - Pivots are 0-based row indices. LU info: 0 ok, j > 0 = zero pivot in column j-1
  *     (LAPACK zgetrf convention; the reference does not detect it, utils/gpu.py:189-191).
+ *   - State: tuning knobs, grown workspaces (split-BF16 operand planes, cooperative-panel
+ *     scratch), the lookahead side stream and GEMM timing records belong to a handle. Every call
+ *     works through the handle bound to the calling host thread (mnp_handle_bind); a thread that
+ *     binds none has its own default handle. Threads that run at once must not share a handle.
  */
 #ifndef MNPBEM_HIP_H
 #define MNPBEM_HIP_H
@@ -24,8 +28,25 @@
 extern "C" {
 #endif
 
-/* ABI revision of this header (1). */
+/* ABI revision of this header (2). */
 int mnp_abi_version(void);
+
+/* Library handle (opaque). The reference keeps its solver state in Python objects and runs one
+ * solve at a time per process (bem/bem_ret.py:248-259); a handle gives each host thread its own
+ * knobs, workspaces and side stream, as a per-thread handle replaces the reference's
+ * process-global cupy/cuSolver context (utils/gpu.py:145-301). */
+typedef struct mnp_handle_s* mnp_handle;
+/* New handle with the default knobs (environment overrides MNP_LU_NBO, MNP_GEMM_SPLIT, ...). */
+int mnp_handle_create(mnp_handle* h);
+/* Waits for the handle's work, frees its workspaces, stream and events, and destroys it; a
+ * thread bound to it falls back to its default handle. */
+int mnp_handle_destroy(mnp_handle h);
+/* Bind h to the calling thread (NULL: the thread's default handle). */
+int mnp_handle_bind(mnp_handle h);
+/* Waits for the device and frees the workspaces, side streams and events of every handle (they
+ * are made again on demand). Meant for process exit: the Python binding registers it with
+ * atexit so that nothing of the library is alive when the HIP runtime and tools shut down. */
+int mnp_release(void);
 
 /* Test hook: one wave runs v_mfma_f64_16x16x4f64, v_mfma_f32_16x16x4f32 and
  * v_mfma_f32_32x32x2f32 on known operands and writes their result registers to out[0..255],
@@ -289,16 +310,15 @@ int mnp_probe_mfma_bf16(float* out, void* stream);
 
 
 /* ---------------------------------------------------------------------------------------------
- * LU tuning knobs (process wide, host values; no counterpart in the reference). key 0: outer
- * block width of the blocked LU and right-side solves (16..1024 columns, default 128); 1:
- * lookahead (trailing update split over two streams) on/off; 2: register-resident panel for
- * complex float on/off; 3: panel height from which the multi-workgroup panel is used (>= 256);
- * 4: complex-float GEMM on the FP32 matrix cores (0, default) or on the BF16 matrix cores with
- * operands split into 2 or 3 bfloat16 terms, per tile (2, 3) or once into planes (12, 13; 2
- * terms: about 16 significant bits, 3: about 24, as FP32); 5: workgroup tile of the plane GEMM
- * (11: 64 x 64, 21: 128 x 64, 12: 64 x 128, 22: 128 x 128); 6: row tiles per group of the
- * plane GEMM's XCD-aware tile order (0: plain grid order).
- * mnpbem_amd.autotune picks them per problem shape by timing. */
+ * LU tuning knobs of the calling thread's handle (host values; no counterpart in the
+ * reference). key 0: outer block width of the blocked LU and right-side solves (16..1024
+ * columns, default 128); 1: lookahead (trailing update split over two streams) on/off; 2:
+ * register-resident panel for complex float on/off; 3: panel height from which the
+ * multi-workgroup panel is used (>= 256); 4: complex-float GEMM on the FP32 matrix cores (0,
+ * default) or on the BF16 matrix cores with operands split once into planes of 2 or 3 bfloat16
+ * terms (12, 13; 2 terms keep about 16 significant bits, 3 about 24, as FP32); 5: workgroup
+ * tile of the plane GEMM (11: 64 x 64, the only one); 6: row tiles per group of the plane
+ * GEMM's XCD-aware tile order (0: plain grid order). mnpbem_amd.autotune picks them per problem shape by timing. */
 int mnp_set_tuning(int key, int value);
 int mnp_get_tuning(int key, int* value);
 

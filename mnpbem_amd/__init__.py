@@ -13,9 +13,10 @@ from .greenfun import CompGreenRet, CompGreenStat, CompStruct
 from .bem import BEMRet
 from .bem_stat import BEMStat, PlaneWaveStat
 from .bem_stat_iter import ACACompGreenStat, BEMStatIter
+from .bem_iter import BEMRetIter
 from .planewave import PlaneWaveRet, SpectrumRet
 
 __all__ = ["EpsConst", "EpsDrude", "EpsTable", "Particle", "trisphere", "trirod", "trispheresegment",
-           "triellipsoid", "fvgrid", "ComParticle", "ComPoint", "MeshField", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMStat",
+           "triellipsoid", "fvgrid", "ComParticle", "ComPoint", "MeshField", "CompGreenRet", "CompGreenStat", "CompStruct", "BEMRet", "BEMRetIter", "BEMStat",
            "BEMStatIter", "ACACompGreenStat", "PlaneWaveStat", "PlaneWaveRet",
            "SpectrumRet"]

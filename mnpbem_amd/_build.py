@@ -17,7 +17,7 @@ ARCH = "gfx950"
 # Packed FP32 instructions are disabled for all device code (no v_pk_* f32 forms).
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-result"]
+         "-Wno-unused-result", "-Wno-unused-value"]
 
 
 def _compile(src):

@@ -3,6 +3,7 @@
 The library is mandatory: importing this module raises when it is missing, and every call
 raises ``RuntimeError`` on a non-zero status. There is no host fallback.
 """
+import atexit
 import ctypes as C
 import os
 
@@ -16,6 +17,10 @@ L = C.c_long
 # name -> argument types (all return int status)
 _SIGS = {
     "mnp_abi_version": [],
+    "mnp_handle_create": [P],
+    "mnp_handle_destroy": [P],
+    "mnp_handle_bind": [P],
+    "mnp_release": [],
     "mnp_probe_mfma_layout": [P, P],
     "mnp_prof_enable": [I],
     "mnp_prof_collect": [P],
@@ -89,6 +94,9 @@ def _load():
 
 
 lib = _load()
+# nothing of the library (workspaces, side streams, events) stays alive into the HIP runtime's
+# and profilers' own shutdown
+atexit.register(lib.mnp_release)
 
 
 def call(name, *args):
@@ -99,6 +107,35 @@ def call(name, *args):
             raise ValueError("{}: argument {} rejected".format(name, -st))
         raise RuntimeError("{}: HIP launch failure (status {})".format(name, st))
     return st
+
+
+class Handle:
+    """A library handle (knobs, workspaces, lookahead stream) for the calling thread:
+    ``with Handle() as h:`` binds it for the block and restores the thread's default after;
+    ``close()`` frees it."""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        call("mnp_handle_create", C.byref(self._h))
+
+    def bind(self):
+        call("mnp_handle_bind", self._h)
+        return self
+
+    @staticmethod
+    def unbind():
+        call("mnp_handle_bind", None)
+
+    def __enter__(self):
+        return self.bind()
+
+    def __exit__(self, *exc):
+        self.unbind()
+
+    def close(self):
+        if self._h:
+            call("mnp_handle_destroy", self._h)
+            self._h = C.c_void_p()
 
 
 def exported_symbols():

@@ -2,7 +2,7 @@
 
 The batched LU (lu.hip) has three shape parameters that trade panel latency against GEMM
 efficiency: the panel width ``nb`` (16 or 32 columns), the outer block width (the K of the big
-trailing GEMM: 64, 128 or 256) and lookahead (trailing update split over two streams). Which
+trailing GEMM: 64 to 512) and lookahead (trailing update split over two streams). Which
 combination is fastest depends on n, the batch and the precision. ``tune_lu`` times every
 candidate once on random matrices of the problem's shape (one untimed call first) and keeps the
 fastest for the process; ``knobs`` applies a choice around a factorisation and restores the
@@ -53,8 +53,13 @@ def knobs(cfg):
 
 
 def candidates(n=0):
-    return [dict(nb=nb, nbo=nbo, lookahead=la) for nb in (16, 32) for nbo in (64, 128, 256) for la in (0, 1)
-            if nbo >= nb]
+    """(nb, nbo, lookahead) shapes to time. Wider outer blocks halve the passes of the trailing
+    update over the matrix and deepen its K (split-BF16 GEMM at 64 x 2240^2: K 128 -> 132,
+    K 256 -> 158 TFLOP/s complex-equivalent) at the price of more in-block work; for large n the
+    16-column panel never wins."""
+    nbs = (32,) if n >= 1024 else (16, 32)
+    nbos = (64, 128, 256, 384, 512) if n >= 1024 else (64, 128, 256)
+    return [dict(nb=nb, nbo=nbo, lookahead=la) for nb in nbs for nbo in nbos for la in (0, 1) if nbo >= nb]
 
 
 def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):

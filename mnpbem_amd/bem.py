@@ -170,6 +170,9 @@ class BEMRet:
         enei_list = np.atleast_1d(np.asarray(enei_list, float))
         batch = int(batch or self.batch)
         precision = precision or self.precision
+        # the sweep refactors the solver object that init() may have factored: the factors no
+        # longer belong to self.enei, so the next solve() factors again
+        self.enei = None
         s = self._get_solver(batch, precision)
         geo = s.geo
         npol = exc.pol.shape[0]

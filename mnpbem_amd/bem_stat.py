@@ -50,6 +50,11 @@ class BEMStat:
         ipiv = torch.empty((B, n), dtype=torch.int32, device=geo.dev)
         info = torch.empty(B, dtype=torch.int32, device=geo.dev)
         call("mnp_getrf_batched", 0, n, B, M.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), self.nb, sh)
+        inf = info.cpu().numpy()
+        if np.any(inf < 0):
+            raise RuntimeError("BEMStat: LU panel synchronisation timed out (info {}); factors are invalid".format(inf))
+        if np.any(inf > 0):
+            engine.warn_singular("BEMStat", inf)
         pf = torch.empty((B, n), dtype=torch.int32, device=geo.dev)
         q = torch.empty_like(pf)
         call("mnp_perm_batched", n, B, ipiv.data_ptr(), n, pf.data_ptr(), q.data_ptr(), n, sh)

@@ -10,6 +10,7 @@
 #include "common.hpp"
 #include "args.hpp"
 #include "hmat.hpp"
+#include "ctx.hpp"
 #include "../../include/mnpbem_hip.h"
 
 namespace mnp {
@@ -81,7 +82,31 @@ static int launch_status() {
 
 extern "C" {
 
-int mnp_abi_version(void) { return 1; }
+int mnp_abi_version(void) { return 2; }
+
+// Handles: knobs, workspaces and the lookahead stream of the library, per host thread.
+int mnp_handle_create(mnp_handle* h) {
+    REQUIRE(h, 1);
+    *h = (mnp_handle)ctx_create();
+    return 0;
+}
+
+int mnp_handle_destroy(mnp_handle h) {
+    REQUIRE(h && ctx_valid((Ctx*)h), 1);
+    const bool touched = ((Ctx*)h)->release();
+    ctx_destroy((Ctx*)h);
+    return touched ? launch_status() : 0;
+}
+
+int mnp_handle_bind(mnp_handle h) {
+    REQUIRE(!h || ctx_valid((Ctx*)h), 1);
+    ctx_bind((Ctx*)h);
+    return 0;
+}
+
+int mnp_release(void) {
+    return ctx_release_all() ? launch_status() : 0;
+}
 
 int mnp_probe_mfma_layout(double* out, void* stream) {
     REQUIRE(out, 1);
@@ -89,8 +114,8 @@ int mnp_probe_mfma_layout(double* out, void* stream) {
     return launch_status();
 }
 
-// GEMM timing for roofline reports: while enabled, every matrix-core GEMM launch is
-// bracketed by HIP events on its stream. Not thread safe; meant for benchmarks.
+// GEMM timing for roofline reports: while enabled, every matrix-core GEMM launch of the calling
+// thread's handle is bracketed by HIP events on its stream; meant for benchmarks.
 int mnp_prof_enable(int on) {
     prof_enable(on != 0);
     return 0;

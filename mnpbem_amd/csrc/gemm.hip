@@ -12,6 +12,7 @@
 // column (a contiguous segment in column-major C). Lane layouts are in Mfma<T> below
 // (measured on gfx950 by k_probe_layout).
 #include "common.hpp"
+#include "ctx.hpp"
 
 #include <stdint.h>
 #include <vector>
@@ -19,38 +20,19 @@
 namespace mnp {
 
 // Optional per-launch timing of the GEMM (HIP events on the launch stream), for roofline
-// reporting. Off by default; see mnp_prof_enable / mnp_prof_collect in capi.hip.
-struct ProfRec {
-    hipEvent_t a, b;
-    double flops, bytes;
-    int plane;                              // 1: the split-BF16 plane kernel
-};
-static bool g_prof_on = false;
-static std::vector<ProfRec> g_prof;
-static ProfRec g_prof_pending;              // record being filled by the plane GEMM path
-static std::vector<hipEvent_t> g_pool;
-
-static hipEvent_t pool_get() {
-    if (!g_pool.empty()) {
-        hipEvent_t e = g_pool.back();
-        g_pool.pop_back();
-        return e;
-    }
-    hipEvent_t e;
-    hipEventCreate(&e);
-    return e;
-}
-
-void prof_enable(bool on) { g_prof_on = on; }
+// reporting. Off by default; see mnp_prof_enable / mnp_prof_collect in capi.hip. Records live in
+// the calling thread's handle.
+void prof_enable(bool on) { ctx().prof_on = on; }
 
 // Sum of GEMM kernel time (ms), algorithmic flops and bytes since the last call, over the
 // launches of the split-BF16 plane kernel when there were any, else over all; waits for the
 // events.
 void prof_collect(double* ms, double* flops, double* launches, double* bytes) {
+    Ctx& c = ctx();
     double t = 0, f = 0, by = 0, nl = 0;
     bool any_plane = false;
-    for (auto& r : g_prof) any_plane |= r.plane != 0;
-    for (auto& r : g_prof) {
+    for (auto& r : c.prof) any_plane |= r.plane != 0;
+    for (auto& r : c.prof) {
         if (!any_plane || r.plane) {
             hipEventSynchronize(r.b);
             float e = 0;
@@ -60,15 +42,15 @@ void prof_collect(double* ms, double* flops, double* launches, double* bytes) {
             by += r.bytes;
             nl += 1;
         }
-        g_pool.push_back(r.a);
-        g_pool.push_back(r.b);
+        c.pool.push_back(r.a);
+        c.pool.push_back(r.b);
     }
     hipDeviceSynchronize();
     *ms = t;
     *flops = f;
     *launches = nl;
     *bytes = by;
-    g_prof.clear();
+    c.prof.clear();
 }
 
 // Matrix-core tile of one wave: 32 x 32 of C.
@@ -283,180 +265,28 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64v(int m, int n, int k, cons
 // ---------------------------------------------------------------------------------------------
 // Complex-float GEMM on the BF16 matrix cores with split operands. Every float x is written as
 // a sum of NS bfloat16 terms (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)), and a
-// real product keeps the terms of order 0 and 1 (NS = 2: x0y0 + x0y1 + x1y0, about 16
-// significant bits) or up to order 2 (NS = 3: six products, about 24 bits, the float's own
-// precision), accumulated in FP32 by v_mfma_f32_32x32x16_bf16. The v_mfma_f32_32x32x16_bf16
-// rate is 16x the FP32 MFMA rate, so even the six-product form leaves 2.7x headroom.
+// real product keeps the term pairs of order < NS (NS = 2: x0y0 + x0y1 + x1y0, about 16
+// significant bits; NS = 3: six term products, about 24 bits, the float's own precision),
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16.
 //
-// Workgroup of 4 waves owns a 128x128 tile of C, each wave 64x64 (2x2 MFMA tiles of 32x32,
-// real and imaginary accumulators). K slices of 16: each thread loads 8 complex values of A
-// (one row, 8 consecutive k) or B (one column, 8 consecutive k), splits them and writes one
-// 16-byte row segment per term and component into LDS rows of 16 k (+8 padding: 48-byte row
-// stride, so 16 lanes of a 16-byte read hit distinct banks). A is stored negated (C -= A B);
-// the imaginary term needs +Ai once, which is one sign flip per fragment in registers.
-// Operand fragments: lane l reads 8 consecutive k at row l % 32, k offset 8 (l / 32); the MFMA
-// is fed (B^T, A^T) as the FP32 kernels, so results store as contiguous column segments.
-// Used for even leading dimensions and 16-byte aligned A, B (the other cases keep the FP32
-// kernels).
+// The complex product takes four real products per term pair, C += (-A) B with (re re, -im im)
+// and (im re, re im). (The three-product Gauss form, P1 = ar br, P2 = ai bi, P3 = (ar + ai)
+// (br + bi), needs a third operand plane per term; measured on this kernel it ran 7-15 %
+// slower than four products, because the kernel is bound by moving operands through LDS, not
+// by the matrix cores.)
+//
+// Plane layout (per batch entry): component c (0 re, 1 im) of term q is plane qc = 2 q + c.
+// Element (qc, row r, k) (r: row of A or column of B) lies at
+//   (((qc KS + k / 16) 2 + (k / 8) % 2) R + r) 8 + k % 8,   KS = ceil(K / 16), R = rows padded
+// to 128: one k slice of a plane is two halves of 8 k, and in a half consecutive rows are 16
+// bytes apart. A 16-byte chunk (8 k of one row) is one MFMA operand fragment of one lane: lane
+// l of a wave takes row l % 32 and half l / 32, so a wave's fragment is two contiguous 512-byte
+// runs. Padding rows and k are zero. A is stored negated (C -= A B).
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f16acc __attribute__((ext_vector_type(16)));
-constexpr int ST = 128, SK = 16, SKP = 24;    // tile, K slice, LDS row length (bf16)
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+constexpr int ST = 128, SK = 16, SKP = 24;    // row padding, K slice, LDS row length (4M kernel, bf16)
 
-template <int NS>
-__global__ __launch_bounds__(256) void k_gemm_sub_c64s(int m, int n, int k, const cx<float>* __restrict__ A, long lda,
-                                                       long sA, const cx<float>* __restrict__ B, long ldb, long sB,
-                                                       cx<float>* __restrict__ C, long ldc, long sC) {
-    // [term][re/im][row][k]
-    __shared__ __attribute__((aligned(16))) __bf16 As[NS][2][ST][SKP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][2][ST][SKP];
-    const int bz = blockIdx.z;
-    A += (long)bz * sA;
-    B += (long)bz * sB;
-    C += (long)bz * sC;
-    const int row0 = blockIdx.x * ST, col0 = blockIdx.y * ST;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-    f16acc cre[2][2], cim[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
-    // loads: A row ai, k = 8 akg .. +7; B column bj, k = 8 bkg .. +7
-    const int ai = t & 127, akg = t >> 7;
-    const int bj = t >> 1, bkg = t & 1;
-    float2 va[8];
-    float4 vb[4];
-    auto load = [&](int k0) {
-        const int gi = row0 + ai;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int gk = k0 + 8 * akg + e;
-            va[e] = make_float2(0.f, 0.f);
-            if (gi < m && gk < k) {
-                const cx<float> v = A[gi + (long)gk * lda];
-                va[e] = make_float2(v.re, v.im);
-            }
-        }
-        const int gj = col0 + bj, gk = k0 + 8 * bkg;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) vb[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (gj < n) {
-            const cx<float>* p = B + gk + (long)gj * ldb;
-            if (gk + 8 <= k) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) vb[e] = ((const float4*)p)[e];
-            } else {
-                for (int e = 0; e < 8; ++e)
-                    if (gk + e < k) {
-                        const float2 w = make_float2(p[e].re, p[e].im);
-                        if (e & 1) { vb[e >> 1].z = w.x; vb[e >> 1].w = w.y; }
-                        else { vb[e >> 1].x = w.x; vb[e >> 1].y = w.y; }
-                    }
-            }
-        }
-    };
-    // split 8 values into NS bf16 terms and store each term as one 16-byte row segment
-    auto split_store = [&](const float (&x)[8], __bf16* dst0, long term_stride) {
-        float r[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) r[e] = x[e];
-#pragma unroll
-        for (int q = 0; q < NS; ++q) {
-            bf8v h;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                h[e] = (__bf16)r[e];
-                r[e] -= (float)h[e];
-            }
-            *(bf8v*)(dst0 + q * term_stride) = h;
-        }
-    };
-    const long tstride = 2L * ST * SKP;      // elements between terms
-    load(0);
-    for (int k0 = 0; k0 < k; k0 += SK) {
-        {
-            float xr[8], xi[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) { xr[e] = -va[e].x; xi[e] = -va[e].y; }
-            split_store(xr, &As[0][0][ai][8 * akg], tstride);
-            split_store(xi, &As[0][1][ai][8 * akg], tstride);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                xr[2 * e] = vb[e].x; xi[2 * e] = vb[e].y;
-                xr[2 * e + 1] = vb[e].z; xi[2 * e + 1] = vb[e].w;
-            }
-            split_store(xr, &Bs[0][0][bj][8 * bkg], tstride);
-            split_store(xi, &Bs[0][1][bj][8 * bkg], tstride);
-        }
-        __syncthreads();
-        if (k0 + SK < k) load(k0 + SK);
-        const int fr = lane & 31, fk = 8 * (lane >> 5);
-        bf8v pr[2][NS], pi[2][NS];
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                pr[tj][q] = *(const bf8v*)&Bs[q][0][wn + 32 * tj + fr][fk];
-                pi[tj][q] = *(const bf8v*)&Bs[q][1][wn + 32 * tj + fr][fk];
-            }
-#pragma unroll
-        for (int ti = 0; ti < 2; ++ti) {
-            bf8v qr[NS], qi[NS], qp[NS];      // -Ar, -Ai, +Ai terms
-#pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                qr[q] = *(const bf8v*)&As[q][0][wm + 32 * ti + fr][fk];
-                qi[q] = *(const bf8v*)&As[q][1][wm + 32 * ti + fr][fk];
-                typedef unsigned u4v __attribute__((ext_vector_type(4)));
-                u4v w = __builtin_bit_cast(u4v, qi[q]);
-                w ^= 0x80008000u;
-                qp[q] = __builtin_bit_cast(bf8v, w);
-            }
-#pragma unroll
-            for (int tj = 0; tj < 2; ++tj) {
-                // term pairs (p, q) with p + q < NS: order 0 .. NS - 1
-#pragma unroll
-                for (int p = 0; p < NS; ++p)
-#pragma unroll
-                    for (int q = 0; q < NS - p; ++q) {
-                        cre[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[tj][p], qr[q], cre[ti][tj], 0, 0, 0);
-                        cre[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[tj][p], qp[q], cre[ti][tj], 0, 0, 0);
-                        cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[tj][p], qr[q], cim[ti][tj], 0, 0, 0);
-                        cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[tj][p], qi[q], cim[ti][tj], 0, 0, 0);
-                    }
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int ti = 0; ti < 2; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < 2; ++tj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gi = row0 + wm + 32 * ti + (lane & 31);
-                const int gj = col0 + wn + 32 * tj + Mfma<float>::drow(lane, r);
-                if (gi < m && gj < n) {
-                    cx<float>* c = C + gi + (long)gj * ldc;
-                    cx<float> v = *c;
-                    v.re += cre[ti][tj][r];
-                    v.im += cim[ti][tj][r];
-                    *c = v;
-                }
-            }
-}
-
-// ---------------------------------------------------------------------------------------------
-// Pre-split operands. The split of the kernel above runs once per loaded tile (every column
-// tile of C re-splits the same rows of A); here A and B are split once into bfloat16 planes,
-// and the GEMM only copies 16-byte chunks. Plane layout (per batch entry): element (qc, r, k)
-// of plane qc = 2 term + component (0 re, 1 im), row r (A: row of A, B: column of B), k at
-// (((qc * KS + k / 16) * 2 + (k / 8) % 2) * R + r) * 8 + k % 8, KS = ceil(K / 16), R = rows
-// padded to 128: one k slice of 128 rows is 4 KB contiguous, as two halves of 8 k in which
-// consecutive rows are 16 bytes apart (consecutive lanes read consecutive rows coalesced and
-// store them to LDS rows 48 bytes apart, conflict-free). Padding rows and k are zero. A is
-// stored negated (C -= A B).
 // One thread per plane row and k slice of 16: reads 16 complex values (TRANS: 128 contiguous
 // bytes of a column of B; else one element per k, coalesced across the threads' rows of A) and
 // writes 16 bytes per half slice, term and component (consecutive threads: consecutive 16 bytes).
@@ -470,34 +300,34 @@ __global__ __launch_bounds__(256) void k_split_planes(int rows, int K, const cx<
     X += (long)blockIdx.z * sX;
     P += (long)blockIdx.z * sP;
     if (r >= R) return;
-    float re[16], im[16];
+    float v[2][16];
     const int k0 = 16 * s;
     if (TRANS && vec && r < rows && k0 + 16 <= K) {
         const float4* p = (const float4*)(X + k0 + (long)r * ldx);
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float4 v = p[e];
-            re[2 * e] = sgn * v.x;
-            im[2 * e] = sgn * v.y;
-            re[2 * e + 1] = sgn * v.z;
-            im[2 * e + 1] = sgn * v.w;
+            const float4 w = p[e];
+            v[0][2 * e] = sgn * w.x;
+            v[1][2 * e] = sgn * w.y;
+            v[0][2 * e + 1] = sgn * w.z;
+            v[1][2 * e + 1] = sgn * w.w;
         }
     } else {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
             const int k = k0 + e;
-            re[e] = 0.f;
-            im[e] = 0.f;
+            v[0][e] = 0.f;
+            v[1][e] = 0.f;
             if (r < rows && k < K) {
-                const cx<float> v = TRANS ? X[k + (long)r * ldx] : X[r + (long)k * ldx];
-                re[e] = sgn * v.re;
-                im[e] = sgn * v.im;
+                const cx<float> w = TRANS ? X[k + (long)r * ldx] : X[r + (long)k * ldx];
+                v[0][e] = sgn * w.re;
+                v[1][e] = sgn * w.im;
             }
         }
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-        float* x = c ? im : re;
+        float* x = v[c];
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
             bf8v h0, h1;
@@ -537,22 +367,20 @@ __device__ inline void xcd_tile(int gm, int& bx, int& by, int& bz) {
     by = (int)(w / rows);
 }
 
-// Workgroup of WR x WC waves (64 WR WC lanes), each wave MI x NJ MFMA tiles of 32 x 32: a
-// (32 MI WR) x (32 NJ WC) tile of C.
-template <int NS, int MI, int NJ, int WR = 2, int WC = 2, int NB = 1>
-__global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, int K, const __bf16* __restrict__ PA,
-                                                                int RA, long sPA, const __bf16* __restrict__ PB,
-                                                                int RB, long sPB, cx<float>* __restrict__ C, long ldc,
-                                                                long sC, int gm) {
-    constexpr int NT = 64 * WR * WC;
-    constexpr int TM = 32 * MI * WR, TN = 32 * NJ * WC;
+// Workgroup of 4 waves (2 x 2), a 64 x 64 tile of C, each wave one 32 x 32 MFMA tile
+// with real and imaginary accumulators. Each k slice is staged through LDS in rows of 16 k (+8
+// padding: 48-byte rows, conflict-free 16-byte reads); the next slice is loaded into registers
+// while the current one is multiplied. +Ai (for the -im im product) is one sign flip per
+// fragment in registers.
+template <int NS>
+__global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
+                                                       long sPA, const __bf16* __restrict__ PB, int RB, long sPB,
+                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
+    constexpr int TM = 64, TN = 64;
     constexpr int CA = 4 * NS * TM, CB = 4 * NS * TN;          // 16-byte chunks per k slice
-    constexpr int UA = (CA + NT - 1) / NT, UB = (CB + NT - 1) / NT;
-    // NB = 2: two LDS buffers, the next k slice is stored while the current one is read (one
-    // barrier per slice)
-    __shared__ __attribute__((aligned(16))) __bf16 As[NB][NS][2][TM][SKP];
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[NB][NS][2][TN][SKP];
-    typedef unsigned u4v __attribute__((ext_vector_type(4)));
+    constexpr int UA = CA / 256, UB = CB / 256;
+    __shared__ __attribute__((aligned(16))) __bf16 As[NS][2][TM][SKP];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NS][2][TN][SKP];
     int tx, ty, bz;
     xcd_tile(gm, tx, ty, bz);
     PA += (long)bz * sPA;
@@ -561,221 +389,82 @@ __global__ __launch_bounds__(64 * WR * WC) void k_gemm_sub_c64p(int m, int n, in
     const int KS = (K + 15) / 16;
     const int row0 = tx * TM, col0 = ty * TN;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave % WR) * 32 * MI, wn = (wave / WR) * 32 * NJ;
-    f16acc cre[MI][NJ], cim[MI][NJ];
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    f16acc cre, cim;
 #pragma unroll
-    for (int a = 0; a < MI; ++a)
-#pragma unroll
-        for (int b = 0; b < NJ; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
-    // chunk e = t + NT u of the A (B) tile: plane e / (2 TM), half (e % (2 TM)) / TM, row e % TM
+    for (int r = 0; r < 16; ++r) { cre[r] = 0; cim[r] = 0; }
+    // chunk e = t + 256 u of the A (B) tile: plane e / (2 TM), half (e % (2 TM)) / TM, row e % TM
     u4v ra[UA], rb[UB];
     auto load = [&](int s) {
 #pragma unroll
         for (int u = 0; u < UA; ++u) {
-            const int e = t + NT * u, qc = e / (2 * TM), w = e % (2 * TM);
-            if (CA % NT == 0 || e < CA)
-                ra[u] = *(const u4v*)(PA + ((((long)qc * KS + s) * 2 + w / TM) * RA + row0 + w % TM) * 8);
+            const int e = t + 256 * u, qc = e / (2 * TM), w = e % (2 * TM);
+            ra[u] = *(const u4v*)(PA + ((((long)qc * KS + s) * 2 + w / TM) * RA + row0 + w % TM) * 8);
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int e = t + NT * u, qc = e / (2 * TN), w = e % (2 * TN);
-            if (CB % NT == 0 || e < CB)
-                rb[u] = *(const u4v*)(PB + ((((long)qc * KS + s) * 2 + w / TN) * RB + col0 + w % TN) * 8);
+            const int e = t + 256 * u, qc = e / (2 * TN), w = e % (2 * TN);
+            rb[u] = *(const u4v*)(PB + ((((long)qc * KS + s) * 2 + w / TN) * RB + col0 + w % TN) * 8);
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&]() {
 #pragma unroll
         for (int u = 0; u < UA; ++u) {
-            const int e = t + NT * u, qc = e / (2 * TM), w = e % (2 * TM);
-            if (CA % NT == 0 || e < CA) *(u4v*)&As[buf][qc >> 1][qc & 1][w % TM][8 * (w / TM)] = ra[u];
+            const int e = t + 256 * u, qc = e / (2 * TM), w = e % (2 * TM);
+            *(u4v*)&As[qc >> 1][qc & 1][w % TM][8 * (w / TM)] = ra[u];
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            const int e = t + NT * u, qc = e / (2 * TN), w = e % (2 * TN);
-            if (CB % NT == 0 || e < CB) *(u4v*)&Bs[buf][qc >> 1][qc & 1][w % TN][8 * (w / TN)] = rb[u];
+            const int e = t + 256 * u, qc = e / (2 * TN), w = e % (2 * TN);
+            *(u4v*)&Bs[qc >> 1][qc & 1][w % TN][8 * (w / TN)] = rb[u];
         }
-    };
-    auto compute = [&](int buf) {
-        const int fr = lane & 31, fk = 8 * (lane >> 5);
-        bf8v pr[NJ][NS], pi[NJ][NS], qr[MI][NS], qi[MI][NS], qp[MI][NS];
-#pragma unroll
-        for (int tj = 0; tj < NJ; ++tj)
-#pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                pr[tj][q] = *(const bf8v*)&Bs[buf][q][0][wn + 32 * tj + fr][fk];
-                pi[tj][q] = *(const bf8v*)&Bs[buf][q][1][wn + 32 * tj + fr][fk];
-            }
-#pragma unroll
-        for (int ti = 0; ti < MI; ++ti)
-#pragma unroll
-            for (int q = 0; q < NS; ++q) {
-                qr[ti][q] = *(const bf8v*)&As[buf][q][0][wm + 32 * ti + fr][fk];
-                qi[ti][q] = *(const bf8v*)&As[buf][q][1][wm + 32 * ti + fr][fk];
-                u4v w = __builtin_bit_cast(u4v, qi[ti][q]);
-                w ^= 0x80008000u;
-                qp[ti][q] = __builtin_bit_cast(bf8v, w);
-            }
-        // term pairs (p, q), p + q < NS; the four real products of each pair go to 2 MI NJ
-        // independent accumulators in turn
-#pragma unroll
-        for (int p = 0; p < NS; ++p)
-#pragma unroll
-            for (int q = 0; q < NS - p; ++q) {
-#pragma unroll
-                for (int ti = 0; ti < MI; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < NJ; ++tj) {
-                        cre[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[tj][p], qr[ti][q], cre[ti][tj], 0, 0, 0);
-                        cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[tj][p], qr[ti][q], cim[ti][tj], 0, 0, 0);
-                    }
-#pragma unroll
-                for (int ti = 0; ti < MI; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < NJ; ++tj) {
-                        cre[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[tj][p], qp[ti][q], cre[ti][tj], 0, 0, 0);
-                        cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[tj][p], qi[ti][q], cim[ti][tj], 0, 0, 0);
-                    }
-            }
     };
     load(0);
-    if (NB == 1) {
-        for (int s = 0; s < KS; ++s) {
-            store(0);
-            __syncthreads();
-            if (s + 1 < KS) load(s + 1);
-            compute(0);
-            __syncthreads();
-        }
-    } else {
-        store(0);
+    for (int s = 0; s < KS; ++s) {
+        store();
         __syncthreads();
-        for (int s = 0; s < KS; ++s) {
-            if (s + 1 < KS) load(s + 1);
-            compute(s & 1);
-            if (s + 1 < KS) store((s + 1) & 1);
-            __syncthreads();
+        if (s + 1 < KS) load(s + 1);
+        const int fr = lane & 31, fk = 8 * (lane >> 5);
+        bf8v pr[NS], pi[NS], qr[NS], qi[NS], qp[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            pr[q] = *(const bf8v*)&Bs[q][0][wn + fr][fk];
+            pi[q] = *(const bf8v*)&Bs[q][1][wn + fr][fk];
+            qr[q] = *(const bf8v*)&As[q][0][wm + fr][fk];
+            qi[q] = *(const bf8v*)&As[q][1][wm + fr][fk];
+            u4v w = __builtin_bit_cast(u4v, qi[q]);
+            w ^= 0x80008000u;
+            qp[q] = __builtin_bit_cast(bf8v, w);
         }
-    }
-#pragma unroll
-    for (int ti = 0; ti < MI; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < NJ; ++tj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gi = row0 + wm + 32 * ti + (lane & 31);
-                const int gj = col0 + wn + 32 * tj + Mfma<float>::drow(lane, r);
-                if (gi < m && gj < n) {
-                    cx<float>* c = C + gi + (long)gj * ldc;
-                    cx<float> v = *c;
-                    v.re += cre[ti][tj][r];
-                    v.im += cim[ti][tj][r];
-                    *c = v;
-                }
-            }
-}
-
-// Variant without LDS: in the plane layout lane l's 16 bytes of an MFMA operand (row l % 32,
-// k 8 (l / 32) .. +7) lie in two 512-byte runs (one per half slice), so each wave loads its
-// operands straight from memory with one coalesced 16-byte load per operand fragment (the two waves sharing a row or column tile meet in the L1),
-// and the next k slice is loaded while the current one is multiplied. No barriers.
-template <int NS, int MI, int NJ>
-__global__ __launch_bounds__(256) void k_gemm_sub_c64d(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
-                                                       long sPA, const __bf16* __restrict__ PB, int RB, long sPB,
-                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
-    typedef unsigned u4v __attribute__((ext_vector_type(4)));
-    int tx, ty, bz;
-    xcd_tile(gm, tx, ty, bz);
-    PA += (long)bz * sPA;
-    PB += (long)bz * sPB;
-    C += (long)bz * sC;
-    const int KS = (K + 15) / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = tx * 64 * MI + (wave & 1) * 32 * MI, wn = ty * 64 * NJ + (wave >> 1) * 32 * NJ;
-    const int lr = lane & 31, lh = lane >> 5;
-    f16acc cre[MI][NJ], cim[MI][NJ];
-#pragma unroll
-    for (int a = 0; a < MI; ++a)
-#pragma unroll
-        for (int b = 0; b < NJ; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
-    // fragments of one k slice: A[ti][qc], B[tj][qc]
-    u4v fa[2][MI][2 * NS], fb[2][NJ][2 * NS];
-    auto load = [&](int s, int buf) {
-#pragma unroll
-        for (int qc = 0; qc < 2 * NS; ++qc) {
-#pragma unroll
-            for (int ti = 0; ti < MI; ++ti)
-                fa[buf][ti][qc] = *(const u4v*)(PA + ((((long)qc * KS + s) * 2 + lh) * RA + wm + 32 * ti + lr) * 8);
-#pragma unroll
-            for (int tj = 0; tj < NJ; ++tj)
-                fb[buf][tj][qc] = *(const u4v*)(PB + ((((long)qc * KS + s) * 2 + lh) * RB + wn + 32 * tj + lr) * 8);
-        }
-    };
-    auto step = [&](int buf) {
 #pragma unroll
         for (int p = 0; p < NS; ++p)
 #pragma unroll
             for (int q = 0; q < NS - p; ++q) {
-#pragma unroll
-                for (int ti = 0; ti < MI; ++ti)
-#pragma unroll
-                    for (int tj = 0; tj < NJ; ++tj) {
-                        const bf8v pr = __builtin_bit_cast(bf8v, fb[buf][tj][2 * p]);
-                        const bf8v pi = __builtin_bit_cast(bf8v, fb[buf][tj][2 * p + 1]);
-                        const bf8v qr = __builtin_bit_cast(bf8v, fa[buf][ti][2 * q]);
-                        const bf8v qi = __builtin_bit_cast(bf8v, fa[buf][ti][2 * q + 1]);
-                        const bf8v qp = __builtin_bit_cast(bf8v, fa[buf][ti][2 * q + 1] ^ 0x80008000u);
-                        cre[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr, qr, cre[ti][tj], 0, 0, 0);
-                        cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi, qr, cim[ti][tj], 0, 0, 0);
-                        cre[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi, qp, cre[ti][tj], 0, 0, 0);
-                        cim[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr, qi, cim[ti][tj], 0, 0, 0);
-                    }
+                cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[p], qr[q], cre, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[p], qr[q], cim, 0, 0, 0);
+                cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[p], qp[q], cre, 0, 0, 0);
+                cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[p], qi[q], cim, 0, 0, 0);
             }
-    };
-    load(0, 0);
-    int s = 0;
-    for (; s + 2 <= KS; s += 2) {
-        load(s + 1, 1);
-        step(0);
-        if (s + 2 < KS) load(s + 2, 0);
-        step(1);
+        __syncthreads();
     }
-    if (s < KS) step(0);
 #pragma unroll
-    for (int ti = 0; ti < MI; ++ti)
-#pragma unroll
-        for (int tj = 0; tj < NJ; ++tj)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gi = wm + 32 * ti + (lane & 31);
-                const int gj = wn + 32 * tj + Mfma<float>::drow(lane, r);
-                if (gi < m && gj < n) {
-                    cx<float>* c = C + gi + (long)gj * ldc;
-                    cx<float> v = *c;
-                    v.re += cre[ti][tj][r];
-                    v.im += cim[ti][tj][r];
-                    *c = v;
-                }
-            }
+    for (int r = 0; r < 16; ++r) {
+        const int gi = row0 + wm + (lane & 31);
+        const int gj = col0 + wn + Mfma<float>::drow(lane, r);
+        if (gi < m && gj < n) {
+            cx<float>* c = C + gi + (long)gj * ldc;
+            cx<float> v = *c;
+            v.re += cre[r];
+            v.im += cim[r];
+            *c = v;
+        }
+    }
 }
 
-// Workgroup tile of the plane GEMM: 11 (64 x 64), 21 (128 x 64), 12 (64 x 128), 22 (128 x 128).
-static int g_plane_tile = getenv("MNP_GEMM_TILE") ? atoi(getenv("MNP_GEMM_TILE")) : 11;
-// Row tiles per group of the XCD-aware order (0: plain grid order).
-static int g_plane_group = getenv("MNP_GEMM_GROUP") ? atoi(getenv("MNP_GEMM_GROUP")) : 8;
-
-// Plane workspace per stream (the LU runs GEMMs on two streams at once), grown on demand.
-struct PlaneWs {
-    hipStream_t s;
-    void* p;
-    size_t bytes;
-};
-static std::vector<PlaneWs> g_plane_ws;
+// Plane workspace of the calling thread's handle for stream s, grown on demand.
 static void* plane_ws(hipStream_t s, size_t bytes) {
-    for (auto& w : g_plane_ws)
+    Ctx& c = ctx();
+    for (auto& w : c.plane_ws)
         if (w.s == s) {
             if (w.bytes < bytes) {
                 hipStreamSynchronize(s);
@@ -792,128 +481,81 @@ static void* plane_ws(hipStream_t s, size_t bytes) {
         }
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    g_plane_ws.push_back(PlaneWs{s, p, bytes});
+    c.plane_ws.push_back(PlaneWs{s, p, bytes});
     return p;
 }
 
+// Bytes of the plane workspace of one launch.
+static size_t plane_bytes(int NS, int m, int n, int k, int batch) {
+    const long KS = (k + 15) / 16;
+    const long RA = (m + ST - 1) / ST * ST, RB = (n + ST - 1) / ST * ST;
+    return (size_t)2 * NS * KS * 16 * (RA + RB) * batch * sizeof(__bf16);
+}
+
+// Split A and B into planes at P0 (workspace of plane_bytes; null: the handle's own) and run
+// the product. False when the workspace cannot be allocated.
 template <int NS>
 static bool gemm_planes(int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb, long sB,
-                        void* C, long ldc, long sC, int batch, hipStream_t s) {
+                        void* C, long ldc, long sC, int batch, hipStream_t s, void* P0, ProfRec* rec) {
     const int KS = (k + 15) / 16;
     const int RA = (m + ST - 1) / ST * ST, RB = (n + ST - 1) / ST * ST;
     const long sPA = 2L * NS * KS * RA * 16, sPB = 2L * NS * KS * RB * 16;
-    __bf16* P = (__bf16*)plane_ws(s, (size_t)(sPA + sPB) * batch * sizeof(__bf16));
+    __bf16* P = (__bf16*)(P0 ? P0 : plane_ws(s, plane_bytes(NS, m, n, k, batch)));
     if (!P) return false;
     __bf16* PA = P;
     __bf16* PB = P + sPA * batch;
-    ProfRec* rec = g_prof_on ? &g_prof_pending : nullptr;
     const int vb = ((ldb | sB) & 1) == 0 && ((uintptr_t)B & 15) == 0;   // 16-byte loads along a column of B
     hipLaunchKernelGGL((k_split_planes<NS, false>), dim3(RA / 256 + (RA % 256 != 0), KS, batch), dim3(256), 0, s, m,
                        k, (const cx<float>*)A, lda, sA, PA, RA, sPA, -1.0f, 0);
     hipLaunchKernelGGL((k_split_planes<NS, true>), dim3(RB / 256 + (RB % 256 != 0), KS, batch), dim3(256), 0, s, n,
                        k, (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, vb);
     if (rec) hipEventRecord(rec->a, s);    // the timed region is the GEMM kernel alone
-    switch (g_plane_tile) {
-        case 11:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s,
-                               m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-            break;
-        case 21:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 2, 1>), dim3((m + 127) / 128, (n + 63) / 64, batch), dim3(256), 0,
-                               s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-            break;
-        case 12:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 2>), dim3((m + 63) / 64, (n + 127) / 128, batch), dim3(256), 0,
-                               s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-            break;
-        case 3011:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 2, 2, 2>), dim3((m + 63) / 64, (n + 63) / 64, batch),
-                               dim3(256), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 3142:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 2, 2>), dim3((m + 127) / 128, (n + 63) / 64, batch),
-                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 1142:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 2>), dim3((m + 127) / 128, (n + 63) / 64, batch),
-                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 1124:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 2, 4>), dim3((m + 63) / 64, (n + 127) / 128, batch),
-                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 1144:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 1, 4, 4>), dim3((m + 127) / 128, (n + 127) / 128, batch),
-                               dim3(1024), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 1242:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 1, 2, 4, 2>), dim3((m + 127) / 128, (n + 127) / 128, batch),
-                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 2124:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 2, 1, 2, 4>), dim3((m + 127) / 128, (n + 127) / 128, batch),
-                               dim3(512), 0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                               g_plane_group);
-            break;
-        case 111:
-            hipLaunchKernelGGL((k_gemm_sub_c64d<NS, 1, 1>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s,
-                               m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-            break;
-        case 121:
-            hipLaunchKernelGGL((k_gemm_sub_c64d<NS, 2, 1>), dim3((m + 127) / 128, (n + 63) / 64, batch), dim3(256), 0,
-                               s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-            break;
-        case 112:
-            hipLaunchKernelGGL((k_gemm_sub_c64d<NS, 1, 2>), dim3((m + 63) / 64, (n + 127) / 128, batch), dim3(256), 0,
-                               s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-            break;
-        default:
-            hipLaunchKernelGGL((k_gemm_sub_c64p<NS, 2, 2>), dim3((m + 127) / 128, (n + 127) / 128, batch), dim3(256),
-                               0, s, m, n, k, PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, g_plane_group);
-    }
+    hipLaunchKernelGGL((k_gemm_sub_c64p<NS>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n, k, PA,
+                       RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
     return true;
 }
 
-// 0: FP32 matrix cores; 2 or 3: BF16 matrix cores with that many operand terms, split per
-// tile; 12 or 13: the same with operands split once into planes.
-static int g_gemm_split = getenv("MNP_GEMM_SPLIT") ? atoi(getenv("MNP_GEMM_SPLIT")) : 0;
+// 0: FP32 matrix cores; 12 or 13: BF16 matrix cores, operands split once into planes of 2 or 3
+// terms.
 int set_gemm_split(int v) {
-    if (v != 0 && v != 2 && v != 3 && v != 12 && v != 13) return -2;
-    g_gemm_split = v;
+    if (v != 0 && v != 12 && v != 13) return -2;
+    ctx().gemm_split = v;
     return 0;
 }
-int get_gemm_split() { return g_gemm_split; }
-int set_plane_tile(int v) {
-    if (v != 11 && v != 21 && v != 12 && v != 22 && v != 111 && v != 121 && v != 112 && v != 1142 && v != 1124 &&
-        v != 1144 && v != 1242 && v != 2124 && v != 3011 && v != 3142)
-        return -2;
-    g_plane_tile = v;
-    return 0;
-}
-int get_plane_tile() { return g_plane_tile; }
+int get_gemm_split() { return ctx().gemm_split; }
+// The 64 x 64 tile is the only one kept (wider tiles and two LDS buffers measured slower).
+int set_plane_tile(int v) { return v == 11 ? 0 : -2; }
+int get_plane_tile() { return 11; }
 int set_plane_group(int v) {
     if (v < 0 || v > 64) return -2;
-    g_plane_group = v;
+    ctx().plane_group = v;
     return 0;
 }
-int get_plane_group() { return g_plane_group; }
+int get_plane_group() { return ctx().plane_group; }
 
-// C -= A * B for `batch` independent problems.
+// Whether a complex-float launch of this shape runs on the split-BF16 plane kernels under the
+// handle's mode: the split passes cost more than they save when C is narrow (measured at
+// m = 2496, k = 256..2048: 3 terms lose to the FP32 matrix cores below ~160 columns, 2 terms
+// below ~96).
+static bool plane_launch(int mode, int m, int n) {
+    const int narrow = (m < n ? m : n);
+    return mode >= 12 && narrow >= ((mode % 10) == 3 ? 160 : 96);
+}
+
+// C -= A * B for `batch` independent problems with complex-float GEMM mode `mode` (0, 12, 13).
+// P: plane workspace of at least plane_bytes for this shape, or null for the handle's own;
+// prof: record the launch when the handle's timing is on.
 template <typename T>
-void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb, long sB, void* C,
-              long ldc, long sC, int batch, hipStream_t s) {
+static void gemm_sub_impl(int mode, bool prof, int m, int n, int k, const void* A, long lda, long sA, const void* B,
+                          long ldb, long sB, void* C, long ldc, long sC, int batch, hipStream_t s, void* P) {
     if (m <= 0 || n <= 0 || k <= 0 || batch <= 0) return;
+    Ctx& c = ctx();
+    prof = prof && c.prof_on;
     dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN, batch);
     ProfRec rec{};
-    if (g_prof_on) {
-        rec.a = pool_get();
-        rec.b = pool_get();
+    if (prof) {
+        rec.a = c.event();
+        rec.b = c.event();
         rec.flops = 8.0 * m * (double)n * k * batch;
         // A and B read once, C read and written once
         rec.bytes = 2.0 * sizeof(T) * ((double)m * k + (double)k * n + 2.0 * m * n) * batch;
@@ -922,48 +564,41 @@ void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void*
     // 16-byte loads need even leading dimensions and batch strides and 16-byte aligned bases
     const bool vec = sizeof(T) == 4 && ((lda | ldb | sA | sB) & 1) == 0 && ((uintptr_t)A & 15) == 0 &&
                      ((uintptr_t)B & 15) == 0;
-    // The plane GEMM splits A (m x k) and B (k x n) per launch: with a narrow C the split passes
-    // outweigh the faster products (measured at m = 2496, k = 256..2048: 3 terms lose to the
-    // FP32 matrix cores below ~160 columns, 2 terms below ~96), so narrow launches keep FP32.
-    const int narrow = (m < n ? m : n);
-    if (sizeof(T) == 4 && g_gemm_split >= 12 && narrow >= (g_gemm_split == 13 ? 160 : 96)) {
-        if (g_prof_on) {
+    if (sizeof(T) == 4 && plane_launch(mode, m, n)) {
+        const int terms = mode % 10;
+        if (prof) {
             // the kernel reads A and B as bfloat16 planes (2 bytes x 2 components x terms)
-            const double terms = g_gemm_split - 10;
             rec.bytes = (4.0 * terms * ((double)m * k + (double)k * n) + 16.0 * m * n) * batch;
             rec.plane = 1;
-            g_prof_pending = rec;
         }
-        const bool ok = g_gemm_split == 12
-                            ? gemm_planes<2>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s)
-                            : gemm_planes<3>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
+        ProfRec* rp = prof ? &rec : nullptr;
+        const bool ok = terms == 2 ? gemm_planes<2>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp)
+                                   : gemm_planes<3>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp);
         if (ok) {
-            if (g_prof_on) {
+            if (prof) {
                 hipEventRecord(rec.b, s);
-                g_prof.push_back(rec);
+                c.prof.push_back(rec);
             }
             return;
         }
-        if (g_prof_on) hipEventRecord(rec.a, s);    // fell through: time the kernel below
+        if (prof) hipEventRecord(rec.a, s);    // fell through: time the kernel below
     }
-    if (vec && g_gemm_split >= 2 && g_gemm_split < 12) {
-        dim3 g2((m + ST - 1) / ST, (n + ST - 1) / ST, batch);
-        if (g_gemm_split == 2)
-            hipLaunchKernelGGL(k_gemm_sub_c64s<2>, g2, dim3(256), 0, s, m, n, k, (const cx<float>*)A, lda, sA,
-                               (const cx<float>*)B, ldb, sB, (cx<float>*)C, ldc, sC);
-        else
-            hipLaunchKernelGGL(k_gemm_sub_c64s<3>, g2, dim3(256), 0, s, m, n, k, (const cx<float>*)A, lda, sA,
-                               (const cx<float>*)B, ldb, sB, (cx<float>*)C, ldc, sC);
-    } else if (vec)
+    if (vec)
         hipLaunchKernelGGL(k_gemm_sub_c64v, grid, dim3(256), 0, s, m, n, k, (const cx<float>*)A, lda, sA,
                            (const cx<float>*)B, ldb, sB, (cx<float>*)C, ldc, sC);
     else
         hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, s, m, n, k, (const cx<T>*)A, lda, sA, (const cx<T>*)B,
                            ldb, sB, (cx<T>*)C, ldc, sC);
-    if (g_prof_on) {
+    if (prof) {
         hipEventRecord(rec.b, s);
-        g_prof.push_back(rec);
+        c.prof.push_back(rec);
     }
+}
+
+template <typename T>
+void gemm_sub(int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb, long sB, void* C,
+              long ldc, long sC, int batch, hipStream_t s) {
+    gemm_sub_impl<T>(ctx().gemm_split, true, m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, nullptr);
 }
 
 template void gemm_sub<double>(int, int, int, const void*, long, long, const void*, long, long, void*, long, long,

@@ -142,6 +142,18 @@ __device__ void block_argmax(double& v, int& idx, double* shv, int* shi) {
 
 // One workgroup per admissible block. U: m x kr, V: n x kr complex double (kr = min(m, n, kmax))
 // at uoff[b], voff[b]; rank[4 b + kind] = number of terms kept.
+// Confirmation row of the ACA stopping test: an unused row chosen from a fixed pseudo-random
+// sequence of the block and step (the same sequence as oracle/aca_oracle.py), m if none is left.
+__device__ inline int aca_fresh_row(const unsigned char* used_r, int m, int b, int r) {
+    const unsigned long long h = (6364136223846793005ULL * (unsigned long long)(b * 131 + r + 1) + 1442695040888963407ULL);
+    const int start = (int)((h >> 33) % (unsigned long long)m);
+    for (int q = 0; q < m; ++q) {
+        const int i = (start + q) % m;
+        if (!used_r[i]) return i;
+    }
+    return m;
+}
+
 __global__ __launch_bounds__(ACA_T) void k_hm_aca(GreenGeom a, const int* c2p, int kind, const int* lr,
                                                   const long* uoff, const long* voff, cx<double>* U, cx<double>* V,
                                                   int* rank_out, double htol, int kmax) {
@@ -158,6 +170,7 @@ __global__ __launch_bounds__(ACA_T) void k_hm_aca(GreenGeom a, const int* c2p, i
     __syncthreads();
     const int maxr = min(min(m, n), kmax);
     int pr = 0, rank = 0;
+    bool confirming = false;
     double frob = 0;
     for (int r = 0; r < maxr; ++r) {
         // residual row pr -> V[:, r]
@@ -243,7 +256,18 @@ __global__ __launch_bounds__(ACA_T) void k_hm_aca(GreenGeom a, const int* c2p, i
         rank = r + 1;
         const double term = acc[0] * acc[1];
         frob += term + cross;
-        if (frob > 0 && sqrt(term) < htol * sqrt(fabs(frob))) break;
+        if (frob > 0 && sqrt(term) < htol * sqrt(fabs(frob))) {
+            // converged by the cross just taken: accept only if a fresh row, taken away from the
+            // pivot sequence, confirms it (its cross is small too)
+            if (confirming) break;
+            confirming = true;
+            if (t == 0) shi[0] = aca_fresh_row(used_r, m, b, r);
+            __syncthreads();
+            bi = shi[0];
+            __syncthreads();
+        } else {
+            confirming = false;
+        }
         if (bi >= m) break;
         pr = bi;
     }

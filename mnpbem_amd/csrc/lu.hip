@@ -12,6 +12,7 @@
 // After the factorisation k_perm turns the interchange list into a row permutation pf
 // (P b = b[pf]) and its inverse, so the solves gather instead of swapping sequentially.
 #include "common.hpp"
+#include "ctx.hpp"
 
 #include <stdint.h>
 #include <stdlib.h>
@@ -118,7 +119,6 @@ template <>
 struct RegChunk<double> {
     static constexpr int CW = 4;
 };
-static bool g_panel_reg = getenv("MNP_PANEL_REG") ? atoi(getenv("MNP_PANEL_REG")) != 0 : true;
 
 template <typename T, int CW>
 __global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>* A, long lda, long sA, int* ipiv,
@@ -329,7 +329,6 @@ struct CoopRows<double> {
     static constexpr int value = 256;
 };
 constexpr long CP_SPIN_LIMIT = 1L << 22;
-static int g_coop_min_rows = -1;   // tall-panel threshold (rows below the diagonal block)
 
 __device__ __forceinline__ void matrix_barrier(unsigned* cnt, unsigned target, int* flag) {
     __threadfence();
@@ -456,11 +455,8 @@ __global__ __launch_bounds__(CoopRows<T>::value) void k_panel_coop(int n, int j0
             if (c < w) A[row + (long)(j0 + c) * lda] = x[c];
 }
 
-// Scratch of the cooperative panel (grows on demand, kept for the process): candidates,
+// Scratch of the cooperative panel (in the calling thread's handle, grown on demand): candidates,
 // diagonal rows and the barrier counters (batch x n).
-static void* g_coop_ws = nullptr;
-static size_t g_coop_bytes = 0;
-
 struct CoopWS {
     void* crow;
     void* drow;
@@ -478,29 +474,26 @@ __global__ void k_zero_u32(long count, unsigned* p) {
 // Sizes the scratch for panels of up to n rows and clears the counters; false if unusable.
 template <typename T>
 static bool coop_prepare(int n, int batch, hipStream_t s, CoopWS& ws) {
-    if (g_coop_min_rows < 0) {
-        const char* e = getenv("MNP_PANEL_COOP_MIN_ROWS");
-        g_coop_min_rows = e ? atoi(e) : 4096;
-    }
-    if (n < g_coop_min_rows) return false;
+    Ctx& h = ctx();
+    if (n < h.coop_min_rows) return false;
     constexpr int CP_T = CoopRows<T>::value;
     const int gmax = (n + CP_T - 1) / CP_T;
     const size_t ncand = 2 * (size_t)batch * gmax;
     const size_t bytes = ncand * CP_W * sizeof(cx<T>) + 2 * (size_t)batch * CP_W * sizeof(cx<T>) +
                          ncand * (sizeof(T) + sizeof(int)) + (size_t)batch * n * sizeof(unsigned) + 1024;
-    if (bytes > g_coop_bytes) {
-        if (g_coop_ws) {
+    if (bytes > h.coop_bytes) {
+        if (h.coop_ws) {
             hipStreamSynchronize(s);
-            hipFree(g_coop_ws);
+            hipFree(h.coop_ws);
         }
-        if (hipMalloc(&g_coop_ws, bytes) != hipSuccess) {
-            g_coop_ws = nullptr;
-            g_coop_bytes = 0;
+        if (hipMalloc(&h.coop_ws, bytes) != hipSuccess) {
+            h.coop_ws = nullptr;
+            h.coop_bytes = 0;
             return false;
         }
-        g_coop_bytes = bytes;
+        h.coop_bytes = bytes;
     }
-    char* p = (char*)g_coop_ws;
+    char* p = (char*)h.coop_ws;
     ws.crow = p;
     p += ncand * CP_W * sizeof(cx<T>);
     ws.drow = p;
@@ -530,7 +523,7 @@ static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* i
     }
     const int G = (n - j0 + CP_T - 1) / CP_T;
     // a barrier per column costs more than a single workgroup walking a short panel
-    if (n - j0 < g_coop_min_rows || w > CP_W || G > ws.gmax || (long)G * batch > capacity) return false;
+    if (n - j0 < ctx().coop_min_rows || w > CP_W || G > ws.gmax || (long)G * batch > capacity) return false;
     // candidate slots are laid out for G = gmax
     T* cv = (T*)ws.cv;
     int* ci = ws.ci;
@@ -641,21 +634,11 @@ static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, 
 // Two-level blocking: outer blocks of NBO columns carry the large trailing update (a GEMM with
 // k = NBO, so C is read and written once per NBO columns), inner panels of nb columns are
 // factored by k_panel inside the outer block.
-static int g_nbo = getenv("MNP_LU_NBO") ? atoi(getenv("MNP_LU_NBO")) : 128;
-static bool g_lookahead = getenv("MNP_LU_LOOKAHEAD") ? atoi(getenv("MNP_LU_LOOKAHEAD")) != 0 : true;
-static int coop_min_rows() {
-    if (g_coop_min_rows < 0) {
-        const char* e = getenv("MNP_PANEL_COOP_MIN_ROWS");
-        g_coop_min_rows = e ? atoi(e) : 4096;
-    }
-    return g_coop_min_rows;
-}
-
-// Tuning knobs (process wide): 0 = outer block width (columns, >= nb), 1 = lookahead on/off,
-// 2 = register-resident panel on/off, 3 = rows from which panels use the cooperative kernel,
-// 4 = complex-float GEMM operand terms (0: FP32 matrix cores, 2 or 3: split BF16 per tile,
-// 12 or 13: split once into planes), 5 = workgroup tile of the plane GEMM (11, 21, 12, 22),
-// 6 = row tiles per group of its XCD-aware tile order (0: grid order).
+// Tuning knobs of the calling thread's handle: 0 = outer block width (columns, >= nb), 1 =
+// lookahead on/off, 2 = register-resident panel on/off, 3 = rows from which panels use the
+// cooperative kernel, 4 = complex-float GEMM mode (0: FP32 matrix cores; 12, 13: split-BF16
+// planes of 2 or 3 terms), 5 = workgroup tile of the plane GEMM (11 only), 6 = row tiles per
+// group of its XCD-aware tile order (0: grid order).
 int set_gemm_split(int v);
 int get_gemm_split();
 int set_plane_tile(int v);
@@ -667,19 +650,19 @@ int set_tuning(int key, int value) {
         case 4: return set_gemm_split(value);
         case 5: return set_plane_tile(value);
         case 6: return set_plane_group(value);
-        case 0: if (value < 16 || value > 1024) return -2; g_nbo = value; return 0;
-        case 1: g_lookahead = value != 0; return 0;
-        case 2: g_panel_reg = value != 0; return 0;
-        case 3: if (value < 256) return -2; g_coop_min_rows = value; return 0;
+        case 0: if (value < 16 || value > 1024) return -2; ctx().nbo = value; return 0;
+        case 1: ctx().lookahead = value != 0; return 0;
+        case 2: ctx().panel_reg = value != 0; return 0;
+        case 3: if (value < 256) return -2; ctx().coop_min_rows = value; return 0;
         default: return -1;
     }
 }
 int get_tuning(int key) {
     switch (key) {
-        case 0: return g_nbo;
-        case 1: return g_lookahead ? 1 : 0;
-        case 2: return g_panel_reg ? 1 : 0;
-        case 3: return coop_min_rows();
+        case 0: return ctx().nbo;
+        case 1: return ctx().lookahead ? 1 : 0;
+        case 2: return ctx().panel_reg ? 1 : 0;
+        case 3: return ctx().coop_min_rows;
         case 4: return get_gemm_split();
         case 5: return get_plane_tile();
         case 6: return get_plane_group();
@@ -692,28 +675,29 @@ int get_tuning(int key) {
 // while the next block's panels are factored: those touch only the next block's columns, and
 // the interchanges outside the block wait for the rest to finish (event).
 struct SideStream {
-    hipStream_t st = nullptr;
-    hipEvent_t ready = nullptr, done = nullptr;
+    hipStream_t st;
+    hipEvent_t ready, done;
 };
-static SideStream& side_stream() {
-    static SideStream ss;
-    if (!ss.st) {
-        hipStreamCreateWithFlags(&ss.st, hipStreamNonBlocking);
-        hipEventCreateWithFlags(&ss.ready, hipEventDisableTiming);
-        hipEventCreateWithFlags(&ss.done, hipEventDisableTiming);
+static SideStream side_stream() {
+    Ctx& h = ctx();
+    if (!h.side) {
+        hipStreamCreateWithFlags(&h.side, hipStreamNonBlocking);
+        hipEventCreateWithFlags(&h.side_ready, hipEventDisableTiming);
+        hipEventCreateWithFlags(&h.side_done, hipEventDisableTiming);
     }
-    return ss;
+    return SideStream{h.side, h.side_ready, h.side_done};
 }
 
 template <typename T>
 void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                    hipStream_t s) {
     cx<T>* a = (cx<T>*)A;
-    const int NB = (g_nbo > nb) ? (g_nbo / nb) * nb : nb;
+    Ctx& h = ctx();
+    const int NB = (h.nbo > nb) ? (h.nbo / nb) * nb : nb;
     CoopWS ws{};
     const bool coop = coop_prepare<T>(n, batch, s, ws);
-    SideStream& side = side_stream();
-    const bool lookahead = g_lookahead;
+    SideStream side = side_stream();
+    const bool lookahead = h.lookahead;
     bool pending = false;   // a rest update is running on the side stream
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int W = (n - J0 < NB) ? (n - J0) : NB;
@@ -723,7 +707,7 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
             int w = (JE - j0 < nb) ? (JE - j0) : nb;
             if (!coop || !panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, ws, s)) {
                 // complex float only: complex double needs 4-column chunks and runs slower than k_panel
-                if (sizeof(T) == 4 && n - j0 <= RP_T * RP_RPT && w <= RP_MAXW && g_panel_reg)
+                if (sizeof(T) == 4 && n - j0 <= RP_T * RP_RPT && w <= RP_MAXW && h.panel_reg)
                     hipLaunchKernelGGL((k_panel_reg<T, RegChunk<T>::CW>), dim3(batch), dim3(RP_T), 0, s, n, j0, w, a, lda,
                                        sA, ipiv, sP, info);
                 else
@@ -892,7 +876,8 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
     const cx<T>* a = (const cx<T>*)LU;
     cx<T>* bm = (cx<T>*)B;
     dim3 g((m + 127) / 128, batch);
-    const int NB = (g_nbo > nb) ? (g_nbo / nb) * nb : nb;
+    const int nbo = ctx().nbo;
+    const int NB = (nbo > nb) ? (nbo / nb) * nb : nb;
     auto upper = [&](int j0, int w) {
         if (nb <= 32)
             hipLaunchKernelGGL((k_rtrsm_upper<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);

@@ -9,6 +9,8 @@ Per wavelength batch:
   assemble G1 G2 H1 H2 -> factor (LU chain) -> plane-wave right-hand side -> solve
   (+ FP64 iterative refinement in mixed precision) -> far field -> cross sections.
 """
+import warnings
+
 import numpy as np
 import torch
 
@@ -44,6 +46,19 @@ def connectivity(inout):
             b = io[:, r2][None, :]
             con[r1][r2] = np.where(a == b, a, 0).astype(np.int32)
     return con
+
+
+class SingularMatrixWarning(RuntimeWarning):
+    """An LU factor met an exact zero pivot (LAPACK info > 0): the solution holds inf/NaN."""
+
+
+def warn_singular(who, info):
+    """Warn about zero pivots (LAPACK zgetrf convention: info j > 0 = column j-1), as scipy's
+    LinAlgWarning does; the reference does not check (utils/gpu.py:189-191)."""
+    bad = np.asarray(info).ravel()
+    bad = bad[bad > 0]
+    warnings.warn("{}: exactly singular factor (zero pivot in column {}); results hold inf/NaN"
+                  .format(who, int(bad[0]) - 1), SingularMatrixWarning, stacklevel=3)
 
 
 class DeviceGeometry:
@@ -200,6 +215,10 @@ class RetSolverBatch:
         if np.any(info < 0):
             raise RuntimeError("LU panel synchronisation timed out (info {}); factors are invalid".format(info))
         self.stats["zero_pivot"] = int(np.count_nonzero(info))
+        # mixed precision: a zero pivot of the complex64 chain shows up as a stalled refinement,
+        # and that wavelength is redone with the complex128 chain
+        if self.precision != "mixed" and np.any(info > 0):
+            warn_singular("BEMRet ({})".format(self.precision), info)
         return self
 
     def check_info(self):
@@ -227,6 +246,11 @@ class RetSolverBatch:
         n, nl = self.geo.n, self.nlam
         if rhs.dim() != 3 or rhs.shape[1:] != (8, n) or rhs.stride() != (8 * n, n, 1):
             raise ValueError("rhs must be a contiguous (batch, 8, n) tensor")
+        if rhs.shape[0] < nl:
+            raise ValueError("rhs holds {} right-hand sides; the factors cover {} wavelengths"
+                             .format(rhs.shape[0], nl))
+        if rhs.dtype != C128:
+            raise ValueError("rhs must be complex128")
         if self.precision == "double":
             out = torch.empty(rhs.shape, dtype=rhs.dtype, device=rhs.device)
             self._chain(rhs, out)

@@ -12,6 +12,13 @@ THIS MODULE IS THE CHECKER, NOT THE PRODUCT. Only ``tests/`` may import it.
     |u| |v| < htol |S|_F; next pivot row = argmax |u| with used rows set to 0;
   * at most min(m, n, kmax) terms.
 
+``confirm=True`` (the product's rule, not the reference's): a stop is accepted only when the
+next cross, taken from a fresh row (an unused row from a fixed pseudo-random sequence of the
+block index and step, ``fresh_row``), meets the criterion too; otherwise the pivoting goes on.
+The reference's rule (``confirm=False``) stops at the first small cross, which on blocks with
+paired singular values (two rods related by symmetry) can leave a residual of 1e-3 of the block
+whatever htol is.
+
 Pinned in tests/test_hmatrix_host.py by exact recovery of random low-rank matrices (rank r
 matrices come back with r terms and rounding-level error) and by the error of smooth
 Green-function blocks against the truncated SVD.
@@ -19,7 +26,21 @@ Green-function blocks against the truncated SVD.
 import numpy as np
 
 
-def aca(A, htol=1e-6, kmax=100):
+def fresh_row(used_r, block, step):
+    """Unused row from the pseudo-random sequence of (block, step): start at
+    ((6364136223846793005 (131 block + step + 1) + 1442695040888963407) mod 2^64) >> 33 mod m and
+    walk forward to the first unused row; m when all rows are used."""
+    m = len(used_r)
+    h = (6364136223846793005 * (block * 131 + step + 1) + 1442695040888963407) % (1 << 64)
+    start = (h >> 33) % m
+    for q in range(m):
+        i = (start + q) % m
+        if not used_r[i]:
+            return i
+    return m
+
+
+def aca(A, htol=1e-6, kmax=100, confirm=True, block=0):
     """Returns U (m, k), V (n, k) with A ~ U V^T, and k."""
     A = np.asarray(A)
     m, n = A.shape
@@ -31,7 +52,8 @@ def aca(A, htol=1e-6, kmax=100):
     pr = 0
     frob = 0.0
     k = 0
-    for _ in range(maxr):
+    confirming = False
+    for step in range(maxr):
         row = A[pr, :].astype(complex) - V[:, :k] @ U[pr, :k]
         ar = np.abs(row)
         ar[used_c] = 0.0
@@ -54,9 +76,17 @@ def aca(A, htol=1e-6, kmax=100):
             dv = np.conj(V[:, :k - 1]).T @ v
             cross = 2.0 * float(np.real(np.dot(du, dv)))
         frob += term + cross
-        if frob > 0 and np.sqrt(term) < htol * np.sqrt(abs(frob)):
-            break
         au = np.abs(u)
-        au[used_r] = 0.0
-        pr = int(np.argmax(au))
+        au[used_r] = -1.0
+        nxt = int(np.argmax(au)) if np.any(~used_r) else m
+        if frob > 0 and np.sqrt(term) < htol * np.sqrt(abs(frob)):
+            if not confirm or confirming:
+                break
+            confirming = True
+            nxt = fresh_row(used_r, block, step)
+        else:
+            confirming = False
+        if nxt >= m:
+            break
+        pr = nxt
     return U[:, :k], V[:, :k], k

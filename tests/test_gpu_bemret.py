@@ -92,9 +92,16 @@ def test_bemret_solution_matches_oracle(sphere, sphere_oracle):
     G1, G2, H1, H2 = O.matrices(sphere_oracle, ks)
     e1, e2 = cp.eps1(lam), cp.eps2(lam)
     ob = O.BEMRetOracle(G1, G2, H1, H2, s.nvec, e1, e2, 2 * np.pi / lam)
-    pot = exc.potential(cp, lam)
-    phi, a, alpha, De = O.excitation(s.nvec, 2 * np.pi / lam, e1, e2, pot.a1, pot.a1p, pot.a2, pot.a2p)
+    # the oracle's own plane wave (planewave_ret.py:257-361): outside faces excited, medium 1
+    a2, a2p = O.planewave(s.pos, s.nvec, np.array([1.0, 0, 0]), np.array([0, 0, 1.0]), lam, 1.0, np.arange(s.n))
+    z = np.zeros_like(a2)
+    phi, a, alpha, De = O.excitation(s.nvec, 2 * np.pi / lam, e1, e2, z, z, a2, a2p)
     ref = ob.solve(phi, a, alpha, De)
+    # the product's host excitation equals the oracle's
+    pot = exc.potential(cp, lam)
+    assert np.max(np.abs(pot.a2 - a2)) / np.max(np.abs(a2)) < 1e-14
+    assert np.max(np.abs(pot.a2p - a2p)) / np.max(np.abs(a2p)) < 1e-14
+    assert np.max(np.abs(pot.a1)) == 0 and np.max(np.abs(pot.a1p)) == 0
     for got, want in zip((sig.sig1, sig.h1, sig.sig2, sig.h2), ref):
         assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
     # cross sections through the device far field
@@ -194,3 +201,100 @@ def test_bemstat_matches_matlab_table():
         sig, _ = bem.solve(exc(cp, row[0]))
         assert abs(exc.extinction(sig) - row[1]) / row[1] < 1e-9
         assert abs(exc.scattering(sig) - row[2]) / row[2] < 1e-9
+
+
+def test_bem_operator_matches_oracle_afun(sphere, sphere_oracle):
+    """mnp_bem_operator (the exact complex128 operator of the refinement and of BEMRetIter,
+    bem_ret_iter.py:1362-1501) against the oracle's afun on random packed vectors, two
+    wavelengths in one batch; also the residual form y = rhs - A x and its norms."""
+    from mnpbem_amd import _lib
+    from mnpbem_amd.engine import stream_handle
+    cp, s, fp, closed = sphere
+    lams = np.array([480.0, 705.0])
+    banner("bem operator vs oracle afun n={}".format(s.n))
+    geo = M.BEMRet(cp).g.geo
+    kmat, L1, L2, kvac, _, _ = geo.media(lams)
+    gb = GreenBatch(geo, 2, torch.complex128)
+    gb.assemble(kmat, 2)
+    n = s.n
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 8 * n)) + 1j * rng.standard_normal((2, 8 * n))
+    rhs = rng.standard_normal((2, 8 * n)) + 1j * rng.standard_normal((2, 8 * n))
+    xd = torch.tensor(x, device=geo.dev)
+    rd = torch.tensor(rhs, device=geo.dev)
+    y = torch.empty_like(xd)
+    work = torch.empty((2, 32, n), dtype=torch.complex128, device=geo.dev)
+    norms = torch.zeros((2, 2), dtype=torch.float64, device=geo.dev)
+    p = lambda t: t.data_ptr()
+    _lib.call("mnp_bem_operator", n, 2, p(gb.G1), p(gb.G2), p(gb.H1), p(gb.H2), gb.ld, gb.stride, p(geo.nvec),
+              p(L1), p(L2), p(kvac), p(xd), 8 * n, None, 8 * n, p(y), 8 * n, p(work), 32 * n, None, stream_handle())
+    yr = torch.empty_like(xd)
+    _lib.call("mnp_bem_operator", n, 2, p(gb.G1), p(gb.G2), p(gb.H1), p(gb.H2), gb.ld, gb.stride, p(geo.nvec),
+              p(L1), p(L2), p(kvac), p(xd), 8 * n, p(rd), 8 * n, p(yr), 8 * n, p(work), 32 * n, p(norms),
+              stream_handle())
+    y, yr, nr = y.cpu().numpy(), yr.cpu().numpy(), norms.cpu().numpy()
+    for b, lam in enumerate(lams):
+        ks = [e(lam)[1] for e in cp.eps]
+        e1, e2, k0 = cp.eps1(lam), cp.eps2(lam), 2 * np.pi / lam
+        # the operator's arithmetic on the device's own matrices: 1e-13
+        Gd = [GreenBatch.to_numpy(t, b) for t in (gb.G1, gb.G2, gb.H1, gb.H2)]
+        want = O.afun(*Gd, s.nvec, e1, e2, k0, x[b])
+        assert np.max(np.abs(y[b] - want)) / np.max(np.abs(want)) < 1e-13
+        res = rhs[b] - want
+        assert np.max(np.abs(yr[b] - res)) / np.max(np.abs(res)) < 1e-13
+        # end to end on the oracle's matrices: their refined self terms agree to ~1e-10
+        G1, G2, H1, H2 = O.matrices(sphere_oracle, ks)
+        want_o = O.afun(G1, G2, H1, H2, s.nvec, e1, e2, k0, x[b])
+        assert np.max(np.abs(y[b] - want_o)) / np.max(np.abs(want_o)) < 1e-10
+        assert abs(nr[b, 0] - np.vdot(res, res).real) / np.vdot(res, res).real < 1e-12
+        assert abs(nr[b, 1] - np.vdot(rhs[b], rhs[b]).real) / np.vdot(rhs[b], rhs[b]).real < 1e-12
+
+
+def test_mixed_stall_falls_back_to_double(sphere):
+    """With no refinement step allowed every wavelength stalls: each is re-solved with the
+    complex128 chain (BEMRet._redo_double) and the sweep equals the complex128 sweep."""
+    cp, s, fp, closed = sphere
+    lams = np.linspace(420, 780, 5)
+    banner("mixed precision stall fallback")
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    ref = M.BEMRet(cp, batch=5).solve_spectrum(exc, lams, scattering=True)
+    out = M.BEMRet(cp, batch=5, precision="mixed", max_refine=0, autotune=False).solve_spectrum(
+        exc, lams, scattering=True)
+    assert sorted(out["fallback"]) == sorted(lams.tolist())
+    assert np.max(np.abs(out["ext"] - ref["ext"]) / np.abs(ref["ext"])) < 1e-12
+    assert np.max(np.abs(out["sca"] - ref["sca"]) / np.abs(ref["sca"])) < 1e-12
+    # a partial fallback: the refinement tolerance below what the residual can reach stalls
+    # some wavelengths only after a few steps; the result still equals the complex128 sweep
+    out2 = M.BEMRet(cp, batch=5, precision="mixed", refine_tol=1e-300, max_refine=6, autotune=False).solve_spectrum(
+        exc, lams)
+    assert len(out2["fallback"]) == len(lams)
+    assert np.max(np.abs(out2["ext"] - ref["ext"]) / np.abs(ref["ext"])) < 1e-12
+
+
+def test_solve_after_spectrum_uses_fresh_factors(sphere, sphere_oracle):
+    """A spectrum sweep refactors the solver object that init() made; a later solve() at the
+    wavelength init() saw must factor again (and must not read a batch-sized right-hand side)."""
+    cp, s, fp, closed = sphere
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    banner("solve after solve_spectrum")
+    for batch in (1, 4):
+        bem = M.BEMRet(cp, enei=600.0, batch=batch, autotune=False)
+        bem.solve_spectrum(exc, np.array([450.0, 520.0, 640.0, 780.0]))
+        sig, _ = bem.solve(exc.potential(cp, 600.0))
+        fresh, _ = M.BEMRet(cp).solve(exc.potential(cp, 600.0))
+        for f in ("sig1", "h1", "sig2", "h2"):
+            a, b = getattr(sig, f), getattr(fresh, f)
+            assert np.max(np.abs(a - b)) / np.max(np.abs(b)) < 1e-12
+    ks = [e(600.0)[1] for e in cp.eps]
+    ext_ref = O.solve_spectrum(s, fp, [[2, 1]], _oracle_eps(cp), [600.0], np.array([1.0, 0, 0]),
+                               np.array([0, 0, 1.0]), closed_sets=closed)["ext"][0]
+    assert abs(exc.extinction(sig) - ext_ref) / abs(ext_ref) < 1e-10
+
+
+def test_solver_rejects_short_rhs(sphere):
+    cp, s, fp, closed = sphere
+    banner("short rhs rejected")
+    slv = RetSolverBatch(M.BEMRet(cp).g.geo, 3, "double")
+    slv.factor([500.0, 600.0, 700.0])
+    with pytest.raises(ValueError):
+        slv.solve(torch.zeros((1, 8, s.n), dtype=torch.complex128, device="cuda"))

@@ -1,0 +1,181 @@
+"""Parity of the benchmark workloads themselves against the CPU oracle.
+
+Each test runs the device path on the geometry, media and excitation of one benchmark workload
+and compares it with the FP64 oracle (numpy + LAPACK restatement of bem/bem_ret.py:293-525,
+:1569-1831 and bem/bem_ret_iter.py:1362-1744):
+
+* capped rod 10 x 40 nm, 2,496 curved triangles, eps_out 1.77, gold Drude, axial plane wave:
+  mixed precision (3- and 2-term split-BF16 GEMMs + FP64 refinement) vs the oracle at the
+  longitudinal resonance and at 600 nm (surface charges/currents and ext/sca <= 1e-8);
+* the smallest sphere, prolate ellipsoid and gap dimer of the 64-particle batch (eps 1.33),
+  mixed precision vs the oracle;
+* a rod dimer of two 20 x 80 nm rods (about 1.4k faces each, 2 nm gap) through BEMRetIter with
+  H-matrices (eta 2.5, leaf 64, ACA tolerance 1e-6, GMRES(50) to 1e-6) vs the oracle (1e-5);
+* the full 24,544-face dimer: H-matrix GMRES vs the dense complex128 BEMRet on the device.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, banner
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import bem_oracle as O                                       # noqa: E402
+import mnpbem_amd as M                                                   # noqa: E402
+
+DRUDE = (10.0, 9.065, 0.0708)
+PINF = np.load(os.path.join(ROOT, "mnpbem_amd", "data", "pinfty256.npz"))
+
+
+def _rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b)))
+
+
+def _oracle(cp, lams, pol, direc, medium=1):
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+    funcs = [lambda lam, e=e: e(lam) for e in cp.eps]
+    out = O.solve_spectrum(s, cp.face_particle, cp.inout, funcs, lams, np.asarray(pol, float),
+                           np.asarray(direc, float), pinf_nvec=PINF["nvec"], pinf_area=PINF["area"], medium=medium,
+                           return_fields=True)
+    return s, out
+
+
+def _check_solution(sig, ref, tol):
+    for name, got, want in zip(("sig1", "h1", "sig2", "h2"), (sig.sig1, sig.h1, sig.sig2, sig.h2), ref):
+        err = _rel(got, want)
+        assert err < tol, (name, err)
+
+
+@pytest.fixture(scope="module")
+def rod2496():
+    rod = M.trirod(10.0, 40.0, [25, 11, 34], triangles=True)
+    cp = M.ComParticle([M.EpsConst(1.77), M.EpsDrude(*DRUDE)], [rod], [[2, 1]], 1, interp="curv")
+    exc = M.PlaneWaveRet([0.0, 0.0, 1.0], [1.0, 0.0, 0.0])
+    assert cp.nfaces == 2496
+    lams = np.linspace(500, 1000, 201)
+    banner("rod2496 mixed sweep 201 wavelengths batch 67")
+    sweep = M.BEMRet(cp, batch=67, precision="mixed").solve_spectrum(exc, lams)
+    ext = sweep["ext"][:, 0]
+    assert np.all(np.isfinite(ext)) and not sweep["fallback"]
+    lam_res = float(lams[int(np.argmax(ext))])
+    sel = [lam_res, 600.0]
+    banner("rod2496 oracle at {} nm (resonance) and 600 nm".format(lam_res))
+    s, ref = _oracle(cp, sel, [0, 0, 1.0], [1.0, 0, 0])
+    return cp, exc, lams, ext, sel, ref
+
+
+def test_rod2496_sweep_ext_matches_oracle(rod2496):
+    cp, exc, lams, ext, sel, ref = rod2496
+    # the resonance lies inside the band (not at an edge) and is the longitudinal one
+    assert 500 < sel[0] < 1000
+    for q, lam in enumerate(sel):
+        i = int(np.argmin(np.abs(lams - lam)))
+        assert abs(ext[i] - ref["ext"][q]) / abs(ref["ext"][q]) < 1e-8
+
+
+@pytest.mark.parametrize("split", [3, 2])
+def test_rod2496_mixed_solution_matches_oracle(rod2496, split):
+    """Per-wavelength API in mixed precision: sig1, h1, sig2, h2 and ext/sca vs the oracle."""
+    cp, exc, lams, ext, sel, ref = rod2496
+    bem = M.BEMRet(cp, precision="mixed", gemm_split=split)
+    for q, lam in enumerate(sel):
+        banner("rod2496 mixed split={} lam={}".format(split, lam))
+        sig, _ = bem.solve(exc.potential(cp, lam))
+        _check_solution(sig, ref["fields"][q]["sol"], 1e-8)
+        assert abs(exc.extinction(sig) - ref["ext"][q]) / abs(ref["ext"][q]) < 1e-8
+        sca, _ = exc.scattering(sig)
+        assert abs(sca - ref["sca"][q]) / abs(ref["sca"][q]) < 1e-8
+
+
+def test_rod2496_double_solution_matches_oracle(rod2496):
+    """The complex128 chain on the same workload: 1e-10."""
+    cp, exc, lams, ext, sel, ref = rod2496
+    bem = M.BEMRet(cp)
+    banner("rod2496 double lam={}".format(sel[0]))
+    sig, _ = bem.solve(exc.potential(cp, sel[0]))
+    _check_solution(sig, ref["fields"][0]["sol"], 1e-10)
+
+
+def _smallest_of_each_kind():
+    import bench
+    kinds = {}
+    for k, cp in bench.particles64():
+        if k not in kinds or cp.nfaces < kinds[k].nfaces:
+            kinds[k] = cp
+    return kinds
+
+
+@pytest.mark.parametrize("kind", ["sphere", "ellipsoid", "dimer"])
+def test_batch64_particles_match_oracle(kind):
+    """Particles of the 64-particle batch (gold Drude in eps 1.33, 32 wavelengths 450-900 nm):
+    the batched mixed sweep and a per-wavelength mixed solve vs the oracle."""
+    cp = _smallest_of_each_kind().get(kind)
+    if cp is None:
+        pytest.skip("no {} in the seeded set".format(kind))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    lams = np.linspace(450, 900, 32)
+    banner("batch64 {} n={} mixed sweep".format(kind, cp.nfaces))
+    out = M.BEMRet(cp, batch=32, precision="mixed").solve_spectrum(exc, lams, scattering=True)
+    sel = [0, 19]
+    s, ref = _oracle(cp, lams[sel], [1.0, 0, 0], [0, 0, 1.0])
+    assert _rel(out["ext"][sel, 0], ref["ext"]) < 1e-8
+    assert _rel(out["sca"][sel, 0], ref["sca"]) < 1e-8
+    sig, _ = M.BEMRet(cp, precision="mixed").solve(exc.potential(cp, lams[19]))
+    _check_solution(sig, ref["fields"][1]["sol"], 1e-8)
+
+
+def _rod_dimer(nrod, gap=2.0):
+    p1 = M.trirod(20, 80, nrod, triangles=True).shift([0, 0, -(40 + gap / 2)])
+    p2 = M.trirod(20, 80, nrod, triangles=True).shift([0, 0, 40 + gap / 2])
+    return M.ComParticle([M.EpsConst(1.0), M.EpsDrude(*DRUDE)], [p1, p2], [[2, 1], [2, 1]], 1, 2,
+                         interp="curv")
+
+
+def test_rod_dimer_hmatrix_gmres_matches_oracle():
+    """The 25k-face workload's solver settings on a rod dimer the oracle can run: BEMRetIter
+    with H-matrices (eta 2.5, leaf 64, htol 1e-6, kmax 100) and GMRES(50) to 1e-6 at 750 nm."""
+    cp = _rod_dimer([24, 7, 20])
+    exc = M.PlaneWaveRet([0, 0, 1.0], [1.0, 0, 0])
+    lam = 750.0
+    banner("rod dimer n={} hmatrix gmres".format(cp.nfaces))
+    assert 2500 <= cp.nfaces <= 3500
+    it = M.BEMRetIter(cp, tol=1e-6, maxit=500, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5)
+    sig, _ = it.solve(exc(cp, lam))
+    flag, relres, niter = it.info()
+    assert flag == 0 and relres <= 1e-6
+    banner("rod dimer oracle")
+    s, ref = _oracle(cp, [lam], [0, 0, 1.0], [1.0, 0, 0])
+    assert abs(exc.extinction(sig) - ref["ext"][0]) / abs(ref["ext"][0]) < 1e-5
+    # GMRES to 1e-6 leaves a solution error of the same order
+    assert _rel(sig.sig2, ref["fields"][0]["sol"][2]) < 1e-4
+
+
+def test_rod_dimer_25k_hmatrix_vs_dense():
+    """Full size (24,544 faces): H-matrix GMRES to 1e-6 vs the dense complex128 BEMRet on the
+    device (whose chain is pinned against the oracle on the smaller workloads)."""
+    import bench
+    cp = bench._dimer25k()
+    exc = M.PlaneWaveRet([0, 0, 1.0], [1.0, 0, 0])
+    lam = 750.0
+    g = M.CompGreenRet(cp, cp)
+    banner("dimer25k n={} hmatrix gmres".format(cp.nfaces))
+    it = M.BEMRetIter(cp, tol=1e-6, maxit=500, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5,
+                      green=g)
+    sig_i, _ = it.solve(exc(cp, lam))
+    flag, relres, niter = it.info()
+    assert flag == 0 and relres <= 1e-6
+    ext_i = exc.extinction(sig_i)
+    del it
+    torch.cuda.empty_cache()
+    banner("dimer25k dense complex128")
+    bem = M.BEMRet(cp, green=g)
+    sig_d, _ = bem.solve(exc(cp, lam))
+    ext_d = exc.extinction(sig_d)
+    bem.clear()
+    assert abs(ext_i - ext_d) / abs(ext_d) < 1e-5
+    assert _rel(sig_i.sig2, sig_d.sig2) < 1e-4

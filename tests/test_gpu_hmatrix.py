@@ -52,14 +52,14 @@ def test_dense_blocks_and_aca(built):
         for b, (r0, c0, m, n) in enumerate(hm.lowrank_blocks()):
             blk = Ac[r0:r0 + m, c0:c0 + n]
             errs.append(np.linalg.norm(Hc[r0:r0 + m, c0:c0 + n] - blk) / np.linalg.norm(blk))
-            _, _, k = aca(blk, HTOL, 100)
+            _, _, k = aca(blk, HTOL, 100, block=b)
             same += int(k == hm.ranks[b, q])
             diff.append(abs(k - hm.ranks[b, q]))
         errs = np.array(errs)
         print("kind {} median {:.2e} max {:.2e} ranks equal {}/{}".format(q, np.median(errs), errs.max(), same,
                                                                           len(errs)))
         assert np.median(errs) < 3 * HTOL
-        assert errs.max() < 1e-3
+        assert errs.max() < 100 * HTOL
         assert same >= 0.9 * len(errs) and max(diff) <= 3
 
 

@@ -83,21 +83,15 @@ def test_gemm_sub(prec, m, n, k, lda, ldb):
         assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < tol
 
 
-@pytest.mark.parametrize("split,tile", [(2, 11), (3, 11), (12, 11), (13, 11), (12, 111), (13, 111), (13, 121),
-                                        (13, 112), (13, 21), (12, 12), (13, 22), (13, 1142), (13, 1124),
-                                        (13, 1144), (12, 1144), (13, 1242), (13, 2124), (13, 3011), (12, 3011),
-                                        (13, 3142)])
+@pytest.mark.parametrize("split,tile", [(12, 11), (13, 11)])
 @pytest.mark.parametrize("m,n,k,lda,ldb", [(200, 167, 45, 200, 46), (300, 260, 128, 300, 128), (192, 192, 16, 192, 16),
                                            (331, 270, 37, 332, 38), (171, 165, 37, 171, 37), (150, 70, 37, 150, 38)])
 def test_gemm_sub_split_bf16(split, tile, m, n, k, lda, ldb):
-    """Complex-float GEMM on the BF16 matrix cores with 2 or 3 operand terms, split per tile
-    (2, 3) or once into planes (12, 13) with the LDS-staged plane kernel (tiles 11, 21, 12, 22
-    of 4 waves; 1142, 1124, 1242, 2124 of 8 waves; 1144 of 16; 3011, 3142 with two LDS
-    buffers) or the direct one (111, 121,
-    112); ragged edges, K not a multiple of 16, padded and odd leading dimensions, and a narrow
-    C (70 columns) that the plane modes leave to the FP32 matrix cores: error against the exact
-    product within the terms kept (2^-15 and 2^-22 relative to |A||B|, with FP32
-    accumulation)."""
+    """Complex-float GEMM on the BF16 matrix cores with 2 or 3 operand terms split once into
+    planes (12, 13); ragged edges, K not a multiple of 16, padded and odd leading dimensions,
+    and a narrow C (70 columns) that the plane modes leave to the FP32 matrix cores: error
+    against the exact product within the terms kept (2^-15 and 2^-22 relative to |A||B|, with
+    FP32 accumulation)."""
     banner("gemm_sub split={} tile={} m={} n={} k={}".format(split, tile, m, n, k))
     rng = np.random.default_rng(2)
     B_ = 2
@@ -271,7 +265,7 @@ def test_autotune_picks_a_candidate_and_restores_knobs():
         _lib().call("mnp_get_tuning", key, ctypes.addressof(v))
         before.append(v.value)
     cfg = autotune.tune_lu(300, 4, torch.complex64, _dev())
-    assert {k: cfg[k] for k in ("nb", "nbo", "lookahead")} in autotune.candidates()
+    assert {k: cfg[k] for k in ("nb", "nbo", "lookahead")} in autotune.candidates(300)
     assert cfg["ms"] > 0
     after = []
     for key in (0, 1):
@@ -283,7 +277,8 @@ def test_autotune_picks_a_candidate_and_restores_knobs():
     rng = np.random.default_rng(9)
     A = _rand(rng, 300, 300)
     lu, piv = sla.lu_factor(A)
-    for c in autotune.candidates():
+    # every shape either tuner range can pick, including outer blocks wider than the matrix
+    for c in autotune.candidates(300) + autotune.candidates(2000):
         t = _cm([A], torch.complex128)
         ipiv = torch.zeros((1, 300), dtype=torch.int32, device=_dev())
         info = torch.zeros(1, dtype=torch.int32, device=_dev())

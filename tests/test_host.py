@@ -88,7 +88,39 @@ def test_library_exports_header():
         assert hasattr(lib, s), s
     from mnpbem_amd import _lib
     assert sorted(_lib.exported_symbols()) == syms
-    assert lib.mnp_abi_version() == 1
+    assert lib.mnp_abi_version() == 2
+
+
+def test_handles_keep_their_own_knobs_without_gpu_work():
+    """Knobs belong to the handle bound to the calling thread (no device work is done)."""
+    import threading
+    from mnpbem_amd import _lib
+    v = ctypes.c_int(0)
+    _lib.call("mnp_get_tuning", 0, ctypes.addressof(v))
+    default = v.value
+    h = _lib.Handle()
+    try:
+        with h:
+            _lib.call("mnp_set_tuning", 0, 512)
+            _lib.call("mnp_get_tuning", 0, ctypes.addressof(v))
+            assert v.value == 512
+            seen = []
+
+            def other():
+                w = ctypes.c_int(0)
+                _lib.call("mnp_get_tuning", 0, ctypes.addressof(w))
+                seen.append(w.value)
+            t = threading.Thread(target=other)
+            t.start()
+            t.join()
+            assert seen == [128]          # another thread has its own default handle
+        _lib.call("mnp_get_tuning", 0, ctypes.addressof(v))
+        assert v.value == default
+    finally:
+        h.close()
+    with pytest.raises(ValueError):
+        _lib.call("mnp_handle_bind", ctypes.c_void_p(12345))      # not a handle of the library
+    assert _lib.lib.mnp_release() == 0       # nothing allocated: no device call
 
 
 def test_cabi_rejects_bad_arguments_without_gpu_work():
@@ -153,3 +185,29 @@ def test_compoint_media_and_grid_expansion():
     assert gx.shape == gy.shape == gz.shape == x.shape and np.all(gy == 0)
     gx, gy, gz = _expand_grid(x, z, np.array([0.0, 1.0, 2.0]))
     assert gx.shape == (4, 5, 3) and np.array_equal(gz[1, 2], [0.0, 1.0, 2.0])
+
+
+def test_singular_factor_warns():
+    """Zero pivots (LAPACK info > 0) raise a SingularMatrixWarning, as scipy's LinAlgWarning."""
+    from mnpbem_amd.engine import SingularMatrixWarning, warn_singular
+    with pytest.warns(SingularMatrixWarning, match="column 4"):
+        warn_singular("test", np.array([[0, 5], [0, 0]]))
+
+
+def test_particles64_face_counts_realised():
+    """The 64-particle workload draws face counts uniform in 300-3000 and the meshes realise
+    them (a sphere mesh of m longitudes and t bands has exactly 2 m t faces)."""
+    import bench
+    rng = np.random.default_rng(20261203)
+    want = []
+    for _ in range(64):
+        kind = ("sphere", "ellipsoid", "dimer")[int(rng.integers(3))]
+        want.append((kind, int(rng.integers(300, 3001))))
+        if kind != "sphere":
+            rng.uniform(1, 5)
+    got = bench.particles64()
+    assert len(got) == 64
+    for (kind, cp), (k2, nf) in zip(got, want):
+        assert kind == k2
+        assert abs(cp.nfaces - nf) <= max(0.03 * nf, 16), (kind, cp.nfaces, nf)
+    assert {k for k, _ in got} == {"sphere", "ellipsoid", "dimer"}
