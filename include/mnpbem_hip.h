@@ -83,6 +83,22 @@ int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const do
 int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb,
                          long sB, void* C, long ldc, long sC, int batch, void* stream);
 
+/* C -= A B in complex float on the BF16 matrix cores with operands split into `terms` (2 or 3)
+ * bfloat16 terms: the kernels and results of mnp_gemm_sub_batched(prec = 1) under tuning key 4
+ * = 10 + terms, but allocating nothing. No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc). The operand planes live in a static device array of
+ * mnp_noalloc_scratch_bytes() (32 MiB) per GPU; a launch needs
+ * 2 terms ceil(k/16) 16 (ceil(m/128) + ceil(n/128)) 128 batch 2 bytes of it when its C has at
+ * least 160 (3 terms) or 96 (2 terms) rows and columns (narrower launches use the FP32 matrix
+ * cores and no scratch). Returns 2 without enqueuing work when that exceeds the scratch, 3 when
+ * the scratch's device address is unavailable. Calls that overlap on one GPU race on the
+ * scratch: serialise them (one stream, or events). */
+int mnp_gemm_sub_batched_noalloc(int terms, int m, int n, int k, const void* A, long lda, long sA, const void* B,
+                                 long ldb, long sB, void* C, long ldc, long sC, int batch, void* stream);
+/* Size in bytes of the static scratch behind mnp_gemm_sub_batched_noalloc (host long). */
+int mnp_noalloc_scratch_bytes(long* bytes);
+
 /* In-place LU with partial pivoting of `batch` n x n matrices (panel width nb, 1..64).
  * info[b]: 0 ok, j > 0 zero pivot in column j-1, -1 a multi-workgroup panel's barrier timed out
  * (factors invalid). Replaces lu_factor_dispatch (utils/gpu.py:145-191: scipy zgetrf / cupyx getrf). */

@@ -26,6 +26,8 @@ _SIGS = {
     "mnp_prof_collect": [P],
     "mnp_green_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, I, P, P, P, P, L, L, P],
     "mnp_gemm_sub_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
+    "mnp_gemm_sub_batched_noalloc": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
+    "mnp_noalloc_scratch_bytes": [P],
     "mnp_getrf_batched": [I, I, I, P, L, L, P, L, P, I, P],
     "mnp_perm_batched": [I, I, P, L, P, P, L, P],
     "mnp_getrs_batched": [I, I, I, I, P, L, L, P, L, P, L, L, P, L, L, P],
@@ -99,12 +101,18 @@ lib = _load()
 atexit.register(lib.mnp_release)
 
 
+_STATUS = {2: "the operand planes of this shape exceed the static scratch (see mnp_noalloc_scratch_bytes)",
+           3: "the static scratch's device address is unavailable"}
+
+
 def call(name, *args):
     """Call ``name`` and raise on a non-zero status."""
     st = getattr(lib, name)(*args)
     if st != 0:
         if st < 0:
             raise ValueError("{}: argument {} rejected".format(name, -st))
+        if st in _STATUS:
+            raise ValueError("{}: {}".format(name, _STATUS[st]))
         raise RuntimeError("{}: HIP launch failure (status {})".format(name, st))
     return st
 

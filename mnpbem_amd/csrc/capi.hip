@@ -21,6 +21,9 @@ template <typename T>
 void gemm_sub(int, int, int, const void*, long, long, const void*, long, long, void*, long, long, int, hipStream_t);
 template <typename T>
 void getrf_batched(int, int, void*, long, long, int*, long, int*, int, hipStream_t);
+int gemm_sub_noalloc(int, int, int, int, const void*, long, long, const void*, long, long, void*, long, long, int,
+                     hipStream_t);
+size_t noalloc_scratch_bytes();
 void perm_batched(int, int, const int*, long, int*, int*, long, hipStream_t);
 template <typename T>
 void rsolve_batched(int, int, int, const void*, long, long, const int*, long, void*, long, long, void*, long, long, int,
@@ -167,6 +170,23 @@ int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda,
     else
         gemm_sub<float>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, (hipStream_t)stream);
     return launch_status();
+}
+
+int mnp_gemm_sub_batched_noalloc(int terms, int m, int n, int k, const void* A, long lda, long sA, const void* B,
+                                 long ldb, long sB, void* C, long ldc, long sC, int batch, void* stream) {
+    REQUIRE(terms == 2 || terms == 3, 1);
+    REQUIRE(m >= 0 && n >= 0 && k >= 0, 2);
+    REQUIRE(A && B && C, 5);
+    REQUIRE(lda >= (m > 0 ? m : 1) && ldb >= (k > 0 ? k : 1) && ldc >= (m > 0 ? m : 1), 6);
+    REQUIRE(batch >= 1, 14);
+    const int st = gemm_sub_noalloc(terms, m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, (hipStream_t)stream);
+    return st ? st : launch_status();
+}
+
+int mnp_noalloc_scratch_bytes(long* bytes) {
+    REQUIRE(bytes, 1);
+    *bytes = (long)noalloc_scratch_bytes();
+    return 0;
 }
 
 int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,

@@ -15,6 +15,8 @@
 #include "ctx.hpp"
 
 #include <stdint.h>
+
+#include <mutex>
 #include <vector>
 
 namespace mnp {
@@ -605,6 +607,50 @@ template void gemm_sub<double>(int, int, int, const void*, long, long, const voi
                                int, hipStream_t);
 template void gemm_sub<float>(int, int, int, const void*, long, long, const void*, long, long, void*, long, long,
                               int, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
+// Entry without allocation: the operand planes of the split-BF16 GEMM live in a static device
+// array. In host code `scratch` names a host-side stand-in; its device address is taken once per
+// device with hipGetSymbolAddress and only that address is handed to kernels.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 32u << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEV = 64;
+static void* g_scratch_ptr[NOALLOC_MAX_DEV];
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device (null on failure).
+static void* scratch_ptr() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEV) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+size_t noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+// C -= A B in complex float with operands split into `terms` bfloat16 terms (as gemm_sub under
+// mode 10 + terms), planes in the static scratch. Returns 0, 2 when the planes of this shape
+// exceed the scratch (nothing enqueued), 3 when the scratch address is unavailable. Calls that
+// overlap on one device race on the scratch; every plane element a launch reads is written by
+// its own split pass, so nothing needs clearing between calls.
+int gemm_sub_noalloc(int terms, int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb,
+                     long sB, void* C, long ldc, long sC, int batch, hipStream_t s) {
+    const int mode = 10 + terms;
+    const bool planes = m > 0 && n > 0 && k > 0 && batch > 0 && plane_launch(mode, m, n);
+    void* P = nullptr;
+    if (planes) {
+        if (plane_bytes(terms, m, n, k, batch) > NOALLOC_SCRATCH_BYTES) return 2;
+        P = scratch_ptr();
+        if (!P) return 3;
+    }
+    gemm_sub_impl<float>(mode, false, m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P);
+    return 0;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Lane-layout probe: D = P * Q with P[x][k] = x + 16 k (16x16x4) or x + 16 k (32x32x2) and

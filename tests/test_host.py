@@ -211,3 +211,15 @@ def test_particles64_face_counts_realised():
         assert kind == k2
         assert abs(cp.nfaces - nf) <= max(0.03 * nf, 16), (kind, cp.nfaces, nf)
     assert {k for k, _ in got} == {"sphere", "ellipsoid", "dimer"}
+
+
+def test_noalloc_gemm_rejects_oversize_without_gpu_work():
+    """The allocation-free GEMM checks its plane size against the static scratch before any
+    device call."""
+    from mnpbem_amd import _lib
+    nb = ctypes.c_long(0)
+    _lib.call("mnp_noalloc_scratch_bytes", ctypes.byref(nb))
+    assert nb.value == 32 << 20
+    st = _lib.lib.mnp_gemm_sub_batched_noalloc(3, 2048, 2048, 4096, 16, 2048, 0, 16, 4096, 0, 16, 2048, 0, 1, None)
+    assert st == 2
+    assert _lib.lib.mnp_gemm_sub_batched_noalloc(4, 10, 10, 10, 16, 10, 0, 16, 10, 0, 16, 10, 0, 1, None) == -1
