@@ -16,6 +16,10 @@
  *     1000 + hipError_t when a launch failed.
  *   - Pivots are 0-based row indices. LU info: 0 ok, j > 0 = zero pivot in column j-1
  *     (LAPACK zgetrf convention; the reference does not detect it, utils/gpu.py:189-191).
+ *   - Geometry of a batch: entry points that take `sface` read the per-face arrays (nvec, pos,
+ *     area, side) of batch entry b at sface * b faces (times 3 doubles for vectors): 0 = one
+ *     geometry shared by every entry (a wavelength sweep), n = one geometry per entry (particles
+ *     of different shape padded to one size, mnp_pad_batched).
  *   - State: tuning knobs, grown workspaces (split-BF16 operand planes, cooperative-panel
  *     scratch), the lookahead side stream and GEMM timing records belong to a handle. Every call
  *     works through the handle bound to the calling host thread (mnp_handle_bind); a thread that
@@ -28,7 +32,7 @@
 extern "C" {
 #endif
 
-/* ABI revision of this header (2). */
+/* ABI revision of this header (3). */
 int mnp_abi_version(void);
 
 /* Library handle (opaque). The reference keeps its solver state in Python objects and runs one
@@ -134,40 +138,50 @@ int mnp_scale_diag_batched(int prec, int n, int batch, double s, const void* A, 
  * Sigma1 = H1 G1^{-1}, Sigma2 = H2 G2^{-1}, Delta^{-1}, Sigma = Sigma1 L1 - Sigma2 L2 +
  * k^2 L (Delta^{-1} o n n^T) L, LU(Sigma). In: G1, G2, H1, H2; out: G1, G2 (LU), S1 (Sigma1),
  * S2 (LU of Sigma), D (Delta^{-1}); H1, H2 destroyed. ipiv: 4 x batch x n; perm: 8 x batch x n;
- * info: 4 x batch; nvec: n x 3; L1, L2: batch x n complex (inside / outside eps per face);
- * kvac: batch (2 pi / lambda). */
+ * info: 4 x batch; nvec: n x 3 per geometry (sface); L1, L2: batch x n complex (inside /
+ * outside eps per face); kvac: batch (2 pi / lambda). */
 int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, void* H2, void* S1, void* S2, void* D,
-                      long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, const double* L1,
-                      const double* L2, const double* kvac, int nb, void* stream);
+                      long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, long sface,
+                      const double* L1, const double* L2, const double* kvac, int nb, void* stream);
+
+/* Particles of different face counts in one batch (no counterpart in the reference, which solves
+ * one particle per process): matrix b holds a particle of nact[b] <= n faces in its leading
+ * block; every entry (i, j) with i or j >= nact[b] is set to d = dre + i dim on the diagonal and
+ * 0 elsewhere. With G1, G2 padded by 1, H1 by 1, H2 by -1, and L1 = L2 = 1, zero normals and
+ * zero excitation on the padding faces, the LU chain of mnp_bemret_factor stays block diagonal
+ * and the padding rows of every solution are 0. */
+int mnp_pad_batched(int prec, int n, int batch, const int* nact, void* A, long ld, long stride, double dre, double dim,
+                    void* stream);
 
 /* BEMRet.solve (bem_ret.py:1569-1831, Eqs. 19-20): in = [phi a De alpha] (n x 8 per
  * wavelength, precision prec) -> out = [sig1 h1 sig2 h2] (n x 8); work: 24 n complex per
  * wavelength. Factors from mnp_bemret_factor. */
 int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2, const void* S1, const void* D,
-                     const void* S2, long ld, long stride, const int* perm, const double* nvec, const double* L1,
-                     const double* L2, const double* kvac, const void* in, long sIn, void* out, long sOut, void* work,
-                     long sW, void* stream);
+                     const void* S2, long ld, long stride, const int* perm, const double* nvec, long sface,
+                     const double* L1, const double* L2, const double* kvac, const void* in, long sIn, void* out,
+                     long sOut, void* work, long sW, void* stream);
 
 /* Exact complex-double BEM operator y = A x, or residual y = rhs - A x when rhs != NULL
  * (BEMRetIter._afun, bem/bem_ret_iter.py:1362-1501); norms (device, 2 x batch, may be NULL):
  * |y|^2 and |rhs|^2 per wavelength. x != y; work: 32 n complex double per wavelength. */
 int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const void* H1, const void* H2, long ld,
-                     long stride, const double* nvec, const double* L1, const double* L2, const double* kvac,
-                     const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work, long sW,
-                     double* norms, void* stream);
+                     long stride, const double* nvec, long sface, const double* L1, const double* L2,
+                     const double* kvac, const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work,
+                     long sW, double* norms, void* stream);
 
 /* Plane-wave right-hand side [phi a De alpha] (PlaneWaveRet.potential,
  * simulation/planewave_ret.py:257-361, with BEMRet._excitation, bem_ret.py:1360-1429).
  * side: n flags (bit 0 inside medium = excitation medium, bit 1 outside); kmed: batch complex;
  * pol, dir: host double[3]. */
-int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, const double* kmed,
-                      const double* kvac, const double* L1, const double* L2, const double* pol, const double* dir,
-                      void* out, long so, void* stream);
+int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, long sface,
+                      const double* kmed, const double* kvac, const double* L1, const double* L2, const double* pol,
+                      const double* dir, void* out, long so, void* stream);
 
 /* Far fields e, h in ndir directions from x = [sig1 h1 sig2 h2] (SpectrumRet._farfield,
  * spectrum/spectrum_ret.py:231-392); out: batch x ndir x 6 complex. */
-int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, const double* kmed,
-                 const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out, void* stream);
+int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, long sface,
+                 const double* kmed, const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out,
+                 void* stream);
 
 /* Arnoldi kernels of the device GMRES (BEMRetIter; the reference calls scipy.sparse.linalg.gmres,
  * bem/bem_iter.py:45-101): h[c] = <V_c, w> and w += sum_c V_c s[c] over nv <= 128 complex double

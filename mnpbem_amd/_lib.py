@@ -34,11 +34,12 @@ _SIGS = {
     "mnp_rsolve_batched": [I, I, I, I, P, L, L, P, L, P, L, L, P, L, L, I, P],
     "mnp_gemv_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, P],
     "mnp_scale_diag_batched": [I, I, I, C.c_double, P, L, P, P, L, L, P],
-    "mnp_bemret_factor": [I, I, I, P, P, P, P, P, P, P, L, L, P, P, P, P, P, P, P, I, P],
-    "mnp_bemret_solve": [I, I, I, P, P, P, P, P, L, L, P, P, P, P, P, P, L, P, L, P, L, P],
-    "mnp_bem_operator": [I, I, P, P, P, P, L, L, P, P, P, P, P, L, P, L, P, L, P, L, P, P],
-    "mnp_planewave_rhs": [I, I, P, P, P, P, P, P, P, P, P, P, L, P],
-    "mnp_farfield": [I, I, P, P, P, P, P, I, P, P, L, P, P],
+    "mnp_bemret_factor": [I, I, I, P, P, P, P, P, P, P, L, L, P, P, P, P, L, P, P, P, I, P],
+    "mnp_pad_batched": [I, I, I, P, P, L, L, C.c_double, C.c_double, P],
+    "mnp_bemret_solve": [I, I, I, P, P, P, P, P, L, L, P, P, L, P, P, P, P, L, P, L, P, L, P],
+    "mnp_bem_operator": [I, I, P, P, P, P, L, L, P, L, P, P, P, P, L, P, L, P, L, P, L, P, P],
+    "mnp_planewave_rhs": [I, I, P, P, P, L, P, P, P, P, P, P, P, L, P],
+    "mnp_farfield": [I, I, P, P, P, L, P, P, I, P, P, L, P, P],
     "mnp_mdot": [L, I, P, L, P, P, P],
     "mnp_mupdate": [L, I, P, L, P, P, P],
     "mnp_cast_batched": [I, L, I, P, L, P, L, P],

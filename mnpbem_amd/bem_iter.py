@@ -330,7 +330,7 @@ class BEMRetIter:
                  engine.stream_handle())
             return
         call("mnp_bem_operator", n, 1, e.G1.data_ptr(), e.G2.data_ptr(), e.H1.data_ptr(), e.H2.data_ptr(), e.ld,
-             e.stride, geo.nvec.data_ptr(), self._L1.data_ptr(), self._L2.data_ptr(), self._kvac.data_ptr(),
+             e.stride, geo.nvec.data_ptr(), 0, self._L1.data_ptr(), self._L2.data_ptr(), self._kvac.data_ptr(),
              x.data_ptr(), 8 * n, rhs.data_ptr() if rhs is not None else None, 8 * n, y.data_ptr(), 8 * n,
              self._opwork.data_ptr(), 32 * n, None, engine.stream_handle())
 

@@ -17,16 +17,21 @@ struct AsmArgs {
 };
 
 // Per-face data of the BEMRet chain: normals, inside/outside dielectric functions (batch x n
-// complex double) and vacuum wavenumber per wavelength.
+// complex double) and vacuum wavenumber per wavelength. sface: faces between the geometry of
+// consecutive batch entries (0: one geometry shared by the batch, n: one per entry, as for
+// particles of different shape padded to one size).
 struct Face {
     int n;
     const double* nvec;
     const double* L1;
     const double* L2;
     const double* kvac;
+    long sface;
+    __host__ __device__ const double* nv(int b) const { return nvec + (long)b * 3 * sface; }
 };
 
-// Plane-wave excitation / far-field inputs.
+// Plane-wave excitation / far-field inputs; sface as in Face (pos, nvec, side and area advance
+// by sface faces per batch entry).
 struct PW {
     int n;
     const double* pos;
@@ -37,6 +42,7 @@ struct PW {
     const double* L1;
     const double* L2;
     double pol[3], dir[3];
+    long sface;
 };
 
 }  // namespace mnp

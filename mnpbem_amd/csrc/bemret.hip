@@ -38,11 +38,12 @@ __global__ __launch_bounds__(256) void k_sub(int n, const cx<T>* A, const cx<T>*
 // Sigma (into S2) from S1, S2, Dinv
 template <typename T>
 __global__ __launch_bounds__(256) void k_sigma(int n, const cx<T>* S1, cx<T>* S2, const cx<T>* Dinv, long ld,
-                                               long stride, const double* nvec, const double* L1, const double* L2,
-                                               const double* kvac) {
+                                               long stride, const double* nvec, long sface, const double* L1,
+                                               const double* L2, const double* kvac) {
     int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, b = blockIdx.z;
     if (i >= n) return;
     long o = (long)b * stride + i + (long)j * ld;
+    nvec += (long)b * 3 * sface;
     const cx<double>* l1 = reinterpret_cast<const cx<double>*>(L1) + (long)b * n;
     const cx<double>* l2 = reinterpret_cast<const cx<double>*>(L2) + (long)b * n;
     double k = kvac[b];
@@ -221,7 +222,7 @@ __global__ void k_glue2(Face f, const cx<T>* IN, long sIn, const cx<T>* W, long 
     cx<T> ik = mk<T>(0, k);
     cx<T> dot = mk<T>(0, 0);
     for (int c = 0; c < 3; ++c) {
-        T nc = (T)f.nvec[3 * i + c];
+        T nc = (T)f.nv(b)[3 * i + c];
         cx<T> al = in[i + (long)(5 + c) * n] - w[i + (long)c * n] + (nc * ik) * l1phi;
         ap[i + (long)c * n] = al;
         dot = dot + nc * (l1 * in[i + (long)(1 + c) * n]);
@@ -239,7 +240,7 @@ __global__ void k_glue3(Face f, const cx<T>* AP, long sAP, const cx<T>* U, long 
     cx<T> L = Lval<T>(f.L1, b, n, i) - Lval<T>(f.L2, b, n, i);
     T k = (T)f.kvac[b];
     cx<T> dot = mk<T>(0, 0);
-    for (int c = 0; c < 3; ++c) dot = dot + (T)f.nvec[3 * i + c] * u[i + (long)c * n];
+    for (int c = 0; c < 3; ++c) dot = dot + (T)f.nv(b)[3 * i + c] * u[i + (long)c * n];
     R[(long)b * sR + i] = ap[i + 3L * n] + mk<T>(0, k) * (L * dot);
 }
 
@@ -252,7 +253,7 @@ __global__ void k_glue4(Face f, const cx<T>* AP, long sAP, const cx<T>* S, long 
     cx<T> L = Lval<T>(f.L1, b, n, i) - Lval<T>(f.L2, b, n, i);
     T k = (T)f.kvac[b];
     cx<T> ls = mk<T>(0, k) * (L * S[(long)b * sS + i]);
-    for (int c = 0; c < 3; ++c) Tt[(long)b * sT + i + (long)c * n] = ((T)f.nvec[3 * i + c]) * ls + ap[i + (long)c * n];
+    for (int c = 0; c < 3; ++c) Tt[(long)b * sT + i + (long)c * n] = ((T)f.nv(b)[3 * i + c]) * ls + ap[i + (long)c * n];
 }
 
 // P1 = [sig2 + phi, h2 + a], P2 = [sig2, h2]  (N x 4 each)
@@ -306,12 +307,12 @@ __global__ void k_op_finish(Face f, const cx<double>* GX1, const cx<double>* GX2
         cx<double> dotn = mk(0.0, 0.0);
         for (int c = 0; c < 3; ++c) {
             out[1 + c] = GX1[o + (1 + c) * N] - GX2[o + (1 + c) * N];
-            dotn = dotn + f.nvec[3 * i + c] * (GX1[o + (5 + c) * N] - GX2[o + (5 + c) * N]);
+            dotn = dotn + f.nv(b)[3 * i + c] * (GX1[o + (5 + c) * N] - GX2[o + (5 + c) * N]);
         }
         out[4] = HX1[oh + 4 * N] - HX2[o + 4 * N] - ik * dotn;
         cx<double> es = GX1[o + 4 * N] - GX2[o + 4 * N];
         for (int c = 0; c < 3; ++c)
-            out[5 + c] = HX1[oh + (1 + c) * N] - HX2[o + (1 + c) * N] - (f.nvec[3 * i + c] * ik) * es;
+            out[5 + c] = HX1[oh + (1 + c) * N] - HX2[o + (1 + c) * N] - (f.nv(b)[3 * i + c] * ik) * es;
         for (int c = 0; c < 8; ++c) {
             cx<double> v = out[c];
             if (rhs) {
@@ -367,18 +368,20 @@ __global__ void k_planewave(PW p, cx<double>* out, long so) {
     if (i >= n) return;
     cx<double> km = reinterpret_cast<const cx<double>*>(p.kmed)[b];
     double k0 = p.kvac[b];
-    double rd = p.pos[3 * i] * p.dir[0] + p.pos[3 * i + 1] * p.dir[1] + p.pos[3 * i + 2] * p.dir[2];
-    double nd = p.nvec[3 * i] * p.dir[0] + p.nvec[3 * i + 1] * p.dir[1] + p.nvec[3 * i + 2] * p.dir[2];
+    const double* pos = p.pos + (long)b * 3 * p.sface;
+    const double* nvec = p.nvec + (long)b * 3 * p.sface;
+    double rd = pos[3 * i] * p.dir[0] + pos[3 * i + 1] * p.dir[1] + pos[3 * i + 2] * p.dir[2];
+    double nd = nvec[3 * i] * p.dir[0] + nvec[3 * i + 1] * p.dir[1] + nvec[3 * i + 2] * p.dir[2];
     // e^{i km rd} / (i k0) = -i/k0 * e^{i km rd}
     cx<double> e = expikd(km, rd);
     cx<double> ph = mk(e.im / k0, -e.re / k0);
     cx<double> dph = mk(-km.im * nd, km.re * nd) * ph;  // i km (n.dir) ph
-    int sd = p.side[i];
+    int sd = p.side[(long)b * p.sface + i];
     double w1 = (sd & 1) ? 1.0 : 0.0, w2 = (sd & 2) ? 1.0 : 0.0;
     cx<double> e1 = Lval<double>(p.L1, b, n, i), e2 = Lval<double>(p.L2, b, n, i);
     cx<double>* o = out + (long)b * so;
     const long N = n;
-    double npol = p.nvec[3 * i] * p.pol[0] + p.nvec[3 * i + 1] * p.pol[1] + p.nvec[3 * i + 2] * p.pol[2];
+    double npol = nvec[3 * i] * p.pol[0] + nvec[3 * i + 1] * p.pol[1] + nvec[3 * i + 2] * p.pol[2];
     o[i] = mk(0.0, 0.0);
     for (int c = 0; c < 3; ++c) {
         o[i + (1 + c) * N] = ((w2 - w1) * p.pol[c]) * ph;
@@ -403,13 +406,16 @@ __global__ __launch_bounds__(256) void k_farfield(PW p, const double* area, int 
     double dx = dirs[3 * d], dy = dirs[3 * d + 1], dz = dirs[3 * d + 2];
     const cx<double>* xv = x + (long)b * sx;
     const long N = n;
+    const double* pos = p.pos + (long)b * 3 * p.sface;
+    const int* side = p.side + (long)b * p.sface;
+    area += (long)b * p.sface;
     // accumulators: S = sum ph sig, Hs = sum ph h (3)
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = t; i < n; i += 256) {
-        double rd = dx * p.pos[3 * i] + dy * p.pos[3 * i + 1] + dz * p.pos[3 * i + 2];
+        double rd = dx * pos[3 * i] + dy * pos[3 * i + 1] + dz * pos[3 * i + 2];
         cx<double> e = expikd(km, -rd);
         cx<double> ph = area[i] * e;
-        int sd = p.side[i];
+        int sd = side[i];
         for (int s = 0; s < 2; ++s) {
             if (!(sd & (1 << s))) continue;
             int base = 4 * s;
@@ -452,9 +458,40 @@ void sub_batched(int n, int batch, const void* A, const void* B, void* C, long l
 }
 template <typename T>
 void sigma_batched(int n, int batch, const void* S1, void* S2, const void* Dinv, long ld, long stride,
-                   const double* nvec, const double* L1, const double* L2, const double* kvac, hipStream_t s) {
+                   const double* nvec, long sface, const double* L1, const double* L2, const double* kvac,
+                   hipStream_t s) {
     hipLaunchKernelGGL(k_sigma<T>, dim3((n + 255) / 256, n, batch), dim3(256), 0, s, n, (const cx<T>*)S1, (cx<T>*)S2,
-                       (const cx<T>*)Dinv, ld, stride, nvec, L1, L2, kvac);
+                       (const cx<T>*)Dinv, ld, stride, nvec, sface, L1, L2, kvac);
+}
+
+__global__ void k_zero_int(long count, int* v) {
+    long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) v[i] = 0;
+}
+
+// Clears count ints with a kernel (no memset node when the caller captures the stream).
+void zero_int(long count, int* v, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_int, dim3((count + 255) / 256), dim3(256), 0, s, count, v);
+}
+
+// Padding of particles smaller than the batch's matrix size: entries (i, j) with i or j >= nact[b]
+// of matrix b become d (i == j) or 0, so the padded system is block diagonal with a constant
+// diagonal block that the LU chain carries through unchanged.
+template <typename T>
+__global__ void k_pad(int n, const int* nact, cx<T>* A, long ld, long stride, double dre, double dim) {
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, b = blockIdx.z;
+    const int na = nact[b];
+    if (i >= n || (i < na && j < na)) return;
+    A[(long)b * stride + i + (long)j * ld] = (i == j) ? mk<T>((T)dre, (T)dim) : mk<T>(0, 0);
+}
+
+void pad_batched(int prec, int n, int batch, const int* nact, void* A, long ld, long stride, double dre, double dim,
+                 hipStream_t s) {
+    dim3 g((n + 255) / 256, n, batch);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_pad<double>, g, dim3(256), 0, s, n, nact, (cx<double>*)A, ld, stride, dre, dim);
+    else
+        hipLaunchKernelGGL(k_pad<float>, g, dim3(256), 0, s, n, nact, (cx<float>*)A, ld, stride, dre, dim);
 }
 template <typename T>
 void identity_batched(int n, int batch, void* A, long ld, long stride, hipStream_t s) {
@@ -507,9 +544,9 @@ template void solve_chain<float>(const Face&, int, const void*, const void*, con
                                  void*, long, hipStream_t);
 template void sub_batched<double>(int, int, const void*, const void*, void*, long, long, hipStream_t);
 template void sub_batched<float>(int, int, const void*, const void*, void*, long, long, hipStream_t);
-template void sigma_batched<double>(int, int, const void*, void*, const void*, long, long, const double*,
+template void sigma_batched<double>(int, int, const void*, void*, const void*, long, long, const double*, long,
                                     const double*, const double*, const double*, hipStream_t);
-template void sigma_batched<float>(int, int, const void*, void*, const void*, long, long, const double*,
+template void sigma_batched<float>(int, int, const void*, void*, const void*, long, long, const double*, long,
                                    const double*, const double*, const double*, hipStream_t);
 template void identity_batched<double>(int, int, void*, long, long, hipStream_t);
 template void identity_batched<float>(int, int, void*, long, long, hipStream_t);

@@ -36,8 +36,10 @@ void gemv_batched(int, int, int, const void*, long, long, const void*, long, lon
 template <typename T>
 void sub_batched(int, int, const void*, const void*, void*, long, long, hipStream_t);
 template <typename T>
-void sigma_batched(int, int, const void*, void*, const void*, long, long, const double*, const double*, const double*,
-                   const double*, hipStream_t);
+void sigma_batched(int, int, const void*, void*, const void*, long, long, const double*, long, const double*,
+                   const double*, const double*, hipStream_t);
+void zero_int(long, int*, hipStream_t);
+void pad_batched(int, int, int, const int*, void*, long, long, double, double, hipStream_t);
 template <typename T>
 void identity_batched(int, int, void*, long, long, hipStream_t);
 template <typename T>
@@ -85,7 +87,7 @@ static int launch_status() {
 
 extern "C" {
 
-int mnp_abi_version(void) { return 2; }
+int mnp_abi_version(void) { return 3; }
 
 // Handles: knobs, workspaces and the lookahead stream of the library, per host thread.
 int mnp_handle_create(mnp_handle* h) {
@@ -200,7 +202,7 @@ int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, in
     REQUIRE(sP >= n, 8);
     REQUIRE(nb >= 1 && nb <= 64, 10);
     hipStream_t s = (hipStream_t)stream;
-    hipMemsetAsync(info, 0, sizeof(int) * batch, s);
+    zero_int(batch, info, s);
     if (prec == 0)
         getrf_batched<double>(n, batch, A, lda, sA, ipiv, sP, info, nb, s);
     else
@@ -277,14 +279,29 @@ int mnp_scale_diag_batched(int prec, int n, int batch, double s, const void* A, 
     return launch_status();
 }
 
+// Padding of matrices whose particle has fewer faces than the batch's n (see the header).
+int mnp_pad_batched(int prec, int n, int batch, const int* nact, void* A, long ld, long stride, double dre, double dim,
+                    void* stream) {
+    REQUIRE(prec == 0 || prec == 1, 1);
+    REQUIRE(n > 0, 2);
+    REQUIRE(batch >= 1 && batch <= 65535, 3);
+    REQUIRE(nact, 4);
+    REQUIRE(A, 5);
+    REQUIRE(ld >= n, 6);
+    REQUIRE(stride >= ld * (long)n || batch == 1, 7);
+    pad_batched(prec, n, batch, nact, A, ld, stride, dre, dim, (hipStream_t)stream);
+    return launch_status();
+}
+
 // Factor the BEMRet system for a batch of wavelengths. Buffers (each `batch` matrices of n x n):
 //   G1, G2 (in: Green matrices; out: LU factors), H1, H2 (in: H matrices; destroyed),
 //   S1 (out: Sigma1), S2 (out: LU of Sigma), D (out: Delta^{-1}).
 // ipiv: 4 x batch x n (G1, G2, Delta, Sigma); perm: 4 x 2 x batch x n (pf, qinv per factor);
-// info: 4 x batch. L1, L2: batch x n complex double; kvac: batch.
+// info: 4 x batch. nvec: n x 3 per geometry, sface faces apart (0: shared); L1, L2: batch x n
+// complex double; kvac: batch.
 int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, void* H2, void* S1, void* S2, void* D,
-                      long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, const double* L1,
-                      const double* L2, const double* kvac, int nb, void* stream) {
+                      long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, long sface,
+                      const double* L1, const double* L2, const double* kvac, int nb, void* stream) {
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
@@ -293,14 +310,15 @@ int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, 
     REQUIRE(stride >= ld * (long)n, 12);
     REQUIRE(ipiv && perm && info, 13);
     REQUIRE(nvec && L1 && L2 && kvac, 16);
-    REQUIRE(nb >= 1 && nb <= 64, 20);
+    REQUIRE(sface == 0 || sface >= n, 17);
+    REQUIRE(nb >= 1 && nb <= 64, 21);
     hipStream_t s = (hipStream_t)stream;
     const long sP = n, sQ = n;
     const long PB = (long)batch * n;  // ints per factor
     int *ip1 = ipiv, *ip2 = ipiv + PB, *ipD = ipiv + 2 * PB, *ipS = ipiv + 3 * PB;
     int *pf1 = perm, *q1 = perm + PB, *pf2 = perm + 2 * PB, *q2 = perm + 3 * PB;
     int *pfD = perm + 4 * PB, *qD = perm + 5 * PB, *pfS = perm + 6 * PB, *qS = perm + 7 * PB;
-    hipMemsetAsync(info, 0, sizeof(int) * 4 * batch, s);
+    zero_int(4L * batch, info, s);
     auto run = [&](auto tag) {
         using T = decltype(tag);
         getrf_batched<T>(n, batch, G1, ld, stride, ip1, sP, info, nb, s);
@@ -314,7 +332,7 @@ int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, 
         perm_batched(n, batch, ipD, sP, pfD, qD, sQ, s);
         identity_batched<T>(n, batch, H2, ld, stride, s);
         rsolve_batched<T>(n, n, batch, H1, ld, stride, qD, sQ, H2, ld, stride, D, ld, stride, nb, s);
-        sigma_batched<T>(n, batch, S1, S2, D, ld, stride, nvec, L1, L2, kvac, s);
+        sigma_batched<T>(n, batch, S1, S2, D, ld, stride, nvec, sface, L1, L2, kvac, s);
         getrf_batched<T>(n, batch, S2, ld, stride, ipS, sP, info + 3 * batch, nb, s);
         perm_batched(n, batch, ipS, sP, pfS, qS, sQ, s);
     };
@@ -329,20 +347,21 @@ int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, 
 // sig1 h1 sig2 h2), with the factors of mnp_bemret_factor. IN/OUT/work are of the factor's
 // precision; work holds 24 n complex values per wavelength.
 int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2, const void* S1, const void* D,
-                     const void* S2, long ld, long stride, const int* perm, const double* nvec, const double* L1,
-                     const double* L2, const double* kvac, const void* in, long sIn, void* out, long sOut, void* work,
-                     long sW, void* stream) {
+                     const void* S2, long ld, long stride, const int* perm, const double* nvec, long sface,
+                     const double* L1, const double* L2, const double* kvac, const void* in, long sIn, void* out,
+                     long sOut, void* work, long sW, void* stream) {
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
     REQUIRE(G1 && G2 && S1 && D && S2, 4);
     REQUIRE(perm, 11);
     REQUIRE(nvec && L1 && L2 && kvac, 12);
-    REQUIRE(in && out && work, 16);
-    REQUIRE(sIn >= 8L * n && sOut >= 8L * n, 17);
-    REQUIRE(sW >= 24L * n, 21);
+    REQUIRE(sface == 0 || sface >= n, 13);
+    REQUIRE(in && out && work, 17);
+    REQUIRE(sIn >= 8L * n && sOut >= 8L * n, 18);
+    REQUIRE(sW >= 24L * n, 22);
     const long PB = (long)batch * n;
-    Face f{n, nvec, L1, L2, kvac};
+    Face f{n, nvec, L1, L2, kvac, sface};
     hipStream_t s = (hipStream_t)stream;
     if (prec == 0)
         solve_chain<double>(f, batch, G1, G2, S1, D, S2, ld, stride, perm, perm + 2 * PB, perm + 6 * PB, n, in, sIn,
@@ -357,17 +376,18 @@ int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2,
 // from G1, G2, H1, H2. x, rhs, y: n x 8 per wavelength; work: 32 n per wavelength. When
 // `norms` is given, norms[2b] = |y_b|^2 and norms[2b+1] = |rhs_b|^2 (device, double).
 int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const void* H1, const void* H2, long ld,
-                     long stride, const double* nvec, const double* L1, const double* L2, const double* kvac,
-                     const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work, long sW,
-                     double* norms, void* stream) {
+                     long stride, const double* nvec, long sface, const double* L1, const double* L2,
+                     const double* kvac, const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work,
+                     long sW, double* norms, void* stream) {
     REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1, 2);
     REQUIRE(G1 && G2 && H1 && H2, 3);
     REQUIRE(nvec && L1 && L2 && kvac, 9);
-    REQUIRE(x && y && work, 13);
-    REQUIRE(x != y, 17);
-    REQUIRE(sW >= 32L * n, 20);
-    Face f{n, nvec, L1, L2, kvac};
+    REQUIRE(sface == 0 || sface >= n, 10);
+    REQUIRE(x && y && work, 14);
+    REQUIRE(x != y, 18);
+    REQUIRE(sW >= 32L * n, 21);
+    Face f{n, nvec, L1, L2, kvac, sface};
     bem_operator(f, batch, G1, G2, H1, H2, ld, stride, x, sx, rhs, sr, y, sy, work, sW, norms, (hipStream_t)stream);
     return launch_status();
 }
@@ -375,29 +395,32 @@ int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const voi
 // Packed plane-wave right-hand side (n x 8 complex double per wavelength). side[i]: bit 0 =
 // inside medium of face i is the excitation medium, bit 1 = outside medium is. kmed: complex
 // wavenumber in that medium per wavelength; pol, dir: host arrays of 3 doubles.
-int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, const double* kmed,
-                      const double* kvac, const double* L1, const double* L2, const double* pol, const double* dir,
-                      void* out, long so, void* stream) {
+int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, long sface,
+                      const double* kmed, const double* kvac, const double* L1, const double* L2, const double* pol,
+                      const double* dir, void* out, long so, void* stream) {
     REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1, 2);
     REQUIRE(pos && nvec && side && kmed && kvac && L1 && L2, 3);
-    REQUIRE(pol && dir, 10);
-    REQUIRE(out, 12);
-    REQUIRE(so >= 8L * n, 13);
-    PW p{n, pos, nvec, side, kmed, kvac, L1, L2, {pol[0], pol[1], pol[2]}, {dir[0], dir[1], dir[2]}};
+    REQUIRE(sface == 0 || sface >= n, 6);
+    REQUIRE(pol && dir, 11);
+    REQUIRE(out, 13);
+    REQUIRE(so >= 8L * n, 14);
+    PW p{n, pos, nvec, side, kmed, kvac, L1, L2, {pol[0], pol[1], pol[2]}, {dir[0], dir[1], dir[2]}, sface};
     planewave_rhs(p, batch, out, so, (hipStream_t)stream);
     return launch_status();
 }
 
 // Far fields for ndir directions: out[(b * ndir + d) * 6 + c] = e_c (c < 3) and h_{c-3}.
-int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, const double* kmed,
-                 const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out, void* stream) {
+int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, long sface,
+                 const double* kmed, const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out,
+                 void* stream) {
     REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1 && batch <= 65535, 2);
     REQUIRE(pos && area && side && kmed && kvac, 3);
-    REQUIRE(ndir >= 1, 8);
-    REQUIRE(dirs && x && out, 9);
-    PW p{n, pos, nullptr, side, kmed, kvac, nullptr, nullptr, {0, 0, 0}, {0, 0, 0}};
+    REQUIRE(sface == 0 || sface >= n, 6);
+    REQUIRE(ndir >= 1, 9);
+    REQUIRE(dirs && x && out, 10);
+    PW p{n, pos, nullptr, side, kmed, kvac, nullptr, nullptr, {0, 0, 0}, {0, 0, 0}, sface};
     farfield(p, area, ndir, dirs, batch, x, sx, out, (hipStream_t)stream);
     return launch_status();
 }

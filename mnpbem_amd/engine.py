@@ -68,6 +68,7 @@ class DeviceGeometry:
         self.dev = dev or device()
         self.cp = cp
         self.n = cp.nfaces
+        self.sface = 0          # one geometry for every matrix of a batch
         d = self.dev
         f64 = torch.float64
         self.pos = torch.tensor(np.ascontiguousarray(cp.pos), dtype=f64, device=d)
@@ -205,12 +206,16 @@ class RetSolverBatch:
         if self.exact is not None:
             self.exact.assemble(self.kmat, nl)
         self.fac.assemble(self.kmat, nl)
-        f = self.fac
+        return self._factor_assembled()
+
+    def _factor_assembled(self):
+        """LU chain of the assembled G1, G2, H1, H2 of the first self.nlam matrices."""
+        g, f, nl = self.geo, self.fac, self.nlam
         from .autotune import knobs
         with knobs(self.tuning):
             call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
                  _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info), _p(g.nvec),
-                 _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
+                 g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
         info = self.check_info()
         if np.any(info < 0):
             raise RuntimeError("LU panel synchronisation timed out (info {}); factors are invalid".format(info))
@@ -231,14 +236,15 @@ class RetSolverBatch:
         """out = M rhs with the factor precision (rhs, out: (B, 8, n) of that dtype)."""
         g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
         call("mnp_bemret_solve", self.prec, n, nl, _p(f.G1), _p(f.G2), _p(self.S1), _p(self.D), _p(self.S2),
-             f.ld, f.stride, _p(self.perm), _p(g.nvec), _p(self.L1), _p(self.L2), _p(self.kvac), _p(rhs), 8 * n,
+             f.ld, f.stride, _p(self.perm), _p(g.nvec), g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), _p(rhs),
+             8 * n,
              _p(out), 8 * n, _p(self.work), 24 * n, stream_handle())
 
     def residual(self, x, rhs, out, norms=None):
         """out = rhs - A x with the exact complex128 operator; norms (B, 2) = |r|^2, |rhs|^2."""
         e, g, n, nl = self.exact, self.geo, self.geo.n, self.nlam
         call("mnp_bem_operator", n, nl, _p(e.G1), _p(e.G2), _p(e.H1), _p(e.H2), e.ld, e.stride, _p(g.nvec),
-             _p(self.L1), _p(self.L2), _p(self.kvac), _p(x), 8 * n, _p(rhs), 8 * n, _p(out), 8 * n,
+             g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), _p(x), 8 * n, _p(rhs), 8 * n, _p(out), 8 * n,
              _p(self.opwork), 32 * n, _p(norms), stream_handle())
 
     def solve(self, rhs):
@@ -295,7 +301,8 @@ def planewave_rhs(geo, kvac, kmed, L1, L2, pol, direc, medium, nlam, batch):
     out = torch.empty((batch, 8, n), dtype=C128, device=geo.dev)
     pol = np.ascontiguousarray(np.asarray(pol, float).ravel())
     direc = np.ascontiguousarray(np.asarray(direc, float).ravel())
-    call("mnp_planewave_rhs", n, nlam, _p(geo.pos), _p(geo.nvec), _p(geo.side(medium)), _p(kmed), _p(kvac),
+    call("mnp_planewave_rhs", n, nlam, _p(geo.pos), _p(geo.nvec), _p(geo.side(medium)), geo.sface, _p(kmed),
+         _p(kvac),
          _p(L1), _p(L2), pol.ctypes.data, direc.ctypes.data, _p(out), 8 * n, stream_handle())
     return out
 
@@ -306,6 +313,7 @@ def farfield(geo, x, kvac, kmed, dirs, medium, nlam):
     dirs_t = torch.tensor(np.ascontiguousarray(np.atleast_2d(dirs), dtype=float), device=geo.dev)
     ndir = dirs_t.shape[0]
     out = torch.empty((nlam, ndir, 6), dtype=C128, device=geo.dev)
-    call("mnp_farfield", n, nlam, _p(geo.pos), _p(geo.area), _p(geo.side(medium)), _p(kmed), _p(kvac), ndir,
+    call("mnp_farfield", n, nlam, _p(geo.pos), _p(geo.area), _p(geo.side(medium)), geo.sface, _p(kmed), _p(kvac),
+         ndir,
          _p(dirs_t), _p(x), 8 * n, _p(out), stream_handle())
     return out

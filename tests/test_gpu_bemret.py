@@ -226,10 +226,10 @@ def test_bem_operator_matches_oracle_afun(sphere, sphere_oracle):
     work = torch.empty((2, 32, n), dtype=torch.complex128, device=geo.dev)
     norms = torch.zeros((2, 2), dtype=torch.float64, device=geo.dev)
     p = lambda t: t.data_ptr()
-    _lib.call("mnp_bem_operator", n, 2, p(gb.G1), p(gb.G2), p(gb.H1), p(gb.H2), gb.ld, gb.stride, p(geo.nvec),
+    _lib.call("mnp_bem_operator", n, 2, p(gb.G1), p(gb.G2), p(gb.H1), p(gb.H2), gb.ld, gb.stride, p(geo.nvec), 0,
               p(L1), p(L2), p(kvac), p(xd), 8 * n, None, 8 * n, p(y), 8 * n, p(work), 32 * n, None, stream_handle())
     yr = torch.empty_like(xd)
-    _lib.call("mnp_bem_operator", n, 2, p(gb.G1), p(gb.G2), p(gb.H1), p(gb.H2), gb.ld, gb.stride, p(geo.nvec),
+    _lib.call("mnp_bem_operator", n, 2, p(gb.G1), p(gb.G2), p(gb.H1), p(gb.H2), gb.ld, gb.stride, p(geo.nvec), 0,
               p(L1), p(L2), p(kvac), p(xd), 8 * n, p(rd), 8 * n, p(yr), 8 * n, p(work), 32 * n, p(norms),
               stream_handle())
     y, yr, nr = y.cpu().numpy(), yr.cpu().numpy(), norms.cpu().numpy()

@@ -88,7 +88,7 @@ def test_library_exports_header():
         assert hasattr(lib, s), s
     from mnpbem_amd import _lib
     assert sorted(_lib.exported_symbols()) == syms
-    assert lib.mnp_abi_version() == 2
+    assert lib.mnp_abi_version() == 3
 
 
 def test_handles_keep_their_own_knobs_without_gpu_work():
@@ -130,7 +130,7 @@ def test_cabi_rejects_bad_arguments_without_gpu_work():
     with pytest.raises(ValueError):
         _lib.call("mnp_getrs_batched", 0, 10, 9, 1, 1, 10, 100, 1, 10, 2, 10, 0, 3, 10, 0, None)  # r > 8
     with pytest.raises(ValueError):
-        _lib.call("mnp_bemret_factor", 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 32, None)
+        _lib.call("mnp_bemret_factor", 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 1, 1, 1, 32, None)
     assert _lib.lib.mnp_rsolve_batched(0, 4, 4, 1, 1, 4, 16, 1, 4, 5, 4, 16, 5, 4, 16, 32, None) == -13  # B == X
 
 
