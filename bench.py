@@ -300,7 +300,8 @@ def batched64(compare_unbatched=True):
     plane wave along z), mixed precision: the particles are sorted by face count and cut into
     groups (padding at most 25 % of the LU work, matrices within the memory budget); each group
     is one batch of (particle, wavelength) matrices through assembly, LU chain, refinement and far
-    field. solves/s over all 2048 solves; per-solve latency = the wall time of the group that
+    field (launched eagerly; the same groups captured as HIP graphs, mnpbem_amd.batch.GraphedGroup,
+    are timed beside it). solves/s over all 2048 solves; per-solve latency = the wall time of the group that
     holds the solve (all its solves complete together), p50/p99 over the 2048 solves; completion
     = time from the start of the set to the group's results. Host geometry set-up and a first
     (untimed) pass that runs the LU autotuner once per group shape are excluded. With
@@ -323,12 +324,20 @@ def batched64(compare_unbatched=True):
     MB.solve_particles(parts, lams, pol, direc, precision="mixed", greens=greens)
     torch.cuda.synchronize()
     t_tune = time.perf_counter() - t_tune
+    # the groups launched eagerly (the timed result)
     timings = []
     torch.cuda.synchronize()
     t_all = time.perf_counter()
     out = MB.solve_particles(parts, lams, pol, direc, precision="mixed", greens=greens, timings=timings)
     torch.cuda.synchronize()
     total = time.perf_counter() - t_all
+    # the same groups captured as HIP graphs: each group's graphs are captured, instantiated and
+    # replayed once, so this times capture + instantiation against the launches it saves
+    torch.cuda.synchronize()
+    t_g = time.perf_counter()
+    out_g = MB.solve_particles(parts, lams, pol, direc, precision="mixed", greens=greens, graphs=True)
+    torch.cuda.synchronize()
+    t_g = time.perf_counter() - t_g
     lat, comp, elapsed = [], [], 0.0
     for grp, dt in timings:
         elapsed += dt
@@ -345,7 +354,9 @@ def batched64(compare_unbatched=True):
                completion_ms_p99=round(1e3 * float(np.percentile(comp, 99)), 3),
                path_tflops=round(34.67 * len(lams) * sum(float(f) ** 3 for f in faces) / total / 1e12, 3),
                fp64_fallbacks=len(out["fallback"]), host_setup_s=round(setup, 3),
-               first_pass_with_autotune_s=round(t_tune, 3))
+               first_pass_with_autotune_s=round(t_tune, 3),
+               graphs_solves_per_s=round(len(parts) * len(lams) / t_g, 3),
+               ext_rel_diff_graphs_vs_eager=float(np.max(np.abs(out_g["ext"] - out["ext"]) / np.abs(out["ext"]))))
     if compare_unbatched:
         exc = M.PlaneWaveRet(pol, direc)
         for cp, g in zip(parts, greens):

@@ -109,6 +109,13 @@ int mnp_noalloc_scratch_bytes(long* bytes);
 int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                       void* stream);
 
+/* Grow the calling thread's workspaces for LUs and right-side solves of `batch` n x n matrices
+ * on `stream` (panel width nb, current knobs) so that mnp_getrf_batched, mnp_rsolve_batched and
+ * mnp_bemret_factor of that size allocate nothing afterwards, e.g. while the stream is captured
+ * into a HIP graph. Returns 4 when that is not possible (n at or above the cooperative-panel
+ * threshold, or an allocation failed). No counterpart in the reference. */
+int mnp_lu_reserve(int n, int batch, int nb, void* stream);
+
 /* Row permutation pf (P b = b[pf]) and its inverse from the pivots of mnp_getrf_batched
  * (any n). Part of lu_solve_dispatch (utils/gpu.py:194-251). */
 int mnp_perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* qinv, long sQ, void* stream);

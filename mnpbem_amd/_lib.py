@@ -28,6 +28,7 @@ _SIGS = {
     "mnp_gemm_sub_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
     "mnp_gemm_sub_batched_noalloc": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
     "mnp_noalloc_scratch_bytes": [P],
+    "mnp_lu_reserve": [I, I, I, P],
     "mnp_getrf_batched": [I, I, I, P, L, L, P, L, P, I, P],
     "mnp_perm_batched": [I, I, P, L, P, P, L, P],
     "mnp_getrs_batched": [I, I, I, I, P, L, L, P, L, P, L, L, P, L, L, P],
@@ -103,7 +104,8 @@ atexit.register(lib.mnp_release)
 
 
 _STATUS = {2: "the operand planes of this shape exceed the static scratch (see mnp_noalloc_scratch_bytes)",
-           3: "the static scratch's device address is unavailable"}
+           3: "the static scratch's device address is unavailable",
+           4: "workspaces cannot be reserved for this size (cooperative panels or allocation failure)"}
 
 
 def call(name, *args):

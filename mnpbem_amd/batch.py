@@ -109,12 +109,16 @@ class GroupSolver(engine.RetSolverBatch):
         self.kmats = kmats
         self.kmat_np = np.concatenate([k.cpu().numpy() for k in kmats], axis=0)     # (B, nmat)
         self.eps_np = np.concatenate(epss, axis=0)
-        self.L1 = torch.tensor(L1, dtype=C128, device=d)
-        self.L2 = torch.tensor(L2, dtype=C128, device=d)
-        self.kvac = torch.tensor(kvac, dtype=torch.float64, device=d)
-        self.kmat = torch.tensor(self.kmat_np, dtype=C128, device=d)
+        # written in place once they exist: a captured graph keeps reading the same buffers
+        for name, val, dt in (("L1", L1, C128), ("L2", L2, C128), ("kvac", kvac, torch.float64),
+                              ("kmat", self.kmat_np, C128)):
+            cur = getattr(self, name, None)
+            if isinstance(cur, torch.Tensor) and tuple(cur.shape) == val.shape and cur.dtype == dt:
+                cur.copy_(torch.from_numpy(np.ascontiguousarray(val)).to(dt))
+            else:
+                setattr(self, name, torch.tensor(np.ascontiguousarray(val), dtype=dt, device=d))
 
-    def _assemble(self, gb):
+    def _assemble(self, gb, pads=True):
         g, nl = self.geo, len(self.enei)
         es = gb.G1.element_size()
         for p, pg in enumerate(g.geos):
@@ -124,6 +128,8 @@ class GroupSolver(engine.RetSolverBatch):
                  _p(pg.ref_col), _p(pg.ref_g), _p(pg.ref_f), gb.prec, _p(gb.G1) + off, _p(gb.G2) + off,
                  _p(gb.H1) + off, _p(gb.H2) + off, gb.ld, gb.stride, stream_handle())
         for t, d in ((gb.G1, 1.0), (gb.G2, 1.0), (gb.H1, 1.0), (gb.H2, -1.0)):
+            if not pads:
+                break
             call("mnp_pad_batched", gb.prec, g.n, g.batch, _p(g.nact), _p(t), gb.ld, gb.stride, d, 0.0,
                  stream_handle())
 
@@ -142,12 +148,126 @@ class GroupSolver(engine.RetSolverBatch):
         return self._factor_assembled()
 
 
+def _planewave_into(out, geo, kvac, kmed, L1, L2, pol, direc, medium, batch):
+    n = geo.n
+    pol = np.ascontiguousarray(np.asarray(pol, float).ravel())
+    direc = np.ascontiguousarray(np.asarray(direc, float).ravel())
+    call("mnp_planewave_rhs", n, batch, _p(geo.pos), _p(geo.nvec), _p(geo.side(medium)), geo.sface, _p(kmed),
+         _p(kvac), _p(L1), _p(L2), pol.ctypes.data, direc.ctypes.data, _p(out), 8 * n, stream_handle())
+
+
+class GraphedGroup:
+    """The group's pipeline as two captured HIP graphs on one stream: F = padding, LU chain,
+    plane-wave right-hand side, the first solve with the complex64 chain and its FP64 residual;
+    R = one refinement step (residual cast, chain solve, FP64 update, new residual). The host
+    replays F once and R until every entry's residual is at refine_tol or stops halving (those
+    entries are redone with the complex128 chain, as the eager path does). Assembly (one launch
+    set per particle) runs eagerly before F; workspaces are reserved before capture so nothing is
+    allocated while the stream is captured, and no memset enters the graphs (flags, counters and
+    norms are cleared by kernels)."""
+
+    def __init__(self, solver, enei_list, pol, direc, medium):
+        s, g = solver, solver.geo
+        if s.precision != "mixed":
+            raise ValueError("graph capture covers the mixed-precision pipeline")
+        self.s, self.medium = s, medium
+        B, n = g.batch, g.n
+        d = g.dev
+        self.stream = torch.cuda.Stream(device=d)
+        self.rhs = torch.empty((B, 8, n), dtype=C128, device=d)
+        self.r32 = torch.empty((B, 8, n), dtype=C64, device=d)
+        self.dx = torch.empty_like(self.r32)
+        self.x = torch.empty((B, 8, n), dtype=C128, device=d)
+        self.r = torch.empty_like(self.x)
+        self.norms = torch.zeros((B, 2), dtype=torch.float64, device=d)
+        self.pol, self.direc = pol, direc
+        s.enei = np.atleast_1d(np.asarray(enei_list, float))
+        s.nlam = B
+        s._media(s.enei)
+        self.kmed = s.kmat[:, medium - 1].contiguous()
+        g.side(medium)             # made before capture (a host-to-device copy)
+        with torch.cuda.stream(self.stream):
+            from .autotune import knobs
+            with knobs(s.tuning):
+                call("mnp_lu_reserve", n, B, s.nb, stream_handle())
+        self.stream.synchronize()
+        self.F = torch.cuda.CUDAGraph()
+        self.R = torch.cuda.CUDAGraph()
+        with knobs(s.tuning):
+            with torch.cuda.graph(self.F, stream=self.stream):
+                self._first()
+            with torch.cuda.graph(self.R, stream=self.stream):
+                self._step()
+
+    def _pads(self):
+        s, g = self.s, self.s.geo
+        for gb in (s.exact, s.fac):
+            for t, dval in ((gb.G1, 1.0), (gb.G2, 1.0), (gb.H1, 1.0), (gb.H2, -1.0)):
+                call("mnp_pad_batched", gb.prec, g.n, g.batch, _p(g.nact), _p(t), gb.ld, gb.stride, dval, 0.0,
+                     stream_handle())
+
+    def _factor_call(self):
+        s, g, f = self.s, self.s.geo, self.s.fac
+        call("mnp_bemret_factor", s.prec, g.n, g.batch, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(s.S1),
+             _p(s.S2), _p(s.D), f.ld, f.stride, _p(s.ipiv), _p(s.perm), _p(s.info), _p(g.nvec), g.sface,
+             _p(s.L1), _p(s.L2), _p(s.kvac), s.nb, stream_handle())
+
+    def _first(self):
+        s, g = self.s, self.s.geo
+        n, B = g.n, g.batch
+        self._pads()
+        self._factor_call()
+        _planewave_into(self.rhs, g, s.kvac, self.kmed, s.L1, s.L2, self.pol, self.direc, self.medium, B)
+        call("mnp_cast_batched", 1, 8 * n, B, _p(self.rhs), 8 * n, _p(self.r32), 8 * n, stream_handle())
+        s._chain(self.r32, self.dx)
+        self.x.copy_(self.dx)      # x = first correction, widened to complex128 (an elementwise kernel)
+        s.residual(self.x, self.rhs, self.r, self.norms)
+
+    def _step(self):
+        s, g = self.s, self.s.geo
+        n, B = g.n, g.batch
+        call("mnp_cast_batched", 1, 8 * n, B, _p(self.r), 8 * n, _p(self.r32), 8 * n, stream_handle())
+        s._chain(self.r32, self.dx)
+        call("mnp_accum_batched", 1, 8 * n, B, _p(self.dx), 8 * n, _p(self.x), 8 * n, None, stream_handle())
+        s.residual(self.x, self.rhs, self.r, self.norms)
+
+    def run(self, assemble=True):
+        """Assembly (eager) + F + R until converged; returns the solution tensor and statistics."""
+        s = self.s
+        with torch.cuda.stream(self.stream):
+            if assemble:
+                s._assemble(s.exact, pads=False)
+                s._assemble(s.fac, pads=False)
+            self.F.replay()
+            info = s.check_info()
+            if np.any(info < 0):
+                raise RuntimeError("LU panel synchronisation timed out (info {})".format(info))
+            hist, prev = [], None
+            stalled = np.zeros(s.geo.batch, bool)
+            for it in range(s.max_refine + 1):
+                nr = self.norms.cpu().numpy()
+                rel = np.sqrt(nr[:, 0] / np.maximum(nr[:, 1], 1e-300))
+                hist.append(rel)
+                done = rel <= s.refine_tol
+                if prev is not None:
+                    stalled |= (~done) & (rel > 0.5 * prev)
+                if np.all(done | stalled) or it == s.max_refine:
+                    break
+                prev = rel
+                self.R.replay()
+        torch.cuda.current_stream().wait_stream(self.stream)
+        s.stats = dict(relres=hist[-1], history=np.array(hist), iterations=len(hist) - 1,
+                       fallback=np.nonzero(~(hist[-1] <= s.refine_tol))[0])
+        return self.x
+
+
 def solve_particles(cps, enei_list, pol, direc, precision="mixed", greens=None, medium=1, max_waste=0.25,
-                    mem_bytes=120e9, autotune=True, timings=None):
+                    mem_bytes=120e9, autotune=True, timings=None, graphs=False):
     """Extinction of a plane wave (polarisation pol, direction direc) for every particle of
     ``cps`` at every wavelength of ``enei_list``: (len(cps), nlam) array. Particles are grouped by
     ``plan_groups`` and each group is solved as one batch. ``timings`` (list) receives, per group,
-    (particle indices, seconds from the group's start to its results)."""
+    (particle indices, seconds from the group's start to its results). ``graphs=True`` (mixed
+    precision) runs each group's pipeline as captured HIP graphs (GraphedGroup)."""
     import time
     enei_list = np.atleast_1d(np.asarray(enei_list, float))
     nl = len(enei_list)
@@ -166,10 +286,15 @@ def solve_particles(cps, enei_list, pol, direc, precision="mixed", greens=None, 
             fdt = C128 if precision == "double" else C64
             tuning = AT.tune_lu(n, min(gg.batch, 32), fdt, gg.dev, gemm_split=tuning["gemm_split"])
         s = GroupSolver(gg, precision, tuning.get("nb", 32), tuning=tuning)
-        s.factor(enei_list)
-        kmed = s.kmat[:, medium - 1].contiguous()
-        rhs = engine.planewave_rhs(gg, s.kvac, kmed, s.L1, s.L2, pol, direc, medium, gg.batch, gg.batch)
-        x = s.solve(rhs)
+        if graphs:
+            gr = GraphedGroup(s, enei_list, pol, direc, medium)
+            x = gr.run()
+            kmed, rhs = gr.kmed, gr.rhs
+        else:
+            s.factor(enei_list)
+            kmed = s.kmat[:, medium - 1].contiguous()
+            rhs = engine.planewave_rhs(gg, s.kvac, kmed, s.L1, s.L2, pol, direc, medium, gg.batch, gg.batch)
+            x = s.solve(rhs)
         if precision == "mixed" and len(s.stats["fallback"]):
             fb = s.stats["fallback"]
             fallback.extend(int(b) for b in fb)

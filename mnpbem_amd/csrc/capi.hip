@@ -24,6 +24,7 @@ void getrf_batched(int, int, void*, long, long, int*, long, int*, int, hipStream
 int gemm_sub_noalloc(int, int, int, int, const void*, long, long, const void*, long, long, void*, long, long, int,
                      hipStream_t);
 size_t noalloc_scratch_bytes();
+bool lu_reserve(int, int, int, hipStream_t);
 void perm_batched(int, int, const int*, long, int*, int*, long, hipStream_t);
 template <typename T>
 void rsolve_batched(int, int, int, const void*, long, long, const int*, long, void*, long, long, void*, long, long, int,
@@ -183,6 +184,13 @@ int mnp_gemm_sub_batched_noalloc(int terms, int m, int n, int k, const void* A, 
     REQUIRE(batch >= 1, 14);
     const int st = gemm_sub_noalloc(terms, m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, (hipStream_t)stream);
     return st ? st : launch_status();
+}
+
+int mnp_lu_reserve(int n, int batch, int nb, void* stream) {
+    REQUIRE(n > 0, 1);
+    REQUIRE(batch >= 1, 2);
+    REQUIRE(nb >= 1 && nb <= 64, 3);
+    return lu_reserve(n, batch, nb, (hipStream_t)stream) ? launch_status() : 4;
 }
 
 int mnp_noalloc_scratch_bytes(long* bytes) {

@@ -608,6 +608,18 @@ template void gemm_sub<double>(int, int, int, const void*, long, long, const voi
 template void gemm_sub<float>(int, int, int, const void*, long, long, const void*, long, long, void*, long, long,
                               int, hipStream_t);
 
+// Grows the handle's plane workspace of stream s (and of its lookahead side stream) to what a
+// complex-float launch of m x n x k for `batch` matrices needs under the handle's mode, so that
+// later launches of that size or smaller allocate nothing (work captured into a graph). False
+// when the allocation fails.
+bool gemm_reserve(int m, int n, int k, int batch, hipStream_t s, hipStream_t side) {
+    const int mode = ctx().gemm_split;
+    if (!plane_launch(mode, m, n)) return true;
+    const size_t bytes = plane_bytes(mode % 10, m, n, k, batch);
+    if (!plane_ws(s, bytes)) return false;
+    return side ? plane_ws(side, bytes) != nullptr : true;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Entry without allocation: the operand planes of the split-BF16 GEMM live in a static device
 // array. In host code `scratch` names a host-side stand-in; its device address is taken once per

@@ -688,6 +688,20 @@ static SideStream side_stream() {
     return SideStream{h.side, h.side_ready, h.side_done};
 }
 
+bool gemm_reserve(int m, int n, int k, int batch, hipStream_t s, hipStream_t side);
+
+// Workspaces the blocked LU and right-side solves of n x n matrices need on stream s: the plane
+// workspaces of the widest trailing update (for s and the lookahead side stream). False when an
+// allocation fails or the panels would need the cooperative scratch (n at or above its
+// threshold, which is sized per call).
+bool lu_reserve(int n, int batch, int nb, hipStream_t s) {
+    Ctx& h = ctx();
+    if (n >= h.coop_min_rows) return false;
+    const int NB = (h.nbo > nb) ? (h.nbo / nb) * nb : nb;
+    const int k = NB < n ? NB : n;
+    return gemm_reserve(n, n, k, batch, s, side_stream().st);
+}
+
 template <typename T>
 void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                    hipStream_t s) {

@@ -71,3 +71,32 @@ def test_group_matches_oracle():
         ref = O.solve_spectrum(s, cp.face_particle, cp.inout, funcs, lams, np.array([1.0, 0, 0]),
                                np.array([0, 0, 1.0]))["ext"]
         assert np.max(np.abs(out["ext"][i] - ref) / np.abs(ref)) < 1e-8
+
+
+def test_graphed_group_equals_eager_and_replays_consistently():
+    """The captured-graph pipeline of a group against the eager one, and two runs of the same
+    graphs with eager work in between give the same result (every replay checked)."""
+    cps = _particles()
+    lams = np.linspace(450, 900, 4)
+    pol, direc = [1.0, 0, 0], [0, 0, 1.0]
+    banner("graphed particle group")
+    eager = MB.solve_particles(cps, lams, pol, direc, precision="mixed", max_waste=100.0, autotune=False)
+    graphed = MB.solve_particles(cps, lams, pol, direc, precision="mixed", max_waste=100.0, autotune=False,
+                                 graphs=True)
+    assert not graphed["fallback"]
+    assert np.max(np.abs(graphed["ext"] - eager["ext"]) / np.abs(eager["ext"])) < 1e-10
+    greens = [M.CompGreenRet(cp, cp) for cp in cps]
+    geos = [g.geo for g in greens]
+    n = max(cp.nfaces for cp in cps)
+    gg = MB.GroupGeometry(geos, len(lams), n, geos[0].dev)
+    s = MB.GroupSolver(gg, "mixed", 32, tuning=dict(gemm_split=3))
+    gr = MB.GraphedGroup(s, lams, pol, direc, 1)
+    x1 = gr.run().clone()
+    it1 = s.stats["iterations"]
+    # eager work on the same buffers between the replays
+    s.factor(lams)
+    _ = s.solve(gr.rhs)
+    torch.cuda.synchronize()
+    x2 = gr.run().clone()
+    assert s.stats["iterations"] == it1
+    assert torch.equal(x1, x2)
