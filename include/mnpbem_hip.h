@@ -103,7 +103,9 @@ int mnp_gemm_sub_batched_noalloc(int terms, int m, int n, int k, const void* A, 
 /* Size in bytes of the static scratch behind mnp_gemm_sub_batched_noalloc (host long). */
 int mnp_noalloc_scratch_bytes(long* bytes);
 
-/* In-place LU with partial pivoting of `batch` n x n matrices (panel width nb, 1..64).
+/* In-place LU with partial pivoting of `batch` n x n matrices (panel width nb, 1..64); panels
+ * of at least knob 3's height choose their pivots by tournament (knob 9), so for n at or above
+ * it the pivots may differ from LAPACK's (P A = L U holds all the same).
  * info[b]: 0 ok, j > 0 zero pivot in column j-1, -1 a multi-workgroup panel's barrier timed out
  * (factors invalid). Replaces lu_factor_dispatch (utils/gpu.py:145-191: scipy zgetrf / cupyx getrf). */
 int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
@@ -351,11 +353,18 @@ int mnp_probe_mfma_bf16(float* out, void* stream);
  * reference). key 0: outer block width of the blocked LU and right-side solves (16..1024
  * columns, default 128); 1: lookahead (trailing update split over two streams) on/off; 2:
  * register-resident panel for complex float on/off; 3: panel height from which the
- * multi-workgroup panel is used (>= 256); 4: complex-float GEMM on the FP32 matrix cores (0,
+ * multi-workgroup panel is used (>= 256, default 4096); 4: complex-float GEMM on the FP32 matrix cores (0,
  * default) or on the BF16 matrix cores with operands split once into planes of 2 or 3 bfloat16
  * terms (12, 13; 2 terms keep about 16 significant bits, 3 about 24, as FP32); 5: workgroup
  * tile of the plane GEMM (11: 64 x 64, the only one); 6: row tiles per group of the plane
- * GEMM's XCD-aware tile order (0: plain grid order). mnpbem_amd.autotune picks them per problem shape by timing. */
+ * GEMM's XCD-aware tile order (0: plain grid order); 7: rows per workgroup of the Green-function
+ * assembly (64, 128, 256; the workgroup covers 256 / rows columns); 8: wavelengths per assembly
+ * thread (0: all of the launch; shorter chunks put more workgroups on small matrices); 9:
+ * multi-workgroup panels by tournament pivoting (1, default: each panel's pivot rows chosen by
+ * a reduction tree of partial-pivoting eliminations over blocks of 512 (complex float) or 256
+ * (complex double) rows, no barrier between workgroups) or by partial pivoting with a
+ * cooperative launch (0: LAPACK's pivots, one grid barrier per column).
+ * mnpbem_amd.autotune picks them per problem shape by timing. */
 int mnp_set_tuning(int key, int value);
 int mnp_get_tuning(int key, int* value);
 

@@ -17,7 +17,7 @@ import torch
 from ._lib import call
 
 _CACHE = {}
-_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4}
+_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4, "asm_rows": 7, "asm_lchunk": 8}
 
 
 def _value(key, v):
@@ -36,20 +36,20 @@ def _get(key):
 
 @contextlib.contextmanager
 def knobs(cfg):
-    """Set the library's LU knobs from cfg (keys 'nbo', 'lookahead', 'gemm_split') for the
-    duration."""
+    """Set the library's knobs from cfg (keys 'nbo', 'lookahead', 'gemm_split', 'asm_rows',
+    'asm_lchunk') for the duration."""
     if not cfg:
         yield
         return
-    old = {k: _get(i) for k, i in _KEYS.items()}
+    old = {k: _get(i) for k, i in _KEYS.items() if k in cfg}
     try:
         for k, i in _KEYS.items():
             if k in cfg:
                 call("mnp_set_tuning", i, _value(k, cfg[k]))
         yield
     finally:
-        for k, i in _KEYS.items():
-            call("mnp_set_tuning", i, old[k])
+        for k, v in old.items():
+            call("mnp_set_tuning", _KEYS[k], v)
 
 
 def candidates(n=0):
@@ -107,4 +107,41 @@ def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):
             best = dict(alt, ms=round(ms, 3))
     del A, W
     _CACHE[key] = best
+    return best
+
+
+_ASM_CACHE = {}
+
+
+def asm_candidates(nlam):
+    """Assembly shapes: rows per workgroup x wavelengths per thread (0 = all of the launch)."""
+    chunks = [0] + [c for c in (32, 8, 2) if c < nlam]
+    return [dict(asm_rows=r, asm_lchunk=c) for r in (64, 128, 256) for c in chunks]
+
+
+def tune_assembly(gb, kmat, nlam, reps=1):
+    """Fastest assembly shape for GreenBatch ``gb`` (its geometry, precision and batch) with
+    wavenumbers kmat: every candidate assembles into gb's own matrices (overwritten by the real
+    assembly afterwards), timed after one untimed call; cached per (n, nlam, precision,
+    particles)."""
+    g = gb.geo
+    key = (int(g.n), int(nlam), str(gb.dtype), int(g.npart))
+    if key in _ASM_CACHE:
+        return _ASM_CACHE[key]
+    dev = g.dev
+    best = None
+    for cfg in asm_candidates(nlam):
+        with knobs(cfg):
+            times = []
+            for rep in range(reps + 1):
+                torch.cuda.synchronize(dev)
+                t0 = time.perf_counter()
+                gb.assemble(kmat, nlam)
+                torch.cuda.synchronize(dev)
+                if rep:
+                    times.append(time.perf_counter() - t0)
+        ms = 1e3 * min(times)
+        if best is None or ms < best["ms"]:
+            best = dict(cfg, ms=round(ms, 3))
+    _ASM_CACHE[key] = best
     return best

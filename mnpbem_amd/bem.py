@@ -104,6 +104,7 @@ class BEMRet:
                 nb = tuning["nb"]
             self.tuning = tuning
             s = engine.RetSolverBatch(self.g.geo, batch, precision, nb, self.refine_tol, self.max_refine, tuning)
+            s.tune_assembly = bool(self.autotune if self.autotune is not None else batch >= 8)
             self._solver = s
         return s
 

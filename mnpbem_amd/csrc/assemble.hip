@@ -14,22 +14,28 @@
 // whose coefficients are wavelength independent (greenret_refined.py:738-777).
 //
 // Layout: column-major matrices, interleaved complex, leading dimension ld, one matrix per
-// wavelength at stride `stride` elements. One launch assembles the four matrices for
-// every wavelength of the batch; geometry terms are computed once per entry and reused.
+// wavelength at stride `stride` elements. One launch assembles the four matrices for every
+// wavelength of the batch: a workgroup of 256 lanes owns R rows x (256 / R) columns of entries
+// (lanes along the rows: each store instruction writes R consecutive elements of one column)
+// and a chunk of LB wavelengths (grid z); the geometry terms of an entry are computed once and
+// reused for every wavelength of its chunk. R and LB are tuning knobs of the handle (small
+// matrices fill the GPU only with short wavelength chunks); the autotuner times them per shape.
 #include "common.hpp"
 #include "args.hpp"
+#include "ctx.hpp"
 
 namespace mnp {
 
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 
-template <typename OT>
-__global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, cx<OT>* G1, cx<OT>* G2, cx<OT>* H1,
+template <typename OT, int R>
+__global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, int lb, cx<OT>* G1, cx<OT>* G2, cx<OT>* H1,
                                                      cx<OT>* H2) {
-    int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    int i = blockIdx.x * R + (threadIdx.x % R);
+    int j = blockIdx.y * (256 / R) + (threadIdx.x / R);
     if (i >= a.n || j >= a.n) return;
+    const int l0 = blockIdx.z * lb, l1 = min(a.nlam, l0 + lb);
     double rx = a.pos[3 * i] - a.pos[3 * j];
     double ry = a.pos[3 * i + 1] - a.pos[3 * j + 1];
     double rz = a.pos[3 * i + 2] - a.pos[3 * j + 2];
@@ -43,7 +49,7 @@ __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, cx<OT>* G1, cx<
     double inv_d = diag ? 0.0 : 1.0 / d;
     long off = (long)i + (long)j * a.ld;
     const cx<double>* kk = reinterpret_cast<const cx<double>*>(a.kmat);
-    for (int l = 0; l < a.nlam; ++l) {
+    for (int l = l0; l < l1; ++l) {
         long o = off + (long)l * a.stride;
         for (int side = 0; side < 2; ++side) {
             int m = side ? m2 : m1;
@@ -65,11 +71,12 @@ __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, cx<OT>* G1, cx<
 }
 
 template <typename OT>
-__global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int nref, const int* rrow, const int* rcol,
-                                                         const double* rg, const double* rf, cx<OT>* G1,
-                                                         cx<OT>* G2, cx<OT>* H1, cx<OT>* H2) {
+__global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int lb, int nref, const int* rrow,
+                                                         const int* rcol, const double* rg, const double* rf,
+                                                         cx<OT>* G1, cx<OT>* G2, cx<OT>* H1, cx<OT>* H2) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nref) return;
+    const int l0 = blockIdx.y * lb, l1 = min(a.nlam, l0 + lb);
     int i = rrow[p], j = rcol[p];
     int no = a.order + 1;
     double rx = a.pos[3 * i] - a.pos[3 * j];
@@ -90,7 +97,7 @@ __global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int nref, c
     bool diag = (i == j);
     long off = (long)i + (long)j * a.ld;
     const cx<double>* kk = reinterpret_cast<const cx<double>*>(a.kmat);
-    for (int l = 0; l < a.nlam; ++l) {
+    for (int l = l0; l < l1; ++l) {
         long o = off + (long)l * a.stride;
         for (int side = 0; side < 2; ++side) {
             int m = side ? m2 : m1;
@@ -115,13 +122,37 @@ __global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int nref, c
 template <typename OT>
 void launch_assemble(const AsmArgs& a, int nref, const int* rrow, const int* rcol, const double* rg,
                      const double* rf, void* G1, void* G2, void* H1, void* H2, hipStream_t s) {
-    dim3 grid((a.n + 63) / 64, (a.n + 3) / 4);
-    hipLaunchKernelGGL(k_assemble_far<OT>, grid, dim3(256), 0, s, a, (cx<OT>*)G1, (cx<OT>*)G2, (cx<OT>*)H1,
-                       (cx<OT>*)H2);
-    if (nref > 0) {
-        hipLaunchKernelGGL(k_assemble_refined<OT>, dim3((nref + 255) / 256), dim3(256), 0, s, a, nref, rrow, rcol,
-                           rg, rf, (cx<OT>*)G1, (cx<OT>*)G2, (cx<OT>*)H1, (cx<OT>*)H2);
+    const Ctx& h = ctx();
+    const int R = h.asm_rows;
+    const int lb = (h.asm_lchunk > 0 && h.asm_lchunk < a.nlam) ? h.asm_lchunk : a.nlam;
+    const int nz = (a.nlam + lb - 1) / lb;
+    dim3 grid((a.n + R - 1) / R, (a.n + 256 / R - 1) / (256 / R), nz);
+    cx<OT>* g1 = (cx<OT>*)G1;
+    cx<OT>* g2 = (cx<OT>*)G2;
+    cx<OT>* h1 = (cx<OT>*)H1;
+    cx<OT>* h2 = (cx<OT>*)H2;
+    switch (R) {
+        case 128: hipLaunchKernelGGL((k_assemble_far<OT, 128>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2); break;
+        case 256: hipLaunchKernelGGL((k_assemble_far<OT, 256>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2); break;
+        default: hipLaunchKernelGGL((k_assemble_far<OT, 64>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2);
     }
+    if (nref > 0) {
+        hipLaunchKernelGGL(k_assemble_refined<OT>, dim3((nref + 255) / 256, nz), dim3(256), 0, s, a, lb, nref, rrow,
+                           rcol, rg, rf, g1, g2, h1, h2);
+    }
+}
+
+// Assembly knobs of the calling thread's handle: rows per workgroup (64, 128, 256) and
+// wavelengths per thread (0: all of the launch).
+int set_asm_rows(int v) {
+    if (v != 64 && v != 128 && v != 256) return -2;
+    ctx().asm_rows = v;
+    return 0;
+}
+int set_asm_lchunk(int v) {
+    if (v < 0) return -2;
+    ctx().asm_lchunk = v;
+    return 0;
 }
 
 template void launch_assemble<double>(const AsmArgs&, int, const int*, const int*, const double*, const double*,

@@ -20,7 +20,10 @@ Ctx::Ctx()
       panel_reg(env_int("MNP_PANEL_REG", 1) != 0),
       coop_min_rows(env_int("MNP_PANEL_COOP_MIN_ROWS", 4096)),
       gemm_split(env_int("MNP_GEMM_SPLIT", 0)),
-      plane_group(env_int("MNP_GEMM_GROUP", 8)) {}
+      plane_group(env_int("MNP_GEMM_GROUP", 8)),
+      asm_rows(env_int("MNP_ASM_ROWS", 64)),
+      asm_lchunk(env_int("MNP_ASM_LCHUNK", 0)),
+      tall_panel(env_int("MNP_TALL_PANEL", 1)) {}
 
 hipEvent_t Ctx::event() {
     if (!pool.empty()) {

@@ -33,6 +33,9 @@ struct Ctx {
     int coop_min_rows;     // 3: panel height from which the multi-workgroup panel is used
     int gemm_split;        // 4: complex-float GEMM mode
     int plane_group;       // 6: row tiles per group of the plane GEMM's XCD-aware order
+    int asm_rows;          // 7: rows per workgroup of the Green-function assembly
+    int asm_lchunk;        // 8: wavelengths per assembly thread (0: all)
+    int tall_panel;        // 9: tall panels by tournament pivoting (1) or cooperative partial pivoting (0)
     // cooperative-panel scratch (candidates, diagonal rows, barrier counters)
     void* coop_ws = nullptr;
     size_t coop_bytes = 0;

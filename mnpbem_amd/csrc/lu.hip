@@ -328,6 +328,16 @@ template <>
 struct CoopRows<double> {
     static constexpr int value = 256;
 };
+// candidate rows per workgroup of a tournament round (one per lane; complex float at 1024 lanes
+// would spill the row and the published pivot row out of 128 registers)
+template <typename T>
+struct CaluRows {
+    static constexpr int value = 512;
+};
+template <>
+struct CaluRows<double> {
+    static constexpr int value = 256;
+};
 constexpr long CP_SPIN_LIMIT = 1L << 22;
 
 __device__ __forceinline__ void matrix_barrier(unsigned* cnt, unsigned target, int* flag) {
@@ -463,6 +473,7 @@ struct CoopWS {
     void* cv;
     int* ci;
     unsigned* cnt;
+    void* top;    // factored top blocks of tournament panels (batch x CP_W x CP_W)
     int gmax;
 };
 
@@ -476,11 +487,13 @@ template <typename T>
 static bool coop_prepare(int n, int batch, hipStream_t s, CoopWS& ws) {
     Ctx& h = ctx();
     if (n < h.coop_min_rows) return false;
-    constexpr int CP_T = CoopRows<T>::value;
+    // workgroups per matrix of the cooperative panel or of a tournament round, at most
+    constexpr int CP_T = CoopRows<T>::value < CaluRows<T>::value ? CoopRows<T>::value : CaluRows<T>::value;
     const int gmax = (n + CP_T - 1) / CP_T;
     const size_t ncand = 2 * (size_t)batch * gmax;
+    const size_t tbytes = (size_t)batch * CP_W * CP_W * sizeof(cx<T>);
     const size_t bytes = ncand * CP_W * sizeof(cx<T>) + 2 * (size_t)batch * CP_W * sizeof(cx<T>) +
-                         ncand * (sizeof(T) + sizeof(int)) + (size_t)batch * n * sizeof(unsigned) + 1024;
+                         ncand * (sizeof(T) + sizeof(int)) + (size_t)batch * n * sizeof(unsigned) + tbytes + 1280;
     if (bytes > h.coop_bytes) {
         if (h.coop_ws) {
             hipStreamSynchronize(s);
@@ -503,6 +516,7 @@ static bool coop_prepare(int n, int batch, hipStream_t s, CoopWS& ws) {
     ws.ci = (int*)p;
     p += ncand * sizeof(int);
     ws.cnt = (unsigned*)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+    ws.top = (void*)((((uintptr_t)(ws.cnt + (size_t)batch * n)) + 255) & ~(uintptr_t)255);
     ws.gmax = gmax;
     const long nc = (long)batch * n;
     hipLaunchKernelGGL(k_zero_u32, dim3((nc + 255) / 256), dim3(256), 0, s, nc, ws.cnt);
@@ -533,6 +547,201 @@ static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* i
     void* args[] = {&n, &j0, &w, &a, &lda, &sA, &ipiv, &sP, &info, &cv, &ci, &crow, &drow, &cnt};
     return hipLaunchCooperativeKernel((const void*)k_panel_coop<T>, dim3(G, batch), dim3(CP_T), args, 0, s) ==
            hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Tall panels by tournament pivoting (CALU / TSLU): the w pivot rows of a panel of m rows are
+// chosen by a reduction tree instead of a search across the whole column for every column.
+//   k_calu_round: each workgroup takes up to CaluRows candidate rows (a contiguous block of the
+//          panel, or the winners of the previous round, from their original entries), runs LU
+//          with partial pivoting on them in registers (largest |re|+|im|, the earlier candidate
+//          on a tie, as izamax) and keeps its w pivot rows in pivot order; rounds repeat until
+//          one workgroup remains, which also writes the packed LU of its pivot rows: the
+//          factored w x w top block of the panel;
+//   k_calu_ipiv: LAPACK-style interchanges ipiv[j0 + c] that bring the winners to the top;
+//   k_laswp on the panel's own columns;
+//   k_calu_l21: the factored top block into place and L21 = A21 U11^{-1}, one row per lane.
+// The panel's interchanges and factors keep the layout of the other panel kernels, so the rest
+// of the blocked LU is unchanged. Pivots differ from partial pivoting's; no workgroup waits for
+// another, and every elimination step of a round costs one workgroup barrier: each wave
+// publishes its best candidate row in LDS (double-buffered by step parity) and every wave then
+// picks the same winner from those.
+
+// wave-wide maxima of the device library (DPP reductions; every lane gets the result)
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_max_u32(unsigned int);
+extern "C" __device__ __attribute__((const)) unsigned long long __ockl_wfred_max_u64(unsigned long long);
+
+template <typename T>
+struct PivKey {
+    using type = unsigned;
+    static __device__ __forceinline__ type of(T v) { return __float_as_uint(v) + 1u; }
+    static __device__ __forceinline__ type wave_max(type k) { return __ockl_wfred_max_u32(k); }
+    static __device__ __forceinline__ T value(type k) { return __uint_as_float(k - 1u); }
+};
+template <>
+struct PivKey<double> {
+    using type = unsigned long long;
+    static __device__ __forceinline__ type of(double v) { return (type)__double_as_longlong(v) + 1ull; }
+    static __device__ __forceinline__ type wave_max(type k) { return __ockl_wfred_max_u64(k); }
+    static __device__ __forceinline__ double value(type k) { return __longlong_as_double((long long)(k - 1ull)); }
+};
+
+// One round. in == nullptr: candidates are rows j0 + c CP_T + t (< n); else in[b nin + c CP_T + t]
+// (< nin; -1 = none). out[(b G + c) w + k]: k-th winner (global row, -1 = none). A single
+// workgroup (the last round) also writes best[b w + k] (|pivot|) and the packed factor row k of
+// its winners to top[(b CP_W + k) CP_W + u].
+template <typename T>
+__global__ __launch_bounds__(CaluRows<T>::value) void k_calu_round(int n, int j0, int w, const cx<T>* A, long lda,
+                                                                   long sA, const int* in, int nin, int* out,
+                                                                   T* best, cx<T>* top) {
+    constexpr int CP_T = CaluRows<T>::value;
+    constexpr int NW = CP_T / 64;
+    using K = typename PivKey<T>::type;
+    __shared__ K key_s[2][NW];
+    __shared__ int row_s[2][NW];
+    __shared__ cx<T> prow[2][NW][CP_W];
+    const int G = gridDim.x, c = blockIdx.x, b = blockIdx.y, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    A += (long)b * sA;
+    int row = -1;
+    if (in == nullptr) {
+        const int r = j0 + c * CP_T + t;
+        row = r < n ? r : -1;
+    } else {
+        const int q = c * CP_T + t;
+        row = q < nin ? in[(long)b * nin + q] : -1;
+    }
+    bool live = row >= 0;
+    cx<T> x[CP_W];
+#pragma unroll
+    for (int u = 0; u < CP_W; ++u) x[u] = (live && u < w) ? A[row + (long)(j0 + u) * lda] : mk<T>(0, 0);
+    int* o = out + ((long)b * G + c) * w;
+#pragma clang loop unroll(full)
+    for (int k = 0; k < CP_W; ++k) {
+        if (k < w) {
+            const int buf = k & 1;
+            const K mine = live ? PivKey<T>::of(cabs1(x[k])) : (K)0;
+            const K wmax = PivKey<T>::wave_max(mine);
+            const unsigned long long hit = __ballot(live && mine == wmax);
+            const int wl = hit ? __ffsll(hit) - 1 : -1;
+            if (lane == wl) {
+                key_s[buf][wv] = wmax;
+                row_s[buf][wv] = row;
+#pragma unroll
+                for (int u = 0; u < CP_W; ++u)
+                    if (u < w) prow[buf][wv][u] = x[u];
+            } else if (wl < 0 && lane == 0) {
+                key_s[buf][wv] = 0;
+            }
+            __syncthreads();
+            K bk = key_s[buf][0];
+            int bw = 0;
+#pragma unroll
+            for (int q = 1; q < NW; ++q) {
+                const K kq = key_s[buf][q];
+                if (kq > bk) { bk = kq; bw = q; }
+            }
+            if (bk != 0) {
+                const int win = row_s[buf][bw];
+                if (live && row == win) live = false;
+                const cx<T> piv = prow[buf][bw][k];
+                if (live && !(piv.re == 0 && piv.im == 0)) {
+                    const cx<T> l = x[k] * crecip(piv);
+                    x[k] = l;
+#pragma unroll
+                    for (int u = k + 1; u < CP_W; ++u)
+                        if (u < w) x[u] = fms(x[u], l, prow[buf][bw][u]);
+                }
+                if (t == 0) o[k] = win;
+                if (top != nullptr) {
+                    if (t == 0) best[(long)b * w + k] = PivKey<T>::value(bk);
+                    if (t < w) top[((long)b * CP_W + k) * CP_W + t] = prow[buf][bw][t];
+                }
+            } else {
+                if (t == 0) o[k] = -1;
+                if (top != nullptr && t == 0) best[(long)b * w + k] = (T)0;
+            }
+        }
+    }
+}
+
+// Interchanges that bring the winners sel[0 .. w) (global rows, in pivot order) to rows j0 + c,
+// applied one after the other (LAPACK convention), and info from a zero pivot. One wave per
+// matrix: lane q < w holds the original row now at position j0 + q, lane q < nd one position
+// below the top block that a displaced row now occupies.
+template <typename T>
+__global__ __launch_bounds__(64) void k_calu_ipiv(int j0, int w, const int* sel, const T* best, int* ipiv, long sP,
+                                                  int* info) {
+    const int b = blockIdx.x, q = threadIdx.x;
+    sel += (long)b * w;
+    best += (long)b * w;
+    int cur = q < w ? j0 + q : -1;
+    int dpos = -1, drow = -1, nd = 0, first_zero = 0;
+    for (int c = 0; c < w; ++c) {
+        const int col = j0 + c;
+        const int want = sel[c];
+        int p = col;
+        if (want >= 0) {
+            const unsigned long long in_top = __ballot(q >= c && q < w && cur == want);
+            const unsigned long long in_down = __ballot(q < nd && drow == want);
+            int slot = -1;
+            if (in_top) {
+                p = j0 + __ffsll(in_top) - 1;
+            } else if (in_down) {
+                slot = __ffsll(in_down) - 1;
+                p = __shfl(dpos, slot, 64);
+            } else {
+                p = want;
+            }
+            const int rc = __shfl(cur, c, 64);
+            if (p != col) {
+                if (p < j0 + w) {
+                    if (q == p - j0) cur = rc;
+                } else if (slot >= 0) {
+                    if (q == slot) drow = rc;
+                } else {
+                    if (q == nd) { dpos = p; drow = rc; }
+                    ++nd;
+                }
+                if (q == c) cur = want;
+            }
+        }
+        if (q == 0) ipiv[(long)b * sP + col] = p;
+        if (best[c] == (T)0 && first_zero == 0) first_zero = col + 1;
+    }
+    if (q == 0 && first_zero != 0 && info[b] == 0) info[b] = first_zero;
+}
+
+// The factored top block (rows j0 .. j0 + w) from top[] into place (workgroup x = 0), and
+// L21 = A21 U11^{-1} for the rows j0 + w .. n, one lane per row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_calu_l21(int n, int j0, int w, cx<T>* A, long lda, long sA,
+                                                  const cx<T>* top) {
+    __shared__ cx<T> u[CP_W][CP_W + 1];
+    const int t = threadIdx.x, b = blockIdx.y;
+    A += (long)b * sA;
+    top += (long)b * CP_W * CP_W;
+    for (int e = t; e < w * w; e += 256) u[e / w][e % w] = top[(e / w) * CP_W + e % w];
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int e = t; e < w * w; e += 256) A[j0 + e % w + (long)(j0 + e / w) * lda] = u[e % w][e / w];
+    const int row = j0 + w + blockIdx.x * 256 + t;
+    if (row >= n) return;
+    cx<T> x[CP_W];
+#pragma unroll
+    for (int c = 0; c < CP_W; ++c) x[c] = c < w ? A[row + (long)(j0 + c) * lda] : mk<T>(0, 0);
+#pragma unroll
+    for (int c = 0; c < CP_W; ++c) {
+        if (c < w) {
+#pragma unroll
+            for (int k = 0; k < CP_W; ++k)
+                if (k < c) x[c] = fms(x[c], x[k], u[k][c]);
+            const cx<T> d = u[c][c];
+            if (!(d.re == 0 && d.im == 0)) x[c] = x[c] * crecip(d);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CP_W; ++c)
+        if (c < w) A[row + (long)(j0 + c) * lda] = x[c];
 }
 
 // apply interchanges ipiv[r0 .. r1) to the columns [a0, a1) and [b0, b1)
@@ -621,6 +830,41 @@ static void laswp(int r0, int r1, int a0, int a1, int b0, int b1, cx<T>* a, long
                        sA, ipiv, sP);
 }
 
+// Tall panel by tournament pivoting (see k_calu_round); false when the scratch does not fit.
+template <typename T>
+static bool panel_calu(int n, int j0, int w, cx<T>* a, long lda, long sA, int* ipiv, long sP, int* info, int batch,
+                       const CoopWS& ws, hipStream_t s) {
+    constexpr int CP_T = CaluRows<T>::value;
+    const int m = n - j0;
+    int G = (m + CP_T - 1) / CP_T;
+    if (m < ctx().coop_min_rows || w > CP_W || G > ws.gmax) return false;
+    // two index lists of batch x gmax x CP_W ints in the candidate-row scratch; winners and their
+    // magnitudes in the diagonal-row scratch; the factored top blocks in ws.top
+    int* lst[2] = {(int*)ws.crow, (int*)ws.crow + (long)batch * ws.gmax * CP_W};
+    int* sel = (int*)ws.drow;
+    T* best = (T*)((int*)ws.drow + (long)batch * CP_W);
+    cx<T>* top = (cx<T>*)ws.top;
+    int cur = 0;
+    const int* in = nullptr;
+    int nin = 0;
+    for (;;) {
+        const bool last = G == 1;
+        hipLaunchKernelGGL(k_calu_round<T>, dim3(G, batch), dim3(CP_T), 0, s, n, j0, w, (const cx<T>*)a, lda, sA, in,
+                           nin, last ? sel : lst[cur], best, last ? top : (cx<T>*)nullptr);
+        if (last) break;
+        in = lst[cur];
+        nin = G * w;
+        cur = 1 - cur;
+        G = (nin + CP_T - 1) / CP_T;
+    }
+    hipLaunchKernelGGL(k_calu_ipiv<T>, dim3(batch), dim3(64), 0, s, j0, w, (const int*)sel, (const T*)best, ipiv, sP,
+                       info);
+    laswp<T>(j0, j0 + w, j0, j0 + w, 0, 0, a, lda, sA, ipiv, sP, batch, s);
+    const int gx = m > w ? (m - w + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_calu_l21<T>, dim3(gx, batch), dim3(256), 0, s, n, j0, w, a, lda, sA, (const cx<T>*)top);
+    return true;
+}
+
 template <typename T>
 static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, long sA, int batch, hipStream_t s) {
     if (c1 <= c0) return;
@@ -638,15 +882,21 @@ static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, 
 // lookahead on/off, 2 = register-resident panel on/off, 3 = rows from which panels use the
 // cooperative kernel, 4 = complex-float GEMM mode (0: FP32 matrix cores; 12, 13: split-BF16
 // planes of 2 or 3 terms), 5 = workgroup tile of the plane GEMM (11 only), 6 = row tiles per
-// group of its XCD-aware tile order (0: grid order).
+// group of its XCD-aware tile order (0: grid order), 7 = rows per workgroup of the Green-function
+// assembly (64, 128, 256), 8 = wavelengths per assembly thread (0: all).
 int set_gemm_split(int v);
 int get_gemm_split();
 int set_plane_tile(int v);
 int get_plane_tile();
 int set_plane_group(int v);
 int get_plane_group();
+int set_asm_rows(int v);
+int set_asm_lchunk(int v);
 int set_tuning(int key, int value) {
     switch (key) {
+        case 7: return set_asm_rows(value);
+        case 8: return set_asm_lchunk(value);
+        case 9: if (value != 0 && value != 1) return -2; ctx().tall_panel = value; return 0;
         case 4: return set_gemm_split(value);
         case 5: return set_plane_tile(value);
         case 6: return set_plane_group(value);
@@ -666,6 +916,9 @@ int get_tuning(int key) {
         case 4: return get_gemm_split();
         case 5: return get_plane_tile();
         case 6: return get_plane_group();
+        case 7: return ctx().asm_rows;
+        case 8: return ctx().asm_lchunk;
+        case 9: return ctx().tall_panel;
         default: return -1;
     }
 }
@@ -719,7 +972,11 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
         // factor the tall outer panel A[J0:n, J0:JE]
         for (int j0 = J0; j0 < JE; j0 += nb) {
             int w = (JE - j0 < nb) ? (JE - j0) : nb;
-            if (!coop || !panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info, batch, ws, s)) {
+            const bool tall = coop && (h.tall_panel == 1 ? panel_calu<T>(n, j0, w, a, lda, sA, ipiv, sP, info,
+                                                                          batch, ws, s)
+                                                         : panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info,
+                                                                         batch, ws, s));
+            if (!tall) {
                 // complex float only: complex double needs 4-column chunks and runs slower than k_panel
                 if (sizeof(T) == 4 && n - j0 <= RP_T * RP_RPT && w <= RP_MAXW && h.panel_reg)
                     hipLaunchKernelGGL((k_panel_reg<T, RegChunk<T>::CW>), dim3(batch), dim3(RP_T), 0, s, n, j0, w, a, lda,

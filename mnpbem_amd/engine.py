@@ -189,6 +189,8 @@ class RetSolverBatch:
         self.opwork = torch.empty((batch, 32, n), dtype=C128, device=d) if precision == "mixed" else None
         self.nlam = 0
         self.stats = {}
+        self.tune_assembly = False     # time the assembly shapes on the first factorisation
+        self.asm_cfg = {}              # chosen shape per precision of the assembled matrices
 
     @property
     def prec(self):
@@ -203,10 +205,17 @@ class RetSolverBatch:
         self.kmat, self.L1, self.L2, self.kvac, self.kmat_np, self.eps_np = g.media(enei_list)
         self.enei = enei_list
         self.nlam = nl
-        if self.exact is not None:
-            self.exact.assemble(self.kmat, nl)
-        self.fac.assemble(self.kmat, nl)
+        for gb in ((self.exact, self.fac) if self.exact is not None else (self.fac,)):
+            self._assemble_tuned(gb, nl)
         return self._factor_assembled()
+
+    def _assemble_tuned(self, gb, nl):
+        from .autotune import knobs, tune_assembly
+        key = str(gb.dtype)
+        if self.tune_assembly and key not in self.asm_cfg:
+            self.asm_cfg[key] = tune_assembly(gb, self.kmat, nl)
+        with knobs(self.asm_cfg.get(key)):
+            gb.assemble(self.kmat, nl)
 
     def _factor_assembled(self):
         """LU chain of the assembled G1, G2, H1, H2 of the first self.nlam matrices."""

@@ -298,3 +298,25 @@ def test_solver_rejects_short_rhs(sphere):
     slv.factor([500.0, 600.0, 700.0])
     with pytest.raises(ValueError):
         slv.solve(torch.zeros((1, 8, s.n), dtype=torch.complex128, device="cuda"))
+
+
+def test_assembly_shapes_all_equal_and_tuner_picks_one(sphere):
+    """Every assembly shape the tuner may pick (rows per workgroup x wavelengths per thread)
+    writes the same matrices, bit for bit, as the default; the tuner returns one of them."""
+    from mnpbem_amd import autotune
+    cp, s, fp, closed = sphere
+    geo = M.BEMRet(cp).g.geo
+    lams = np.linspace(450, 800, 11)
+    kmat = geo.media(lams)[0]
+    banner("assembly shapes n={} nlam={}".format(s.n, len(lams)))
+    for dt in (torch.complex128, torch.complex64):
+        ref = GreenBatch(geo, len(lams), dt)
+        ref.assemble(kmat, len(lams))
+        for cfg in autotune.asm_candidates(len(lams)):
+            gb = GreenBatch(geo, len(lams), dt)
+            with autotune.knobs(cfg):
+                gb.assemble(kmat, len(lams))
+            for a, b in zip((gb.G1, gb.G2, gb.H1, gb.H2), (ref.G1, ref.G2, ref.H1, ref.H2)):
+                assert torch.equal(a, b), cfg
+        best = autotune.tune_assembly(ref, kmat, len(lams))
+        assert {k: best[k] for k in ("asm_rows", "asm_lchunk")} in autotune.asm_candidates(len(lams))

@@ -209,8 +209,28 @@ def test_solves(prec):
         assert np.max(np.abs(_np(ty, b) - y)) / np.max(np.abs(y)) < (1e-13 if prec == 0 else 1e-5)
 
 
+class _knobs:
+    """Library knobs (key -> value) for the block, restored afterwards."""
+
+    def __init__(self, **kv):
+        self.kv = {int(k[1:]): v for k, v in kv.items()}
+
+    def __enter__(self):
+        self.old = {}
+        for k, v in self.kv.items():
+            o = ctypes.c_int(0)
+            _lib().call("mnp_get_tuning", k, ctypes.addressof(o))
+            self.old[k] = o.value
+            _lib().call("mnp_set_tuning", k, v)
+
+    def __exit__(self, *exc):
+        for k, v in self.old.items():
+            _lib().call("mnp_set_tuning", k, v)
+
+
 def test_getrf_multiworkgroup_panel_matches_lapack():
-    """Panels taller than one workgroup (cooperative launch): pivots and factors as LAPACK."""
+    """Panels taller than one workgroup with partial pivoting (cooperative launch, tall-panel
+    knob 9 = 0): pivots and factors as LAPACK."""
     banner("getrf n=4700 batch 2 (panel over several workgroups)")
     rng = np.random.default_rng(11)
     n, B = 4700, 2
@@ -218,12 +238,84 @@ def test_getrf_multiworkgroup_panel_matches_lapack():
     t = _cm(A, torch.complex128)
     ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
     info = torch.zeros(B, dtype=torch.int32, device=_dev())
-    _lib().call("mnp_getrf_batched", 0, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    with _knobs(k9=0):
+        _lib().call("mnp_getrf_batched", 0, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
+                    _sh())
     assert np.all(info.cpu().numpy() == 0)
     for b in range(B):
         lu, piv = sla.lu_factor(A[b])
         assert np.array_equal(ipiv[b].cpu().numpy(), piv)
         assert np.max(np.abs(_np(t, b) - lu)) < 1e-10 * np.max(np.abs(lu))
+
+
+@pytest.mark.parametrize("n,tall", [(4700, 4096), (2000, 512)])
+def test_getrf_tournament_panels_match_oracle(n, tall):
+    """Tall panels by tournament pivoting (knob 9 = 1, panels of at least `tall` rows, blocks of
+    256 rows for complex128): pivots equal to the oracle's tournament (oracle/calu_oracle.py)
+    exactly, factors to 1e-10, and P A = L U."""
+    from oracle import calu_oracle as C
+    banner("getrf tournament n={} tall={} batch 2".format(n, tall))
+    rng = np.random.default_rng(12)
+    B = 2
+    A = [_rand(rng, n, n) for _ in range(B)]
+    t = _cm(A, torch.complex128)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    with _knobs(k9=1, k3=tall):
+        _lib().call("mnp_getrf_batched", 0, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
+                    _sh())
+    assert np.all(info.cpu().numpy() == 0)
+    for b in range(B):
+        lu, piv = C.getrf(A[b], nb=32, tall=tall, cp=256)
+        got = ipiv[b].cpu().numpy()
+        assert not np.array_equal(piv, sla.lu_factor(A[b])[1])
+        assert np.array_equal(got, piv)
+        assert np.max(np.abs(_np(t, b) - lu)) < 1e-10 * np.max(np.abs(lu))
+
+
+def test_getrf_tournament_panels_complex64_backward_error():
+    """complex64 (blocks of 1024 rows), panels of at least 1024 rows by tournament: normwise
+    backward error of P A = L U within 3x that of the same factorisation with partial pivoting
+    (knob 9 = 0), bounded multipliers."""
+    banner("getrf tournament complex64 n=3000")
+    rng = np.random.default_rng(13)
+    n = 3000
+    A = _rand(rng, n, n)
+    err = {}
+    for mode in (1, 0):
+        t = _cm([A], torch.complex64)
+        ipiv = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+        info = torch.zeros(1, dtype=torch.int32, device=_dev())
+        with _knobs(k9=mode, k3=1024):
+            _lib().call("mnp_getrf_batched", 1, n, 1, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(),
+                        32, _sh())
+        assert int(info.item()) == 0
+        lu = _np(t, 0).astype(complex)
+        L = np.tril(lu, -1) + np.eye(n)
+        U = np.triu(lu)
+        PA = A.copy()
+        for i, p in enumerate(ipiv[0].cpu().numpy()):
+            PA[[i, p]] = PA[[p, i]]
+        assert np.max(np.abs(L)) < 20
+        err[mode] = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+    print("backward error tournament {:.3g} partial pivoting {:.3g}".format(err[1], err[0]))
+    assert err[0] < 1e-4 and err[1] < 3 * err[0]
+
+
+def test_getrf_tournament_singular_column_reports_info():
+    """A zero column inside a tournament panel: info = its 1-based index, as LAPACK."""
+    banner("getrf tournament singular n=1100")
+    rng = np.random.default_rng(14)
+    n = 1100
+    A = _rand(rng, n, n)
+    A[:, 700] = 0
+    t = _cm([A], torch.complex128)
+    ipiv = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(1, dtype=torch.int32, device=_dev())
+    with _knobs(k9=1, k3=256):
+        _lib().call("mnp_getrf_batched", 0, n, 1, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
+                    _sh())
+    assert int(info.item()) == 701
 
 
 def test_large_single_matrix_factor_and_solve():
