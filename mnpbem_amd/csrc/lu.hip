@@ -611,6 +611,7 @@ __global__ __launch_bounds__(CaluRows<T>::value) void k_calu_round(int n, int j0
         row = q < nin ? in[(long)b * nin + q] : -1;
     }
     bool live = row >= 0;
+    // columns from w on stay zero, so that every step updates all CP_W columns without a branch
     cx<T> x[CP_W];
 #pragma unroll
     for (int u = 0; u < CP_W; ++u) x[u] = (live && u < w) ? A[row + (long)(j0 + u) * lda] : mk<T>(0, 0);
@@ -627,8 +628,7 @@ __global__ __launch_bounds__(CaluRows<T>::value) void k_calu_round(int n, int j0
                 key_s[buf][wv] = wmax;
                 row_s[buf][wv] = row;
 #pragma unroll
-                for (int u = 0; u < CP_W; ++u)
-                    if (u < w) prow[buf][wv][u] = x[u];
+                for (int u = 0; u < CP_W; ++u) prow[buf][wv][u] = x[u];
             } else if (wl < 0 && lane == 0) {
                 key_s[buf][wv] = 0;
             }
@@ -642,15 +642,15 @@ __global__ __launch_bounds__(CaluRows<T>::value) void k_calu_round(int n, int j0
             }
             if (bk != 0) {
                 const int win = row_s[buf][bw];
-                if (live && row == win) live = false;
+                live = live && row != win;
                 const cx<T> piv = prow[buf][bw][k];
-                if (live && !(piv.re == 0 && piv.im == 0)) {
-                    const cx<T> l = x[k] * crecip(piv);
-                    x[k] = l;
+                const bool zero = piv.re == 0 && piv.im == 0;
+                const cx<T> r = zero ? mk<T>(0, 0) : crecip(piv);
+                // a zero pivot leaves every live x[k] at zero: no update, as partial pivoting
+                const cx<T> l = live ? x[k] * r : mk<T>(0, 0);
+                if (live) x[k] = l;
 #pragma unroll
-                    for (int u = k + 1; u < CP_W; ++u)
-                        if (u < w) x[u] = fms(x[u], l, prow[buf][bw][u]);
-                }
+                for (int u = k + 1; u < CP_W; ++u) x[u] = fms(x[u], l, prow[buf][bw][u]);
                 if (t == 0) o[k] = win;
                 if (top != nullptr) {
                     if (t == 0) best[(long)b * w + k] = PivKey<T>::value(bk);
@@ -720,7 +720,17 @@ __global__ __launch_bounds__(256) void k_calu_l21(int n, int j0, int w, cx<T>* A
     const int t = threadIdx.x, b = blockIdx.y;
     A += (long)b * sA;
     top += (long)b * CP_W * CP_W;
-    for (int e = t; e < w * w; e += 256) u[e / w][e % w] = top[(e / w) * CP_W + e % w];
+    __shared__ cx<T> dinv[CP_W];
+    // U11 padded to CP_W x CP_W with the identity, so that columns from w on stay zero; the
+    // reciprocals of its diagonal (1 for a zero pivot: that column is left unscaled)
+    for (int e = t; e < CP_W * CP_W; e += 256) {
+        const int r = e / CP_W, c = e % CP_W;
+        u[r][c] = (r < w && c < w) ? top[r * CP_W + c] : mk<T>(r == c ? 1 : 0, 0);
+    }
+    if (t < CP_W) {
+        const cx<T> d = t < w ? top[t * CP_W + t] : mk<T>(1, 0);
+        dinv[t] = (d.re == 0 && d.im == 0) ? mk<T>(1, 0) : crecip(d);
+    }
     __syncthreads();
     if (blockIdx.x == 0)
         for (int e = t; e < w * w; e += 256) A[j0 + e % w + (long)(j0 + e / w) * lda] = u[e % w][e / w];
@@ -731,13 +741,9 @@ __global__ __launch_bounds__(256) void k_calu_l21(int n, int j0, int w, cx<T>* A
     for (int c = 0; c < CP_W; ++c) x[c] = c < w ? A[row + (long)(j0 + c) * lda] : mk<T>(0, 0);
 #pragma unroll
     for (int c = 0; c < CP_W; ++c) {
-        if (c < w) {
 #pragma unroll
-            for (int k = 0; k < CP_W; ++k)
-                if (k < c) x[c] = fms(x[c], x[k], u[k][c]);
-            const cx<T> d = u[c][c];
-            if (!(d.re == 0 && d.im == 0)) x[c] = x[c] * crecip(d);
-        }
+        for (int k = 0; k < c; ++k) x[c] = fms(x[c], x[k], u[k][c]);
+        x[c] = x[c] * dinv[c];
     }
 #pragma unroll
     for (int c = 0; c < CP_W; ++c)
