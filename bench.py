@@ -32,6 +32,11 @@ DRUDE = (10.0, 9.065, 0.0708)
 ROD_N = [25, 11, 34]
 
 
+def _progress(msg):
+    """One line on stderr per finished stage (the JSON result is the only stdout line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 # HBM bytes per k_gemm_sub_c64p<3,1,1> launch of this sweep (batch 201, 478 launches) from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a 256 MiB
 # copy in the same run (FETCH x2.0, WRITE x1.0); the commands and raw averages are in the file named.
@@ -415,6 +420,7 @@ def main():
         args.batch = len(lams)
     torch.cuda.set_device(0)
     g = M.CompGreenRet(cp, cp)
+    _progress("workload n={} ready".format(n))
 
     def sweep(prec):
         bem = solvers[prec]
@@ -430,6 +436,7 @@ def main():
     for prec in solvers:
         for _ in range(args.warmup):
             sweep(prec)
+            _progress("{} warm-up sweep done".format(prec))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -437,6 +444,7 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         results[prec] = (dt, out)
+        _progress("{} sweep: {:.1f} solves/s".format(prec, args.steps * len(lams) / dt))
         solvers_tuning[prec] = solvers[prec].tuning
         solvers[prec]._solver = None          # the next precision gets the memory (large batches)
         torch.cuda.empty_cache()
@@ -449,6 +457,7 @@ def main():
     _lib.call("mnp_prof_collect", prof.ctypes.data)
     _lib.call("mnp_prof_enable", 0)
     gemm_ms, gemm_flops, gemm_launches, gemm_bytes = prof
+    _progress("GEMM timing sweep: {:.1f} ms in {} launches".format(gemm_ms, int(gemm_launches)))
     # algorithmic work of the GEMM launches (8 m n k real flops per complex product, the
     # complex FP32-equivalent count) over their kernel time, against the peak of the matrix
     # cores the kernel runs on (BF16 for the split operands, FP32 otherwise)
@@ -485,6 +494,7 @@ def main():
         extra["mixed_2term_refine_iterations_max"] = int(max(out2.get("refine_iterations", [0]) or [0]))
         b2._solver = None
         del b2
+        _progress("2-term mixed sweep: {:.1f} solves/s".format(extra["mixed_2term_solves_per_s"]))
         torch.cuda.empty_cache()
     flops_per_solve = 34.67 * n ** 3
     extra["path_tflops_mixed"] = round(value * flops_per_solve / 1e12, 3)
@@ -494,6 +504,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cp, args.cpu_sample)
+        _progress("cpu baseline: {}".format(cpu))
 
     line = {
         "metric": METRIC,
@@ -535,12 +546,16 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_config1:
         line["config1"] = config1(skip_cpu=args.no_cpu)
+        _progress("config1 done")
     if not args.no_batched64:
         line["batched64"] = batched64()
+        _progress("batched64 done")
     if not args.no_gmres25k:
         line["gmres25k"] = gmres25k()
+        _progress("gmres25k done")
     if not args.no_warmstart:
         line["warmstart25k"] = warmstart25k()
+        _progress("warmstart25k done")
     print(json.dumps(line), flush=True)
 
 

@@ -68,6 +68,17 @@ int mnp_probe_mfma_layout(double* out, void* stream);
 int mnp_prof_enable(int on);
 int mnp_prof_collect(double* out);
 
+/* Tracing: every compute entry point below opens a roctx range "mnp:<stage>" (assemble, getrf,
+ * getrs, trsm_right, gemm, gemv, bemret_factor, bemret_solve, refine_operator, planewave,
+ * farfield, near_terms, colsum_far, hmat_dense, aca, hmat_matvec, gmres_operator, surface_eval,
+ * point_eval, mlp_gemm) for the time it takes to enqueue its work, visible to rocprofv3
+ * --marker-trace. These two let the host wrap its own stages (a GMRES iteration, a refinement
+ * step) the same way. The reference has no tracer; its stage timings are perf_counter prints
+ * (bem/bem_ret.py:597-601, :931-937) and BEMIter.tocout (bem/bem_iter.py:134-190), which these
+ * ranges replace. name: host string. */
+int mnp_range_push(const char* name);
+int mnp_range_pop(void);
+
 /* G1, G2, H1, H2 of a composite particle for nlam wavelengths.
  * Replaces CompGreenRet.eval(i, i, 'G'|'H1'|'H2', enei) (greenfun/compgreen_ret.py:507-553,
  * :739-858) with GreenRetRefined.eval (greenfun/greenret_refined.py:654-810) per block.

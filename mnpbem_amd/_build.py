@@ -20,6 +20,11 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-Wno-unused-result", "-Wno-unused-value"]
 
 
+# roctx (rocprofiler-sdk) for the per-stage trace ranges of the C-ABI entry points
+ROCM_LIB = "/opt/rocm/lib"
+LIBS = ["-L" + ROCM_LIB, "-Wl,-rpath," + ROCM_LIB, "-lrocprofiler-sdk-roctx"]
+
+
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
     cmd = [HIPCC] + FLAGS + ["-I", CSRC, "-c", src, "-o", obj]
@@ -36,7 +41,7 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n{}\n{}".format(" ".join(cmd), r.stderr))

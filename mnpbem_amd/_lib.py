@@ -24,6 +24,8 @@ _SIGS = {
     "mnp_probe_mfma_layout": [P, P],
     "mnp_prof_enable": [I],
     "mnp_prof_collect": [P],
+    "mnp_range_push": [C.c_char_p],
+    "mnp_range_pop": [],
     "mnp_green_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, I, P, P, P, P, L, L, P],
     "mnp_gemm_sub_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
     "mnp_gemm_sub_batched_noalloc": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
@@ -118,6 +120,21 @@ def call(name, *args):
             raise ValueError("{}: {}".format(name, _STATUS[st]))
         raise RuntimeError("{}: HIP launch failure (status {})".format(name, st))
     return st
+
+
+class stage:
+    """roctx range around a host stage (``with stage("gmres_iter"):``), named "mnp:<name>" like
+    the library's own per-entry-point ranges."""
+
+    def __init__(self, name):
+        self._name = ("mnp:" + name).encode()
+
+    def __enter__(self):
+        lib.mnp_range_push(self._name)
+        return self
+
+    def __exit__(self, *exc):
+        lib.mnp_range_pop()
 
 
 class Handle:

@@ -22,7 +22,7 @@ import numpy as np
 import torch
 
 from . import engine
-from ._lib import call
+from ._lib import call, stage
 from .bem import _npol, _pack_rhs
 from .greenfun import CompGreenRet, CompStruct
 
@@ -74,34 +74,35 @@ class DeviceGMRES:
             g[0] = beta
             j_done = 0
             for j in range(m):
-                apply_M(self.V[j], self.Z[j])
-                w = self.V[j + 1]
-                apply_A(self.Z[j], w, rhs=None)
-                hcol = np.zeros(j + 2, complex)
-                for _ in range(2):                       # CGS2
-                    hj = self._dot(j + 1, self.V, w)
-                    self._update(j + 1, self.V, -hj, w)
-                    hcol[: j + 1] += hj
-                hn = float(torch.linalg.vector_norm(w).item())
-                hcol[j + 1] = hn
-                if hn > 0:
-                    w.div_(hn)
-                # Givens rotations G_i = [[conj(c), conj(s)], [-s, c]], c = a/rho, s = b/rho
-                for i in range(j):
-                    t = np.conj(cs[i]) * hcol[i] + np.conj(sn[i]) * hcol[i + 1]
-                    hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1]
-                    hcol[i] = t
-                den = np.sqrt(abs(hcol[j]) ** 2 + abs(hcol[j + 1]) ** 2)
-                cs[j] = hcol[j] / den if den else 1.0
-                sn[j] = hcol[j + 1] / den if den else 0.0
-                hcol[j] = den
-                hcol[j + 1] = 0
-                g[j + 1] = -sn[j] * g[j]
-                g[j] = np.conj(cs[j]) * g[j]
-                H[: j + 2, j] = hcol
-                it += 1
-                j_done = j + 1
-                hist.append(abs(g[j + 1]) / bnorm)
+                with stage("gmres_iter"):
+                    apply_M(self.V[j], self.Z[j])
+                    w = self.V[j + 1]
+                    apply_A(self.Z[j], w, rhs=None)
+                    hcol = np.zeros(j + 2, complex)
+                    for _ in range(2):                       # CGS2
+                        hj = self._dot(j + 1, self.V, w)
+                        self._update(j + 1, self.V, -hj, w)
+                        hcol[: j + 1] += hj
+                    hn = float(torch.linalg.vector_norm(w).item())
+                    hcol[j + 1] = hn
+                    if hn > 0:
+                        w.div_(hn)
+                    # Givens rotations G_i = [[conj(c), conj(s)], [-s, c]], c = a/rho, s = b/rho
+                    for i in range(j):
+                        t = np.conj(cs[i]) * hcol[i] + np.conj(sn[i]) * hcol[i + 1]
+                        hcol[i + 1] = -sn[i] * hcol[i] + cs[i] * hcol[i + 1]
+                        hcol[i] = t
+                    den = np.sqrt(abs(hcol[j]) ** 2 + abs(hcol[j + 1]) ** 2)
+                    cs[j] = hcol[j] / den if den else 1.0
+                    sn[j] = hcol[j + 1] / den if den else 0.0
+                    hcol[j] = den
+                    hcol[j + 1] = 0
+                    g[j + 1] = -sn[j] * g[j]
+                    g[j] = np.conj(cs[j]) * g[j]
+                    H[: j + 2, j] = hcol
+                    it += 1
+                    j_done = j + 1
+                    hist.append(abs(g[j + 1]) / bnorm)
                 if abs(g[j + 1]) / bnorm <= tol or it >= maxit or hn == 0:
                     break
             k = j_done

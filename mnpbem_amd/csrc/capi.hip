@@ -13,6 +13,8 @@
 #include "ctx.hpp"
 #include "../../include/mnpbem_hip.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 namespace mnp {
 template <typename OT>
 void launch_assemble(const AsmArgs&, int, const int*, const int*, const double*, const double*, void*, void*, void*,
@@ -77,6 +79,13 @@ int get_tuning(int);
 
 using namespace mnp;
 
+// roctx range over one entry point (host enqueue time; the kernels it launches carry the same
+// correlation in a marker trace). Names: mnp:<stage>.
+struct StageRange {
+    explicit StageRange(const char* name) { roctxRangePushA(name); }
+    ~StageRange() { roctxRangePop(); }
+};
+
 static int launch_status() {
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : 1000 + (int)e;
@@ -129,6 +138,17 @@ int mnp_prof_enable(int on) {
 
 // out[0] = summed GEMM kernel time (ms), out[1] = algorithmic flops (8 m n k per complex
 // GEMM), out[2] = launches, since the previous collect. Host array; waits for the events.
+int mnp_range_push(const char* name) {
+    REQUIRE(name, 1);
+    roctxRangePushA(name);
+    return 0;
+}
+
+int mnp_range_pop(void) {
+    roctxRangePop();
+    return 0;
+}
+
 int mnp_prof_collect(double* out) {
     REQUIRE(out, 1);
     prof_collect(out, out + 1, out + 2, out + 3);
@@ -140,6 +160,7 @@ int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const do
                        const double* kmat, int nref, const int* ref_row, const int* ref_col, const double* ref_g,
                        const double* ref_f, int prec, void* G1, void* G2, void* H1, void* H2, long ld, long stride,
                        void* stream) {
+    StageRange range_("mnp:assemble");
     REQUIRE(n > 0, 1);
     REQUIRE(npart > 0, 2);
     REQUIRE(nmat > 0, 3);
@@ -163,6 +184,7 @@ int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const do
 
 int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb,
                          long sB, void* C, long ldc, long sC, int batch, void* stream) {
+    StageRange range_("mnp:gemm");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(m >= 0 && n >= 0 && k >= 0, 2);
     REQUIRE(A && B && C, 5);
@@ -201,6 +223,7 @@ int mnp_noalloc_scratch_bytes(long* bytes) {
 
 int mnp_getrf_batched(int prec, int n, int batch, void* A, long lda, long sA, int* ipiv, long sP, int* info, int nb,
                       void* stream) {
+    StageRange range_("mnp:getrf");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
@@ -228,6 +251,7 @@ int mnp_perm_batched(int n, int batch, const int* ipiv, long sP, int* pf, int* q
 
 int mnp_getrs_batched(int prec, int n, int r, int batch, const void* LU, long lda, long sA, const int* pf, long sQ,
                       const void* B, long ldb, long sB, void* X, long ldx, long sX, void* stream) {
+    StageRange range_("mnp:getrs");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(r >= 1 && r <= 8, 3);
@@ -243,6 +267,7 @@ int mnp_getrs_batched(int prec, int n, int r, int batch, const void* LU, long ld
 
 int mnp_rsolve_batched(int prec, int m, int n, int batch, const void* LU, long lda, long sA, const int* qinv, long sQ,
                        void* B, long ldb, long sB, void* X, long ldx, long sX, int nb, void* stream) {
+    StageRange range_("mnp:trsm_right");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(m > 0, 2);
     REQUIRE(n > 0, 3);
@@ -259,6 +284,7 @@ int mnp_rsolve_batched(int prec, int m, int n, int batch, const void* LU, long l
 
 int mnp_gemv_batched(int prec, int n, int r, int batch, const void* A, long lda, long sA, const void* X, long ldx,
                      long sX, void* Y, long ldy, long sY, void* stream) {
+    StageRange range_("mnp:gemv");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(r >= 1 && r <= 8, 3);
@@ -310,6 +336,7 @@ int mnp_pad_batched(int prec, int n, int batch, const int* nact, void* A, long l
 int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, void* H2, void* S1, void* S2, void* D,
                       long ld, long stride, int* ipiv, int* perm, int* info, const double* nvec, long sface,
                       const double* L1, const double* L2, const double* kvac, int nb, void* stream) {
+    StageRange range_("mnp:bemret_factor");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
@@ -358,6 +385,7 @@ int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2,
                      const void* S2, long ld, long stride, const int* perm, const double* nvec, long sface,
                      const double* L1, const double* L2, const double* kvac, const void* in, long sIn, void* out,
                      long sOut, void* work, long sW, void* stream) {
+    StageRange range_("mnp:bemret_solve");
     REQUIRE(prec == 0 || prec == 1, 1);
     REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
@@ -387,6 +415,7 @@ int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const voi
                      long stride, const double* nvec, long sface, const double* L1, const double* L2,
                      const double* kvac, const void* x, long sx, const void* rhs, long sr, void* y, long sy, void* work,
                      long sW, double* norms, void* stream) {
+    StageRange range_("mnp:refine_operator");
     REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1, 2);
     REQUIRE(G1 && G2 && H1 && H2, 3);
@@ -406,6 +435,7 @@ int mnp_bem_operator(int n, int batch, const void* G1, const void* G2, const voi
 int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, const int* side, long sface,
                       const double* kmed, const double* kvac, const double* L1, const double* L2, const double* pol,
                       const double* dir, void* out, long so, void* stream) {
+    StageRange range_("mnp:planewave");
     REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1, 2);
     REQUIRE(pos && nvec && side && kmed && kvac && L1 && L2, 3);
@@ -422,6 +452,7 @@ int mnp_planewave_rhs(int n, int batch, const double* pos, const double* nvec, c
 int mnp_farfield(int n, int batch, const double* pos, const double* area, const int* side, long sface,
                  const double* kmed, const double* kvac, int ndir, const double* dirs, const void* x, long sx, void* out,
                  void* stream) {
+    StageRange range_("mnp:farfield");
     REQUIRE(n > 0, 1);
     REQUIRE(batch >= 1 && batch <= 65535, 2);
     REQUIRE(pos && area && side && kmed && kvac, 3);
@@ -500,6 +531,7 @@ static bool geom_ok(const mnp_green_geom* g) {
 
 int mnp_hmat_fill_dense(const mnp_green_geom* g, const int* c2p, int nd, const int* dblk, const long* doff, void* D,
                         long dstride, void* stream) {
+    StageRange range_("mnp:hmat_dense");
     REQUIRE(geom_ok(g), 1);
     REQUIRE(c2p, 2);
     REQUIRE(nd >= 0, 3);
@@ -511,6 +543,7 @@ int mnp_hmat_fill_dense(const mnp_green_geom* g, const int* c2p, int nd, const i
 
 int mnp_hmat_aca(const mnp_green_geom* g, const int* c2p, int kind, int nlr, const int* lblk, const long* uws,
                  const long* vws, void* U, void* V, int* rank, double htol, int kmax, void* stream) {
+    StageRange range_("mnp:aca");
     REQUIRE(geom_ok(g), 1);
     REQUIRE(c2p, 2);
     REQUIRE(kind >= 0 && kind < 4, 3);
@@ -538,6 +571,7 @@ static bool hmat_ok(const mnp_hmat* h) {
 }
 
 int mnp_hmat_matvec(const mnp_hmat* h, int kind, int r, const void* x, long ldx, void* y, long ldy, void* stream) {
+    StageRange range_("mnp:hmat_matvec");
     REQUIRE(hmat_ok(h), 1);
     REQUIRE(kind >= 0 && kind < 4, 2);
     REQUIRE(r >= 1 && r <= HM_MAXR, 3);
@@ -552,6 +586,7 @@ int mnp_hmat_matvec(const mnp_hmat* h, int kind, int r, const void* x, long ldx,
 int mnp_bem_operator_hmat(const mnp_hmat* h, const double* nvec, const double* L1, const double* L2,
                           const double* kvac, const void* x, const void* rhs, void* y, void* work, long sW,
                           double* norms, void* stream) {
+    StageRange range_("mnp:gmres_operator");
     REQUIRE(hmat_ok(h), 1);
     REQUIRE(nvec && L1 && L2 && kvac, 2);
     REQUIRE(x, 6);
@@ -567,6 +602,7 @@ int mnp_bem_operator_hmat(const mnp_hmat* h, const double* nvec, const double* L
 // ------------------------------------------------------------------------- near-field set-up
 int mnp_near_terms(int npair, const int* rows, const int* cols, const long* slot, const double* pos, const int* qptr,
                    const double* qpts, const double* qw, int order, double* g, double* f, void* stream) {
+    StageRange range_("mnp:near_terms");
     REQUIRE(npair >= 0, 1);
     REQUIRE(npair == 0 || (rows && cols && slot), 2);
     REQUIRE(pos && qptr && qpts && qw, 5);
@@ -578,6 +614,7 @@ int mnp_near_terms(int npair, const int* rows, const int* cols, const long* slot
 
 int mnp_colsum_far(int nloc, const int* loc, int nind, const int* ind, const double* pos, const double* nvec,
                    const double* area, double* out, void* stream) {
+    StageRange range_("mnp:colsum_far");
     REQUIRE(nloc >= 0, 1);
     REQUIRE(nloc == 0 || loc, 2);
     REQUIRE(nind >= 0, 3);
@@ -591,6 +628,7 @@ int mnp_colsum_far(int nloc, const int* loc, int nind, const int* ind, const dou
 // ------------------------------------------------------------------------- warm-start MLP
 int mnp_mlp_gemm(int ta, int tb, int epi, int M, int N, int K, const float* A, long lda, const float* B, long ldb,
                  const float* bias, float* Z, float* H, long ldc, void* stream) {
+    StageRange range_("mnp:mlp_gemm");
     REQUIRE(ta == 0 || ta == 1, 1);
     REQUIRE(tb == 0 || tb == 1, 2);
     REQUIRE(epi >= 0 && epi <= 3 && !(epi >= 2 && (ta || tb)), 3);
@@ -660,6 +698,7 @@ int mnp_get_tuning(int key, int* value) {
 // ------------------------------------------------------------------------- surface potentials / fields
 int mnp_surface_eval(const mnp_green_geom* g, int outside, double k0, const void* sig, const void* h, void* out,
                      void* stream) {
+    StageRange range_("mnp:surface_eval");
     REQUIRE(geom_ok(g), 1);
     REQUIRE(outside == 0 || outside == 1, 2);
     REQUIRE(k0 > 0, 3);
@@ -695,6 +734,7 @@ int mnp_green_pick(const mnp_green_geom* g, int outside, int key, const long* in
 // ------------------------------------------------------------------------- fields at points
 int mnp_point_eval(const mnp_point_geom* g, double k0, const void* sig1, const void* h1, const void* sig2,
                    const void* h2, void* out, void* stream) {
+    StageRange range_("mnp:point_eval");
     REQUIRE(g && g->npt >= 0 && g->nface > 0 && g->npart > 0 && g->order >= 0 && g->order < 8, 1);
     REQUIRE(g->ppos && g->pmed && g->fpos && g->farea && g->fpart && g->inout && g->kmat && g->rowptr, 1);
     REQUIRE(k0 > 0, 2);

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import call
+from ._lib import call, stage
 
 C128 = torch.complex128
 C64 = torch.complex64
@@ -297,9 +297,10 @@ class RetSolverBatch:
             prev = rel
             act = (~done) & (~stalled)
             active[:nl] = torch.tensor(act.astype(np.int32), device=rhs.device)
-            call("mnp_cast_batched", 1, 8 * n, nl, _p(r), 8 * n, _p(r32), 8 * n, stream_handle())
-            self._chain(r32, dx)
-            call("mnp_accum_batched", 1, 8 * n, nl, _p(dx), 8 * n, _p(x), 8 * n, _p(active), stream_handle())
+            with stage("refine"):
+                call("mnp_cast_batched", 1, 8 * n, nl, _p(r), 8 * n, _p(r32), 8 * n, stream_handle())
+                self._chain(r32, dx)
+                call("mnp_accum_batched", 1, 8 * n, nl, _p(dx), 8 * n, _p(x), 8 * n, _p(active), stream_handle())
         self.stats = dict(relres=hist[-1], history=np.array(hist), iterations=len(hist) - 1,
                           fallback=np.nonzero(~(hist[-1] <= self.refine_tol))[0])
         return x

@@ -134,6 +134,18 @@ def test_cabi_rejects_bad_arguments_without_gpu_work():
     assert _lib.lib.mnp_rsolve_batched(0, 4, 4, 1, 1, 4, 16, 1, 4, 5, 4, 16, 5, 4, 16, 32, None) == -13  # B == X
 
 
+def test_trace_ranges_without_gpu_work():
+    """roctx ranges of host stages nest and pop without a profiler attached; a null name is
+    rejected."""
+    from mnpbem_amd import _lib
+    with _lib.stage("gmres_iter"):
+        with _lib.stage("refine"):
+            pass
+    assert _lib.lib.mnp_range_push(None) == -1
+    assert _lib.lib.mnp_range_push(b"mnp:test") == 0
+    assert _lib.lib.mnp_range_pop() == 0
+
+
 def test_planewave_field_at_points_and_faces():
     """PlaneWaveRet.field (planewave_ret.py:157-240): e = pol e^{ik dir.r}, h = n_b dir x e in
     the excitation medium, zero elsewhere; for faces the chosen side selects them."""
