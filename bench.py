@@ -32,6 +32,21 @@ DRUDE = (10.0, 9.065, 0.0708)
 ROD_N = [25, 11, 34]
 
 
+def _gemm_kernel(tuning, split):
+    """Name of the complex-float GEMM kernel that the mixed chain's wide launches run."""
+    split = int(tuning.get("gemm_split", split))
+    if split == 0:
+        return "k_gemm_sub_c64v (complex-float GEMM, FP32 matrix cores)"
+    tile = int(tuning.get("plane_tile", 11))
+    mt, nt = {11: (1, 1), 111: (1, 1), 21: (2, 1), 12: (1, 2), 22: (2, 2)}[tile]
+    if split == 16:
+        return "k_gemm_sub_c64h<{}, {}> (complex GEMM on the FP16 matrix cores, scaled fp16 pairs)".format(mt, nt)
+    if tile == 11:
+        return "k_gemm_sub_c64p<{}> (complex GEMM on the BF16 matrix cores, {} operand terms)".format(split, split)
+    return "k_gemm_sub_c64q<{}, {}, {}> (complex GEMM on the BF16 matrix cores, {} operand terms)".format(
+        split, mt, nt, split)
+
+
 def _progress(msg):
     """One line on stderr per finished stage (the JSON result is the only stdout line)."""
     print("bench: " + msg, file=sys.stderr, flush=True)
@@ -173,84 +188,90 @@ def _dimer25k():
                          interp="curv")
 
 
-def warmstart25k(nlam=11, seed=7):
-    """Learned warm start on the rod dimer: the MLP is trained on direct solves of 12 seeded
-    rods and ellipsoids (0.5k-3k faces, 6 wavelengths each, 450-950 nm), then GMRES(50) to 1e-6
-    with H-matrices runs at nlam wavelengths over 600-900 nm from the zero start, the previous
-    wavelength's solution and the learned start. The preconditioner (complex64 BEMRet chain) is
-    factored once at 750 nm and reused across the sweep. Times: GMRES plus, for the learned
-    start, feature extraction and inference."""
+def _training_particles(rng, count):
+    """Seeded capped rods (diameter 8-20 nm, aspect 2.5-6) and prolate ellipsoids (semi-axes
+    5-10 nm, aspect 1-4) alternating, 2.0k-2.9k curved triangles each, gold Drude in vacuum."""
+    import mnpbem_amd as M
+    drude = M.EpsDrude(10, 9.065, 0.0708)
+    cps = []
+    for i in range(count):
+        if i % 2 == 0:
+            d = float(rng.uniform(8, 20))
+            L = float(rng.uniform(2.5, 6.0)) * d
+            n = [int(rng.integers(24, 31)), 9, int(rng.integers(30, 35))]
+            part = M.trirod(d, L, n, triangles=True)
+        else:
+            a = float(rng.uniform(5, 10))
+            part = M.triellipsoid(int(rng.choice([1024, 1225, 1444])), [a, a, a * float(rng.uniform(1, 4))])
+        cps.append(M.ComParticle([M.EpsConst(1.0), drude], [part], [[2, 1]], 1, interp="curv"))
+    return cps
+
+
+def warmstart25k(nlam=31, nparticles=40, lams_per_particle=100, seed=7):
+    """Learned warm start on the rod dimer: the MLP is trained (forward, backward and Adam on the
+    device) on nparticles x lams_per_particle direct solves (mixed precision device sweeps) of
+    seeded rods and ellipsoids with 2.0k-2.9k faces at wavelengths drawn uniformly over
+    450-950 nm; then GMRES(50) to 1e-6 with H-matrices runs at nlam wavelengths over 600-900 nm
+    from the zero start, the previous wavelength's solution, the residual-minimising combination
+    of the last three solutions ('extrapolate'), the learned guess, and the learned guess as a
+    fourth candidate of that combination. The preconditioner (complex64 BEMRet chain) is factored
+    once at 750 nm and applied through its explicit inverses. Times: GMRES including each start's
+    operator applications, plus feature extraction and inference for the learned starts."""
     import torch
     import mnpbem_amd as M
     from mnpbem_amd import warmstart as W
     from mnpbem_amd.bem_iter import BEMRetIter
     pol, direc = [0, 0, 1.0], [1.0, 0, 0]
-    drude = M.EpsDrude(10, 9.065, 0.0708)
     rng = np.random.default_rng(seed)
-    cps = []
-    for i in range(12):
-        if i % 2 == 0:
-            d = float(rng.uniform(8, 20))
-            L = float(rng.uniform(2.5, 6.0)) * d
-            part = M.trirod(d, L, [int(rng.integers(20, 36)), 9, int(rng.integers(14, 40))], triangles=True)
-        else:
-            a = float(rng.uniform(5, 10))
-            part = M.triellipsoid(int(rng.choice([400, 576, 784, 1024])), [a, a, a * float(rng.uniform(1, 4))])
-        cps.append(M.ComParticle([M.EpsConst(1.0), drude], [part], [[2, 1]], 1, interp="curv"))
+    cps = _training_particles(rng, nparticles)
+    lams_train = [np.sort(rng.uniform(450, 950, lams_per_particle)) for _ in cps]
     t0 = time.perf_counter()
-    X, T = W.training_set(cps, rng.uniform(450, 950, 6), pol, direc)
+    X, T = W.training_set(cps, lams_train, pol, direc)
+    torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
+    _progress("warm start: {} training solves in {:.1f} s".format(nparticles * lams_per_particle, t_data))
     net = W.WarmStartMLP(seed=0)
     t0 = time.perf_counter()
-    hist = net.fit(X, T, steps=1500, batch=8192, lr=2e-3)
+    hist = net.fit(X, T, steps=3000, batch=16384, lr=2e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    _progress("warm start: trained in {:.1f} s, loss {:.4f} -> {:.4f}".format(t_train, hist[0], hist[-1]))
     cp = _dimer25k()
     g = M.CompGreenRet(cp, cp)
     exc = M.PlaneWaveRet(pol, direc)
     H = W.mean_curvature(cp)
     it = BEMRetIter(cp, tol=1e-6, maxit=400, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5,
                     green=g, precond_wavelength=750.0)
-    modes = ("zero", "previous", "learned", "previous+learned")
-    res = {m: dict(iterations=[], time_s=0.0, initial_residual=[]) for m in modes}
-    prev = {}
-    precond_s = 0.0
-    for lam in np.linspace(600, 900, nlam):
-        e = exc(cp, lam)
-        for m in modes:
-            t0 = time.perf_counter()
-            x0 = None
-            if m == "previous":
-                x0 = prev.get(m)
-            elif m in ("learned", "previous+learned"):
-                xl = W.guess(cp, net.predict(W.features(cp, lam, pol, direc, curvature=H)), pol)
-                # candidates judged by the residual: the start is their best combination
-                x0 = [xl] if m == "learned" else [prev.get(m), xl]
-            t_guess = time.perf_counter() - t0
-            sig, _ = it.solve(e, x0=x0)
-            if m == modes[0]:
-                precond_s += it.timings.get("precond", 0.0)
-            flag, relres, niter = it.info()
-            r = res[m]
-            r["iterations"].append(int(niter))
-            r["initial_residual"].append(round(float(it.history[0]), 4))
-            r["time_s"] += it.timings["gmres"] + t_guess
-            r["converged"] = r.get("converged", True) and flag == 0
-            prev[m] = sig
+    learned = lambda lam: W.guess(cp, net.predict(W.features(cp, lam, pol, direc, curvature=H)), pol)
+    modes = ("zero", "previous", "extrapolate", "learned", "extrapolate+learned")
+    lams = np.linspace(600, 900, nlam)
+    res = it.solve_spectrum(exc, lams, start=modes, history=3, warm_start=learned)
+    precond_s = round(float(it.timings.get("precond_build", 0.0)), 3)
+    out = {}
     for m in modes:
-        res[m]["time_s"] = round(res[m]["time_s"], 3)
-        res[m]["total_iterations"] = int(sum(res[m]["iterations"]))
+        r = res[m]
+        out[m] = dict(iterations=r["iterations"], total_iterations=int(sum(r["iterations"])),
+                      time_s=round(float(sum(r["time_s"])), 3),
+                      initial_residual=[round(v, 4) for v in r["initial_residual"]],
+                      converged=all(f == 0 for f in r["flag"]))
+    ext_spread = float(max(np.max(np.abs(res[m]["ext"] - res["zero"]["ext"]) / np.abs(res["zero"]["ext"]))
+                           for m in modes))
     del it, g
     torch.cuda.empty_cache()
+    t = {m: out[m]["time_s"] for m in modes}
     return dict(workload="rod dimer n={}, {} wavelengths 600-900 nm, GMRES(50) to 1e-6, H-matrices, complex64 "
-                         "BEMRet-chain preconditioner factored once at 750 nm".format(cp.nfaces, nlam),
-                training=dict(solves=len(cps) * 6, samples=int(X.shape[0]), data_s=round(t_data, 2),
-                              train_s=round(t_train, 2), steps=1500, loss_first=round(hist[0], 4),
-                              loss_last=round(hist[-1], 4)),
-                precond_once_s=round(precond_s, 3), starts=res,
-                speedup_learned_vs_zero=round(res["zero"]["time_s"] / max(res["learned"]["time_s"], 1e-9), 3),
-                speedup_learned_vs_previous=round(res["previous"]["time_s"] /
-                                                  max(res["previous+learned"]["time_s"], 1e-9), 3))
+                         "BEMRet-chain preconditioner factored once at 750 nm, applied through explicit "
+                         "inverses".format(cp.nfaces, nlam),
+                training=dict(solves=nparticles * lams_per_particle, particles=nparticles,
+                              faces=[int(min(c.nfaces for c in cps)), int(max(c.nfaces for c in cps))],
+                              samples=int(X.shape[0]), data_s=round(t_data, 2), train_s=round(t_train, 2),
+                              steps=3000, loss_first=round(hist[0], 4), loss_last=round(hist[-1], 4)),
+                precond_once_s=precond_s, starts=out, ext_max_rel_spread_between_starts=ext_spread,
+                speedup_learned_vs_zero=round(t["zero"] / max(t["learned"], 1e-9), 3),
+                speedup_learned_vs_previous=round(t["previous"] / max(t["learned"], 1e-9), 3),
+                speedup_extrapolate_vs_zero=round(t["zero"] / max(t["extrapolate"], 1e-9), 3),
+                speedup_extrapolate_learned_vs_extrapolate=round(t["extrapolate"] /
+                                                                 max(t["extrapolate+learned"], 1e-9), 3))
 
 
 def sphere_mesh(nfaces, diameter):
@@ -386,7 +407,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0, help="wavelengths per factorisation (0: all 201 at once)")
-    ap.add_argument("--gemm-split", type=int, default=3, choices=(0, 2, 3),
+    ap.add_argument("--gemm-split", type=int, default=3, choices=(0, 2, 3, 16),
                     help="bfloat16 terms per operand in the complex-float GEMMs (0: FP32 matrix cores)")
     ap.add_argument("--no-mixed2", action="store_true", help="skip the extra 2-term mixed sweep")
     ap.add_argument("--pmc-calibration", action="store_true",
@@ -463,8 +484,8 @@ def main():
     # cores the kernel runs on (BF16 for the split operands, FP32 otherwise)
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     # the split-BF16 kernel executes terms (terms + 1) / 2 BF16 products per real product
-    products = {0: 1, 2: 3, 3: 6}[args.gemm_split]
-    peak = PEAK_BF16_TFLOPS if args.gemm_split else PEAK_FP32_TFLOPS
+    products = {0: 1, 2: 3, 3: 6, 16: 3}[args.gemm_split]
+    peak = PEAK_BF16_TFLOPS if args.gemm_split else PEAK_FP32_TFLOPS    # FP16 and BF16 dense peaks are equal
 
     dt_mixed, out_mixed = results["mixed"]
     value = args.steps * len(lams) / dt_mixed
@@ -475,26 +496,32 @@ def main():
         rel = np.max(np.abs(out_mixed["ext"] - out_d["ext"]) / np.abs(out_d["ext"]))
         extra = dict(fp64_solves_per_s=round(v_d, 3), speedup_vs_fp64=round(value / v_d, 3),
                      ext_rel_err_vs_fp64=float(rel))
-    if not args.no_mixed2 and args.gemm_split != 2 and "double" in results:
-        # the same sweep with 2-term operands (about 16 significant bits in the GEMMs); the
-        # 3-term solver's buffers go first (both would not fit at batch 201)
+    # the same sweep with the other GEMM operand forms: 2 bfloat16 terms (about 16 significant
+    # bits) and scaled fp16 pairs (about 22 bits); the main solver's buffers go first (two would
+    # not fit at batch 201)
+    for other, tag in ((2, "mixed_2term"), (16, "mixed_f16pair")):
+        if args.no_mixed2 or args.gemm_split == other or "double" not in results:
+            continue
         solvers["mixed"]._solver = None
         torch.cuda.empty_cache()
-        b2 = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g, gemm_split=2)
+        b2 = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g, gemm_split=other)
         b2.solve_spectrum(exc, lams, precision="mixed")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out2 = b2.solve_spectrum(exc, lams, precision="mixed")
         torch.cuda.synchronize()
         dt2 = time.perf_counter() - t0
-        extra["mixed_2term_solves_per_s"] = round(len(lams) / dt2, 3)
-        extra["mixed_2term_speedup_vs_fp64"] = round(len(lams) / dt2 / extra["fp64_solves_per_s"], 3)
-        extra["mixed_2term_ext_rel_err_vs_fp64"] = float(np.max(np.abs(out2["ext"] - results["double"][1]["ext"]) /
-                                                                np.abs(results["double"][1]["ext"])))
-        extra["mixed_2term_refine_iterations_max"] = int(max(out2.get("refine_iterations", [0]) or [0]))
+        extra[tag + "_solves_per_s"] = round(len(lams) / dt2, 3)
+        extra[tag + "_speedup_vs_fp64"] = round(len(lams) / dt2 / extra["fp64_solves_per_s"], 3)
+        extra[tag + "_ext_rel_err_vs_fp64"] = float(np.max(np.abs(out2["ext"] - results["double"][1]["ext"]) /
+                                                           np.abs(results["double"][1]["ext"])))
+        extra[tag + "_refine_iterations_max"] = int(max(out2.get("refine_iterations", [0]) or [0]))
+        extra[tag + "_fp64_fallbacks"] = len(out2.get("fallback", []))
+        extra[tag + "_lu_tuning"] = b2.tuning
         b2._solver = None
         del b2
-        _progress("2-term mixed sweep: {:.1f} solves/s".format(extra["mixed_2term_solves_per_s"]))
+        torch.cuda.empty_cache()
+        _progress("{} sweep: {:.1f} solves/s".format(tag, extra[tag + "_solves_per_s"]))
         torch.cuda.empty_cache()
     flops_per_solve = 34.67 * n ** 3
     extra["path_tflops_mixed"] = round(value * flops_per_solve / 1e12, 3)
@@ -518,16 +545,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 LU ({}) + c128 refinement".format(
-            {0: "FP32 MFMA", 2: "2-term split-BF16 MFMA", 3: "3-term split-BF16 MFMA, FP32-equivalent"}[args.gemm_split]),
+            {0: "FP32 MFMA", 2: "2-term split-BF16 MFMA", 3: "3-term split-BF16 MFMA, FP32-equivalent",
+             16: "scaled fp16-pair MFMA"}[args.gemm_split]),
         "data": "synthetic",
         "config": {"workload": "BEMRet gold-Drude capped rod 10x40 nm, n={} curved triangles, eps_out=1.77, "
                                "axial plane wave, 201 wavelengths 500-1000 nm, batch {}".format(n, args.batch),
                    "faces": n, "wavelengths": len(lams), "batch": args.batch,
                    "lu_tuning_mixed": solvers_tuning.get("mixed"), "lu_tuning_fp64": solvers_tuning.get("double")},
         "roofline": {"bound": "mfma",
-                     "kernel": ("k_gemm_sub_c64p<{}> (complex GEMM on the BF16 matrix cores, {} operand terms)"
-                                .format(args.gemm_split, args.gemm_split) if args.gemm_split else
-                                "k_gemm_sub_c64v (complex-float GEMM, FP32 matrix cores)"),
+                     "kernel": _gemm_kernel(solvers_tuning.get("mixed") or {}, args.gemm_split),
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
                      "work": "algorithmic 8 m n k real flops per launch (complex FP32-equivalent)",

@@ -175,7 +175,9 @@ int mnp_pad_batched(int prec, int n, int batch, const int* nact, void* A, long l
 
 /* BEMRet.solve (bem_ret.py:1569-1831, Eqs. 19-20): in = [phi a De alpha] (n x 8 per
  * wavelength, precision prec) -> out = [sig1 h1 sig2 h2] (n x 8); work: 24 n complex per
- * wavelength. Factors from mnp_bemret_factor. */
+ * wavelength. Factors from mnp_bemret_factor. perm NULL: G1, G2 and S2 hold the explicit
+ * inverses G1^{-1}, G2^{-1}, Sigma^{-1} (from mnp_rsolve_batched of the identity), applied as
+ * products; for a factorisation reused by many solves (a fixed GMRES preconditioner). */
 int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2, const void* S1, const void* D,
                      const void* S2, long ld, long stride, const int* perm, const double* nvec, long sface,
                      const double* L1, const double* L2, const double* kvac, const void* in, long sIn, void* out,

@@ -34,12 +34,25 @@ def _compile(src):
     return obj
 
 
-def build(verbose=False):
-    """Compile every HIP source and link the shared library. Returns its path."""
+def _stale(src):
+    """True when src or a header is newer than its object."""
+    obj = os.path.join(OBJ, os.path.basename(src)[:-4] + ".o")
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(HERE, "..", "include", "*.h"))
+    return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(obj)
+
+
+def build(verbose=False, changed_only=False):
+    """Compile every HIP source (``changed_only``: those newer than their objects) and link the
+    shared library. Returns its path."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    todo = [s_ for s_ in srcs if _stale(s_)] if changed_only else srcs
+    if todo:
+        with cf.ThreadPoolExecutor(max_workers=min(8, len(todo))) as ex:
+            list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJ, os.path.basename(s_)[:-4] + ".o") for s_ in srcs]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + LIBS
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -52,4 +65,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import sys
+    build(verbose=True, changed_only="--changed" in sys.argv)

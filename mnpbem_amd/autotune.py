@@ -17,14 +17,16 @@ import torch
 from ._lib import call
 
 _CACHE = {}
-_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4, "asm_rows": 7, "asm_lchunk": 8}
+_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4, "plane_tile": 5, "asm_rows": 7, "asm_lchunk": 8}
 
 
 def _value(key, v):
     """Library value of a knob: gemm_split 2 or 3 (bfloat16 terms per operand) selects the
-    pre-split plane GEMM (library modes 12, 13), anything else the FP32 matrix-core GEMM."""
+    pre-split plane GEMM (library modes 12, 13), 16 the scaled fp16 pairs (library mode 16),
+    anything else the FP32 matrix-core GEMM."""
     if key == "gemm_split":
-        return 10 + int(v) if int(v) in (2, 3) else 0
+        v = int(v)
+        return 10 + v if v in (2, 3) else (16 if v == 16 else 0)
     return int(v)
 
 
@@ -36,8 +38,8 @@ def _get(key):
 
 @contextlib.contextmanager
 def knobs(cfg):
-    """Set the library's knobs from cfg (keys 'nbo', 'lookahead', 'gemm_split', 'asm_rows',
-    'asm_lchunk') for the duration."""
+    """Set the library's knobs from cfg (keys 'nbo', 'lookahead', 'gemm_split', 'plane_tile',
+    'asm_rows', 'asm_lchunk') for the duration."""
     if not cfg:
         yield
         return
@@ -92,16 +94,30 @@ def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):
         return 1e3 * min(times)
 
     best = None
+    tried = []
     for cfg in candidates(n):
         cfg = dict(cfg, gemm_split=gemm_split)
+        if gemm_split in (2, 3, 16) and prec == 1:
+            cfg["plane_tile"] = 11
         ms = timed(cfg)
+        tried.append((ms, cfg))
         if best is None or ms < best["ms"]:
             best = dict(cfg, ms=round(ms, 3))
-    if gemm_split == 3 and prec == 1:
+    if gemm_split in (2, 3, 16) and prec == 1:
+        # the plane kernels with direct LDS copies (64 x 64, 128 x 64, 64 x 128 tiles) on the two
+        # fastest shapes (the fp16 pairs always copy directly: 11 and 111 are the same kernel)
+        for _, cfg in sorted(tried, key=lambda e: e[0])[:2]:
+            for tile in ((21, 12) if gemm_split == 16 else (111, 21, 12)):
+                c2 = dict(cfg, plane_tile=tile)
+                ms = timed(c2)
+                if ms < best["ms"]:
+                    best = dict(c2, ms=round(ms, 3))
+    if gemm_split in (3, 16) and prec == 1:
         # the FP32 matrix cores give the same precision: keep them where they are faster
         # (small matrices, where the split passes do not pay off)
         alt = dict(best, gemm_split=0)
         alt.pop("ms")
+        alt.pop("plane_tile", None)
         ms = timed(alt)
         if ms < best["ms"]:
             best = dict(alt, ms=round(ms, 3))

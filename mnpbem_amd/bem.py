@@ -73,10 +73,11 @@ class BEMRet:
         self.autotune = op.get("autotune", None)
         # complex-float GEMMs of the mixed chain: operands split into 3 bfloat16 terms (as
         # accurate as FP32, on the BF16 matrix cores; default), 2 terms (about 16 bits; faster,
-        # the FP64 refinement takes the accuracy back), or 0 (FP32 matrix cores)
+        # the FP64 refinement takes the accuracy back), 16 (scaled fp16 pairs on the FP16 matrix
+        # cores, about 22 bits), or 0 (FP32 matrix cores)
         self.gemm_split = int(op.get("gemm_split", 3))
-        if self.gemm_split not in (0, 2, 3):
-            raise ValueError("BEMRet: gemm_split must be 0, 2 or 3")
+        if self.gemm_split not in (0, 2, 3, 16):
+            raise ValueError("BEMRet: gemm_split must be 0, 2, 3 or 16")
         self.tuning = None
         self.g = op.get("green") or CompGreenRet(p, p, **{k: v for k, v in op.items()
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
@@ -164,10 +165,13 @@ class BEMRet:
         return self
 
     # ------------------------------------------------------------------ device sweep
-    def solve_spectrum(self, exc, enei_list, scattering=False, batch=None, precision=None, timings=None):
+    def solve_spectrum(self, exc, enei_list, scattering=False, batch=None, precision=None, timings=None,
+                       solutions=False):
         """Extinction (and scattering) of a plane wave over ``enei_list`` with every stage
         on the device, ``batch`` wavelengths per factorisation. Returns a dict with 'ext'
-        (nlam, npol), optionally 'sca', and refinement statistics."""
+        (nlam, npol), optionally 'sca', refinement statistics and, with ``solutions``, the
+        surface charges and currents as 'solutions' (nlam, npol, 8, n) complex128 on the host,
+        packed [sig1, h1 (x, y, z), sig2, h2 (x, y, z)] per face."""
         enei_list = np.atleast_1d(np.asarray(enei_list, float))
         batch = int(batch or self.batch)
         precision = precision or self.precision
@@ -180,6 +184,7 @@ class BEMRet:
         ext = np.zeros((len(enei_list), npol))
         sca = np.zeros((len(enei_list), npol)) if scattering else None
         refine_iters, relres, fallback = [], [], []
+        sols = np.zeros((len(enei_list), npol, 8, geo.n), complex) if solutions else None
         sp = exc.spec
         for b0 in range(0, len(enei_list), batch):
             el = enei_list[b0:b0 + batch]
@@ -199,6 +204,8 @@ class BEMRet:
                         fb = s.stats["fallback"]
                         fallback.extend(list(el[fb]))
                         x = self._redo_double(el, fb, rhs, x)
+                if solutions:
+                    sols[b0:b0 + nl, ip] = x[:nl].cpu().numpy()
                 ff = engine.farfield(geo, x, s.kvac, kmed, exc.dir[ip:ip + 1], exc.medium, nl)
                 e = ff[:, 0, :3].cpu().numpy()
                 ext[b0:b0 + nl, ip] = np.real(4 * np.pi / kmed_np * np.imag(e @ np.conj(exc.pol[ip])))
@@ -209,6 +216,8 @@ class BEMRet:
                     nb = np.real(np.sqrt(eps_med))
                     sca[b0:b0 + nl, ip] = (dsca @ sp.area) / (0.5 * nb)
         out = dict(ext=ext, sca=sca)
+        if solutions:
+            out["solutions"] = sols
         if precision == "mixed":
             out.update(refine_iterations=refine_iters, relres=relres, fallback=fallback)
         return out

@@ -54,6 +54,15 @@ class ComParticle:
                 if self.closed[abs(i) - 1] is None:
                     self.closed[abs(i) - 1] = list(idx)
         self.pc = self.p[0] if len(self.p) == 1 else self.p[0].vertcat(*self.p[1:])
+        # integration options (bemoptions 'rule', 'npol', 'refine'): set on the particles after
+        # their geometry, so that they change the integration points of the refined entries and
+        # not the element areas, as the reference does (geometry/comparticle.py:144-161, :176-186)
+        if any(k in kwargs for k in ("rule", "npol", "refine")):
+            from .quadrature import QuadFace
+            q = QuadFace(rule=kwargs.get("rule", 18), npol=kwargs.get("npol", (7, 5)),
+                         refine=kwargs.get("refine", None))
+            for part in self.p + [self.pc]:
+                part.quad_rule = q
         self._offsets = np.concatenate([[0], np.cumsum([q.nfaces for q in self.p])]).astype(np.int64)
 
     # ---------------- geometry delegated to the concatenated particle ----------------

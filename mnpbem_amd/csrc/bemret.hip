@@ -528,12 +528,21 @@ void solve_chain(const Face& f, int batch, const void* G1, const void* G2, const
     hipLaunchKernelGGL(k_glue2<T>, g, dim3(256), 0, s, f, in, sIn, W, sW, AP, sW);
     gemv_batched<T>(n, 3, batch, Dinv, ld, stride, AP, N, sW, U, N, sW, s);
     hipLaunchKernelGGL(k_glue3<T>, g, dim3(256), 0, s, f, AP, sW, U, sW, R, sW);
-    getrs_batched<T>(n, 1, batch, Sg, ld, stride, pfS, sQ, R, N, sW, SG, N, sW, s);
+    // pivots null: G1, G2 and Sg hold explicit inverses (products instead of substitutions)
+    if (pfS)
+        getrs_batched<T>(n, 1, batch, Sg, ld, stride, pfS, sQ, R, N, sW, SG, N, sW, s);
+    else
+        gemv_batched<T>(n, 1, batch, Sg, ld, stride, R, N, sW, SG, N, sW, s);
     hipLaunchKernelGGL(k_glue4<T>, g, dim3(256), 0, s, f, AP, sW, SG, sW, TT, sW);
     gemv_batched<T>(n, 3, batch, Dinv, ld, stride, TT, N, sW, H, N, sW, s);
     hipLaunchKernelGGL(k_glue5<T>, g, dim3(256), 0, s, f, in, sIn, SG, sW, H, sW, P1, P2, sW);
-    getrs_batched<T>(n, 4, batch, G1, ld, stride, pf1, sQ, P1, N, sW, out, N, sOut, s);
-    getrs_batched<T>(n, 4, batch, G2, ld, stride, pf2, sQ, P2, N, sW, out + 4 * N, N, sOut, s);
+    if (pf1) {
+        getrs_batched<T>(n, 4, batch, G1, ld, stride, pf1, sQ, P1, N, sW, out, N, sOut, s);
+        getrs_batched<T>(n, 4, batch, G2, ld, stride, pf2, sQ, P2, N, sW, out + 4 * N, N, sOut, s);
+    } else {
+        gemv_batched<T>(n, 4, batch, G1, ld, stride, P1, N, sW, out, N, sOut, s);
+        gemv_batched<T>(n, 4, batch, G2, ld, stride, P2, N, sW, out + 4 * N, N, sOut, s);
+    }
 }
 
 template void solve_chain<double>(const Face&, int, const void*, const void*, const void*, const void*, const void*,

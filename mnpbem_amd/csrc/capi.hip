@@ -390,7 +390,6 @@ int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2,
     REQUIRE(n > 0, 2);
     REQUIRE(batch >= 1, 3);
     REQUIRE(G1 && G2 && S1 && D && S2, 4);
-    REQUIRE(perm, 11);
     REQUIRE(nvec && L1 && L2 && kvac, 12);
     REQUIRE(sface == 0 || sface >= n, 13);
     REQUIRE(in && out && work, 17);
@@ -399,12 +398,15 @@ int mnp_bemret_solve(int prec, int n, int batch, const void* G1, const void* G2,
     const long PB = (long)batch * n;
     Face f{n, nvec, L1, L2, kvac, sface};
     hipStream_t s = (hipStream_t)stream;
+    // perm null: G1, G2 and S2 are the explicit inverses of G1, G2 and Sigma
+    const int* pf1 = perm ? perm : nullptr;
+    const int* pf2 = perm ? perm + 2 * PB : nullptr;
+    const int* pfS = perm ? perm + 6 * PB : nullptr;
     if (prec == 0)
-        solve_chain<double>(f, batch, G1, G2, S1, D, S2, ld, stride, perm, perm + 2 * PB, perm + 6 * PB, n, in, sIn,
-                            out, sOut, work, sW, s);
+        solve_chain<double>(f, batch, G1, G2, S1, D, S2, ld, stride, pf1, pf2, pfS, n, in, sIn, out, sOut, work, sW,
+                            s);
     else
-        solve_chain<float>(f, batch, G1, G2, S1, D, S2, ld, stride, perm, perm + 2 * PB, perm + 6 * PB, n, in, sIn,
-                           out, sOut, work, sW, s);
+        solve_chain<float>(f, batch, G1, G2, S1, D, S2, ld, stride, pf1, pf2, pfS, n, in, sIn, out, sOut, work, sW, s);
     return launch_status();
 }
 

@@ -20,6 +20,7 @@ Ctx::Ctx()
       panel_reg(env_int("MNP_PANEL_REG", 1) != 0),
       coop_min_rows(env_int("MNP_PANEL_COOP_MIN_ROWS", 4096)),
       gemm_split(env_int("MNP_GEMM_SPLIT", 0)),
+      plane_tile(env_int("MNP_GEMM_TILE", 11)),
       plane_group(env_int("MNP_GEMM_GROUP", 8)),
       asm_rows(env_int("MNP_ASM_ROWS", 64)),
       asm_lchunk(env_int("MNP_ASM_LCHUNK", 0)),

@@ -32,6 +32,7 @@ struct Ctx {
     bool panel_reg;        // 2: register-resident complex-float panel
     int coop_min_rows;     // 3: panel height from which the multi-workgroup panel is used
     int gemm_split;        // 4: complex-float GEMM mode
+    int plane_tile;        // 5: plane GEMM tile (11: 64 x 64, 22: 128 x 128 with direct LDS copies)
     int plane_group;       // 6: row tiles per group of the plane GEMM's XCD-aware order
     int asm_rows;          // 7: rows per workgroup of the Green-function assembly
     int asm_lchunk;        // 8: wavelengths per assembly thread (0: all)

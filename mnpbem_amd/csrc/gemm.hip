@@ -463,6 +463,374 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
     }
 }
 
+// Workgroup of 4 waves (2 x 2) on a (64 MT) x (64 NT) tile of C, each wave MT x NT MFMA tiles of
+// 32 x 32 (real and imaginary accumulators each): every A fragment read from LDS feeds NT MFMA
+// tiles and every B fragment MT, and a k slice of operands moved per workgroup feeds
+// MT NT times the products of the 64 x 64 tile of k_gemm_sub_c64p. Operands go from the planes
+// to LDS without registers (global_load_lds_dwordx4: a wave instruction copies 64 rows x 16
+// bytes of one half slice of one plane, exactly the plane's own layout, so LDS holds chunk
+// (plane, half, 64-row block) at 1 KB each); two LDS stages, the next slice's copies in flight
+// while the current one is multiplied, one barrier per slice. Fragment reads: lane l takes row
+// l % 32 of its MFMA tile from half l / 32, two contiguous 512-byte runs per wave
+// (conflict-free). Each accumulator receives the term products of k_gemm_sub_c64p in the same
+// order: results are identical. The MFMA tiles take turns, so that an accumulator is written
+// again only MT NT MFMAs later.
+template <int NS, int MT, int NT>
+__global__ __launch_bounds__(256) void k_gemm_sub_c64q(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
+                                                       long sPA, const __bf16* __restrict__ PB, int RB, long sPB,
+                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
+    constexpr int TM = 64 * MT, TN = 64 * NT;
+    constexpr int CA = 2 * NS * 2 * MT, CB = 2 * NS * 2 * NT;    // 1 KB chunks per slice: plane x half x block
+    constexpr int CH = CA + CB;
+    static_assert(CH % 4 == 0, "chunks split evenly over the four waves");
+    __shared__ __attribute__((aligned(1024))) unsigned char lds[2][CH * 1024];   // [stage][A chunks, B chunks]
+    int tx, ty, bz;
+    xcd_tile(gm, tx, ty, bz);
+    PA += (long)bz * sPA;
+    PB += (long)bz * sPB;
+    C += (long)bz * sC;
+    const int KS = (K + 15) / 16;
+    const int row0 = tx * TM, col0 = ty * TN;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave & 1) * 32 * MT, wn = (wave >> 1) * 32 * NT;
+    f16acc cre[MT][NT], cim[MT][NT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) { cre[a][b][r] = 0; cim[a][b][r] = 0; }
+    // chunk c < CA of A: plane c / (2 MT), half (c / MT) % 2, row block c % MT; then B likewise;
+    // wave w copies chunks w, w + 4, ...
+    auto issue = [&](int s, int st) {
+#pragma unroll
+        for (int i = 0; i < CH / 4; ++i) {
+            const int ch = wave + 4 * i;
+            const bool isb = ch >= CA;
+            const int c = isb ? ch - CA : ch;
+            const int nbk = isb ? NT : MT;
+            const int qc = c / (2 * nbk), h = (c / nbk) & 1, rb = c % nbk;
+            const __bf16* src = isb ? PB + ((((long)qc * KS + s) * 2 + h) * RB + col0 + 64 * rb) * 8
+                                    : PA + ((((long)qc * KS + s) * 2 + h) * RA + row0 + 64 * rb) * 8;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 8 * lane),
+                                             (__attribute__((address_space(3))) void*)&lds[st][ch * 1024], 16, 0, 0);
+        }
+    };
+    // fragment of MFMA tile row `tr` (a multiple of 32 below 64 MT or 64 NT), plane qc
+    auto fragA = [&](int st, int qc, int tr) {
+        const int r = tr + (lane & 31), h = lane >> 5;
+        return *(const bf8v*)&lds[st][(((qc * 2 + h) * MT + (r >> 6)) * 64 + (r & 63)) * 16];
+    };
+    auto fragB = [&](int st, int qc, int tr) {
+        const int r = tr + (lane & 31), h = lane >> 5;
+        return *(const bf8v*)&lds[st][((CA + (qc * 2 + h) * NT + (r >> 6)) * 64 + (r & 63)) * 16];
+    };
+    issue(0, 0);
+    for (int s = 0; s < KS; ++s) {
+        const int st = s & 1;
+        __builtin_amdgcn_s_waitcnt(0);     // this wave's copies of slice s have landed in LDS
+        __syncthreads();                   // everyone's; and stage st ^ 1 is no longer read
+        if (s + 1 < KS) issue(s + 1, st ^ 1);
+        bf8v pr[NT][NS], pi[NT][NS], qr[MT][NS], qi[MT][NS], qp[MT][NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                pr[u][q] = fragB(st, 2 * q, wn + 32 * u);
+                pi[u][q] = fragB(st, 2 * q + 1, wn + 32 * u);
+            }
+#pragma unroll
+            for (int u = 0; u < MT; ++u) {
+                qr[u][q] = fragA(st, 2 * q, wm + 32 * u);
+                qi[u][q] = fragA(st, 2 * q + 1, wm + 32 * u);
+                u4v w = __builtin_bit_cast(u4v, qi[u][q]);
+                w ^= 0x80008000u;
+                qp[u][q] = __builtin_bit_cast(bf8v, w);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < NS; ++p)
+#pragma unroll
+            for (int q = 0; q < NS - p; ++q) {
+#pragma unroll
+                for (int a = 0; a < MT; ++a)
+#pragma unroll
+                    for (int b = 0; b < NT; ++b)
+                        cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[b][p], qr[a][q], cre[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < MT; ++a)
+#pragma unroll
+                    for (int b = 0; b < NT; ++b)
+                        cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[b][p], qr[a][q], cim[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < MT; ++a)
+#pragma unroll
+                    for (int b = 0; b < NT; ++b)
+                        cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[b][p], qp[a][q], cre[a][b], 0, 0, 0);
+#pragma unroll
+                for (int a = 0; a < MT; ++a)
+#pragma unroll
+                    for (int b = 0; b < NT; ++b)
+                        cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[b][p], qi[a][q], cim[a][b], 0, 0, 0);
+            }
+    }
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gi = row0 + wm + 32 * a + (lane & 31);
+                const int gj = col0 + wn + 32 * b + Mfma<float>::drow(lane, r);
+                if (gi < m && gj < n) {
+                    cx<float>* c = C + gi + (long)gj * ldc;
+                    cx<float> v = *c;
+                    v.re += cre[a][b][r];
+                    v.im += cim[a][b][r];
+                    *c = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Complex-float GEMM on the FP16 matrix cores with operands as scaled pairs of half floats
+// (mode 16; the splitting of Ootomo and Yokota, "Recovering single precision accuracy from Tensor
+// Cores while surpassing the FP32 theoretical peak performance", IJHPCA 2022). Each row i of A and
+// column j of B gets a power-of-two scale s (its largest |re|, |im| brought into [2^14, 2^15)),
+// and every scaled value v = s x is held as hi = fp16(v) and lo = fp16((v - hi) 2^11): hi carries
+// 11 significant bits, lo the next 11, so the pair keeps about 22 bits, and the scales keep both
+// in the normal range of fp16 for everything within 2^-28 of the row's (column's) largest entry.
+// A real product keeps hi hi + (hi lo + lo hi) 2^-11 (the dropped lo lo term is 2^-22 of the
+// product): 3 MFMAs instead of the 6 of three bfloat16 terms. hi hi and the cross terms go to
+// separate FP32 accumulators, combined and unscaled in the epilogue (multiplications by powers
+// of two). Planes: the layout of the two-term bfloat16 planes, term 0 = hi, term 1 = lo, fp16.
+
+// Scales of the rows of A (m x K, column-major): one thread per row. sc[r] = 2^(15 - e) with
+// max_k max(|re|, |im|) in [2^(e-1), 2^e); 1 for a zero row and for the padding rows.
+__global__ __launch_bounds__(256) void k_scale_rows(int rows, int R, int K, const cx<float>* __restrict__ X, long ldx,
+                                                    long sX, float* __restrict__ sc) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    X += (long)blockIdx.y * sX;
+    sc += (long)blockIdx.y * R;
+    if (r >= R) return;
+    float m = 0.f;
+    if (r < rows)
+        for (int k = 0; k < K; ++k) {
+            const cx<float> v = X[r + (long)k * ldx];
+            m = fmaxf(m, fmaxf(fabsf(v.re), fabsf(v.im)));
+        }
+    int e = 0;
+    frexpf(m, &e);
+    e = e < -100 ? -100 : e;
+    sc[r] = m > 0.f ? ldexpf(1.f, 15 - e) : 1.f;
+}
+
+// Scales of the columns of B (K x n, column-major, a column contiguous): one wave per column,
+// lanes along K, wave maximum by butterfly shuffles.
+__global__ __launch_bounds__(256) void k_scale_cols(int cols, int R, int K, const cx<float>* __restrict__ X, long ldx,
+                                                    long sX, float* __restrict__ sc) {
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    X += (long)blockIdx.y * sX;
+    sc += (long)blockIdx.y * R;
+    if (c >= R) return;
+    float m = 0.f;
+    if (c < cols)
+        for (int k = lane; k < K; k += 64) {
+            const cx<float> v = X[k + (long)c * ldx];
+            m = fmaxf(m, fmaxf(fabsf(v.re), fabsf(v.im)));
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0) {
+        int e = 0;
+        frexpf(m, &e);
+        e = e < -100 ? -100 : e;
+        sc[c] = m > 0.f ? ldexpf(1.f, 15 - e) : 1.f;
+    }
+}
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+// Scaled fp16 pair planes: as k_split_planes<2, TRANS> with v = sgn sc[r] x, term 0 = fp16(v),
+// term 1 = fp16((v - term 0) 2^11).
+template <bool TRANS>
+__global__ __launch_bounds__(256) void k_split_f16(int rows, int K, const cx<float>* __restrict__ X, long ldx, long sX,
+                                                   _Float16* __restrict__ P, int R, long sP, float sgn,
+                                                   const float* __restrict__ sc, int vec) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int s = blockIdx.y;
+    const int KS = gridDim.y;
+    X += (long)blockIdx.z * sX;
+    P += (long)blockIdx.z * sP;
+    sc += (long)blockIdx.z * R;
+    if (r >= R) return;
+    float v[2][16];
+    const int k0 = 16 * s;
+    if (TRANS && vec && r < rows && k0 + 16 <= K) {
+        const float4* p = (const float4*)(X + k0 + (long)r * ldx);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float4 w = p[e];
+            v[0][2 * e] = w.x;
+            v[1][2 * e] = w.y;
+            v[0][2 * e + 1] = w.z;
+            v[1][2 * e + 1] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int k = k0 + e;
+            v[0][e] = 0.f;
+            v[1][e] = 0.f;
+            if (r < rows && k < K) {
+                const cx<float> w = TRANS ? X[k + (long)r * ldx] : X[r + (long)k * ldx];
+                v[0][e] = w.re;
+                v[1][e] = w.im;
+            }
+        }
+    }
+    const float f = sgn * sc[r];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int hf = 0; hf < 2; ++hf) {
+            h8v hi, lo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float x = f * v[c][8 * hf + e];
+                hi[e] = (_Float16)x;
+                lo[e] = (_Float16)((x - (float)hi[e]) * 2048.f);
+            }
+            *(h8v*)(P + (((long)(0 * 2 + c) * KS + s) * 2 + hf) * R * 8 + (long)r * 8) = hi;
+            *(h8v*)(P + (((long)(1 * 2 + c) * KS + s) * 2 + hf) * R * 8 + (long)r * 8) = lo;
+        }
+    }
+}
+
+// GEMM on the scaled fp16 pairs: the k_gemm_sub_c64q structure (tiles of (64 MT) x (64 NT), direct
+// LDS copies, two stages) with v_mfma_f32_32x32x16_f16, hi hi and cross-term accumulators, and the
+// epilogue C += (hh + x 2^-11) / (sA_i sB_j).
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void k_gemm_sub_c64h(int m, int n, int K, const _Float16* __restrict__ PA, int RA,
+                                                       long sPA, const _Float16* __restrict__ PB, int RB, long sPB,
+                                                       const float* __restrict__ scA, const float* __restrict__ scB,
+                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
+    constexpr int NS = 2;
+    constexpr int TM = 64 * MT, TN = 64 * NT;
+    constexpr int CA = 2 * NS * 2 * MT, CB = 2 * NS * 2 * NT;
+    constexpr int CH = CA + CB;
+    static_assert(CH % 4 == 0, "chunks split evenly over the four waves");
+    __shared__ __attribute__((aligned(1024))) unsigned char lds[2][CH * 1024];
+    int tx, ty, bz;
+    xcd_tile(gm, tx, ty, bz);
+    PA += (long)bz * sPA;
+    PB += (long)bz * sPB;
+    scA += (long)bz * RA;
+    scB += (long)bz * RB;
+    C += (long)bz * sC;
+    const int KS = (K + 15) / 16;
+    const int row0 = tx * TM, col0 = ty * TN;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave & 1) * 32 * MT, wn = (wave >> 1) * 32 * NT;
+    f16acc hre[MT][NT], him[MT][NT], xre[MT][NT], xim[MT][NT];
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) { hre[a][b][r] = 0; him[a][b][r] = 0; xre[a][b][r] = 0; xim[a][b][r] = 0; }
+    auto issue = [&](int s, int st) {
+#pragma unroll
+        for (int i = 0; i < CH / 4; ++i) {
+            const int ch = wave + 4 * i;
+            const bool isb = ch >= CA;
+            const int c = isb ? ch - CA : ch;
+            const int nbk = isb ? NT : MT;
+            const int qc = c / (2 * nbk), h = (c / nbk) & 1, rb = c % nbk;
+            const _Float16* src = isb ? PB + ((((long)qc * KS + s) * 2 + h) * RB + col0 + 64 * rb) * 8
+                                      : PA + ((((long)qc * KS + s) * 2 + h) * RA + row0 + 64 * rb) * 8;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 8 * lane),
+                                             (__attribute__((address_space(3))) void*)&lds[st][ch * 1024], 16, 0, 0);
+        }
+    };
+    auto fragA = [&](int st, int qc, int tr) {
+        const int r = tr + (lane & 31), h = lane >> 5;
+        return *(const h8v*)&lds[st][(((qc * 2 + h) * MT + (r >> 6)) * 64 + (r & 63)) * 16];
+    };
+    auto fragB = [&](int st, int qc, int tr) {
+        const int r = tr + (lane & 31), h = lane >> 5;
+        return *(const h8v*)&lds[st][((CA + (qc * 2 + h) * NT + (r >> 6)) * 64 + (r & 63)) * 16];
+    };
+    issue(0, 0);
+    for (int s = 0; s < KS; ++s) {
+        const int st = s & 1;
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (s + 1 < KS) issue(s + 1, st ^ 1);
+        h8v pr[NT][NS], pi[NT][NS], qr[MT][NS], qi[MT][NS], qp[MT][NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                pr[u][q] = fragB(st, 2 * q, wn + 32 * u);
+                pi[u][q] = fragB(st, 2 * q + 1, wn + 32 * u);
+            }
+#pragma unroll
+            for (int u = 0; u < MT; ++u) {
+                qr[u][q] = fragA(st, 2 * q, wm + 32 * u);
+                qi[u][q] = fragA(st, 2 * q + 1, wm + 32 * u);
+                u4v w = __builtin_bit_cast(u4v, qi[u][q]);
+                w ^= 0x80008000u;
+                qp[u][q] = __builtin_bit_cast(h8v, w);
+            }
+        }
+        // hi hi
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                hre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][0], qr[a][0], hre[a][b], 0, 0, 0);
+                him[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][0], qr[a][0], him[a][b], 0, 0, 0);
+            }
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) {
+                hre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][0], qp[a][0], hre[a][b], 0, 0, 0);
+                him[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][0], qi[a][0], him[a][b], 0, 0, 0);
+            }
+        // cross terms hi lo + lo hi
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int a = 0; a < MT; ++a)
+#pragma unroll
+                for (int b = 0; b < NT; ++b) {
+                    xre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][p], qr[a][1 - p], xre[a][b], 0, 0, 0);
+                    xim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][p], qr[a][1 - p], xim[a][b], 0, 0, 0);
+                    xre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][p], qp[a][1 - p], xre[a][b], 0, 0, 0);
+                    xim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][p], qi[a][1 - p], xim[a][b], 0, 0, 0);
+                }
+    }
+#pragma unroll
+    for (int a = 0; a < MT; ++a)
+#pragma unroll
+        for (int b = 0; b < NT; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int gi = row0 + wm + 32 * a + (lane & 31);
+                const int gj = col0 + wn + 32 * b + Mfma<float>::drow(lane, r);
+                if (gi < m && gj < n) {
+                    const float u = 1.f / (scA[gi] * scB[gj]);
+                    cx<float>* c = C + gi + (long)gj * ldc;
+                    cx<float> v = *c;
+                    v.re += (hre[a][b][r] + xre[a][b][r] * (1.f / 2048.f)) * u;
+                    v.im += (him[a][b][r] + xim[a][b][r] * (1.f / 2048.f)) * u;
+                    *c = v;
+                }
+            }
+}
+
 // Plane workspace of the calling thread's handle for stream s, grown on demand.
 static void* plane_ws(hipStream_t s, size_t bytes) {
     Ctx& c = ctx();
@@ -494,6 +862,48 @@ static size_t plane_bytes(int NS, int m, int n, int k, int batch) {
     return (size_t)2 * NS * KS * 16 * (RA + RB) * batch * sizeof(__bf16);
 }
 
+// Bytes of the workspace of one fp16-pair launch: the two-term planes and the row / column scales.
+static size_t f16_bytes(int m, int n, int k, int batch) {
+    const long RA = (m + ST - 1) / ST * ST, RB = (n + ST - 1) / ST * ST;
+    return plane_bytes(2, m, n, k, batch) + (size_t)(RA + RB) * batch * sizeof(float);
+}
+
+// Scales, fp16-pair planes of A (negated) and B at P0 (workspace of f16_bytes; null: the
+// handle's own), and the product. False when the workspace cannot be allocated.
+static bool gemm_f16(int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb, long sB, void* C,
+                     long ldc, long sC, int batch, hipStream_t s, void* P0, ProfRec* rec) {
+    const int KS = (k + 15) / 16;
+    const int RA = (m + ST - 1) / ST * ST, RB = (n + ST - 1) / ST * ST;
+    const long sPA = 4L * KS * RA * 16, sPB = 4L * KS * RB * 16;
+    _Float16* P = (_Float16*)(P0 ? P0 : plane_ws(s, f16_bytes(m, n, k, batch)));
+    if (!P) return false;
+    _Float16* PA = P;
+    _Float16* PB = P + sPA * batch;
+    float* scA = (float*)(PB + sPB * batch);
+    float* scB = scA + (long)RA * batch;
+    const int vb = ((ldb | sB) & 1) == 0 && ((uintptr_t)B & 15) == 0;
+    hipLaunchKernelGGL(k_scale_rows, dim3((RA + 255) / 256, batch), dim3(256), 0, s, m, RA, k, (const cx<float>*)A, lda,
+                       sA, scA);
+    hipLaunchKernelGGL(k_scale_cols, dim3((RB + 3) / 4, batch), dim3(256), 0, s, n, RB, k, (const cx<float>*)B, ldb,
+                       sB, scB);
+    hipLaunchKernelGGL((k_split_f16<false>), dim3((RA + 255) / 256, KS, batch), dim3(256), 0, s, m, k,
+                       (const cx<float>*)A, lda, sA, PA, RA, sPA, -1.0f, scA, 0);
+    hipLaunchKernelGGL((k_split_f16<true>), dim3((RB + 255) / 256, KS, batch), dim3(256), 0, s, n, k,
+                       (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, scB, vb);
+    if (rec) hipEventRecord(rec->a, s);
+    const int tile = ctx().plane_tile;
+    if (tile == 21)
+        hipLaunchKernelGGL((k_gemm_sub_c64h<2, 1>), dim3((m + 127) / 128, (n + 63) / 64, batch), dim3(256), 0, s, m, n,
+                           k, PA, RA, sPA, PB, RB, sPB, scA, scB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    else if (tile == 12)
+        hipLaunchKernelGGL((k_gemm_sub_c64h<1, 2>), dim3((m + 63) / 64, (n + 127) / 128, batch), dim3(256), 0, s, m, n,
+                           k, PA, RA, sPA, PB, RB, sPB, scA, scB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    else
+        hipLaunchKernelGGL((k_gemm_sub_c64h<1, 1>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n,
+                           k, PA, RA, sPA, PB, RB, sPB, scA, scB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    return true;
+}
+
 // Split A and B into planes at P0 (workspace of plane_bytes; null: the handle's own) and run
 // the product. False when the workspace cannot be allocated.
 template <int NS>
@@ -512,22 +922,43 @@ static bool gemm_planes(int m, int n, int k, const void* A, long lda, long sA, c
     hipLaunchKernelGGL((k_split_planes<NS, true>), dim3(RB / 256 + (RB % 256 != 0), KS, batch), dim3(256), 0, s, n,
                        k, (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, vb);
     if (rec) hipEventRecord(rec->a, s);    // the timed region is the GEMM kernel alone
-    hipLaunchKernelGGL((k_gemm_sub_c64p<NS>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n, k, PA,
-                       RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    const int tile = ctx().plane_tile;
+    const int gx = tile == 22 || tile == 21 ? (m + 127) / 128 : (m + 63) / 64;
+    const int gy = tile == 22 || tile == 12 ? (n + 127) / 128 : (n + 63) / 64;
+    if (tile == 22)
+        hipLaunchKernelGGL((k_gemm_sub_c64q<NS, 2, 2>), dim3(gx, gy, batch), dim3(256), 0, s, m, n, k, PA, RA, sPA, PB,
+                           RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    else if (tile == 21)
+        hipLaunchKernelGGL((k_gemm_sub_c64q<NS, 2, 1>), dim3(gx, gy, batch), dim3(256), 0, s, m, n, k, PA, RA, sPA, PB,
+                           RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    else if (tile == 12)
+        hipLaunchKernelGGL((k_gemm_sub_c64q<NS, 1, 2>), dim3(gx, gy, batch), dim3(256), 0, s, m, n, k, PA, RA, sPA, PB,
+                           RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    else if (tile == 111)
+        hipLaunchKernelGGL((k_gemm_sub_c64q<NS, 1, 1>), dim3(gx, gy, batch), dim3(256), 0, s, m, n, k, PA, RA, sPA, PB,
+                           RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    else
+        hipLaunchKernelGGL((k_gemm_sub_c64p<NS>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n, k,
+                           PA, RA, sPA, PB, RB, sPB, (cx<float>*)C, ldc, sC, ctx().plane_group);
     return true;
 }
 
 // 0: FP32 matrix cores; 12 or 13: BF16 matrix cores, operands split once into planes of 2 or 3
-// terms.
+// terms; 16: FP16 matrix cores, operands as scaled fp16 pairs.
 int set_gemm_split(int v) {
-    if (v != 0 && v != 12 && v != 13) return -2;
+    if (v != 0 && v != 12 && v != 13 && v != 16) return -2;
     ctx().gemm_split = v;
     return 0;
 }
 int get_gemm_split() { return ctx().gemm_split; }
-// The 64 x 64 tile is the only one kept (wider tiles and two LDS buffers measured slower).
-int set_plane_tile(int v) { return v == 11 ? 0 : -2; }
-int get_plane_tile() { return 11; }
+// Plane kernel tile: 11 = 64 x 64 staged through registers (k_gemm_sub_c64p); with direct LDS
+// copies (k_gemm_sub_c64q): 111 = 64 x 64, 21 = 128 x 64, 12 = 64 x 128, 22 = 128 x 128.
+int set_plane_tile(int v) {
+    if (v != 11 && v != 111 && v != 21 && v != 12 && v != 22) return -2;
+    ctx().plane_tile = v;
+    return 0;
+}
+int get_plane_tile() { return ctx().plane_tile; }
 int set_plane_group(int v) {
     if (v < 0 || v > 64) return -2;
     ctx().plane_group = v;
@@ -541,7 +972,8 @@ int get_plane_group() { return ctx().plane_group; }
 // below ~96).
 static bool plane_launch(int mode, int m, int n) {
     const int narrow = (m < n ? m : n);
-    return mode >= 12 && narrow >= ((mode % 10) == 3 ? 160 : 96);
+    if (mode == 16) return narrow >= 96;
+    return (mode == 12 || mode == 13) && narrow >= (mode == 13 ? 160 : 96);
 }
 
 // C -= A * B for `batch` independent problems with complex-float GEMM mode `mode` (0, 12, 13).
@@ -567,15 +999,17 @@ static void gemm_sub_impl(int mode, bool prof, int m, int n, int k, const void* 
     const bool vec = sizeof(T) == 4 && ((lda | ldb | sA | sB) & 1) == 0 && ((uintptr_t)A & 15) == 0 &&
                      ((uintptr_t)B & 15) == 0;
     if (sizeof(T) == 4 && plane_launch(mode, m, n)) {
-        const int terms = mode % 10;
+        const int terms = mode == 16 ? 2 : mode % 10;
         if (prof) {
-            // the kernel reads A and B as bfloat16 planes (2 bytes x 2 components x terms)
+            // the kernel reads A and B as 16-bit planes (2 bytes x 2 components x terms)
             rec.bytes = (4.0 * terms * ((double)m * k + (double)k * n) + 16.0 * m * n) * batch;
             rec.plane = 1;
         }
         ProfRec* rp = prof ? &rec : nullptr;
-        const bool ok = terms == 2 ? gemm_planes<2>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp)
-                                   : gemm_planes<3>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp);
+        const bool ok =
+            mode == 16 ? gemm_f16(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp)
+            : terms == 2 ? gemm_planes<2>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp)
+                         : gemm_planes<3>(m, n, k, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, P, rp);
         if (ok) {
             if (prof) {
                 hipEventRecord(rec.b, s);
@@ -615,7 +1049,7 @@ template void gemm_sub<float>(int, int, int, const void*, long, long, const void
 bool gemm_reserve(int m, int n, int k, int batch, hipStream_t s, hipStream_t side) {
     const int mode = ctx().gemm_split;
     if (!plane_launch(mode, m, n)) return true;
-    const size_t bytes = plane_bytes(mode % 10, m, n, k, batch);
+    const size_t bytes = mode == 16 ? f16_bytes(m, n, k, batch) : plane_bytes(mode % 10, m, n, k, batch);
     if (!plane_ws(s, bytes)) return false;
     return side ? plane_ws(side, bytes) != nullptr : true;
 }

@@ -328,11 +328,11 @@ template <>
 struct CoopRows<double> {
     static constexpr int value = 256;
 };
-// candidate rows per workgroup of a tournament round (one per lane; complex float at 1024 lanes
-// would spill the row and the published pivot row out of 128 registers)
+// candidate rows per workgroup of a tournament round: complex float, one wave holding two rows
+// per lane (k_calu_round_wave); complex double, one row per lane (k_calu_round)
 template <typename T>
 struct CaluRows {
-    static constexpr int value = 512;
+    static constexpr int value = 128;
 };
 template <>
 struct CaluRows<double> {
@@ -552,20 +552,21 @@ static bool panel_coop(int n, int j0, int w, cx<T>* a, long lda, long sA, int* i
 // ---------------------------------------------------------------------------------------------
 // Tall panels by tournament pivoting (CALU / TSLU): the w pivot rows of a panel of m rows are
 // chosen by a reduction tree instead of a search across the whole column for every column.
-//   k_calu_round: each workgroup takes up to CaluRows candidate rows (a contiguous block of the
-//          panel, or the winners of the previous round, from their original entries), runs LU
-//          with partial pivoting on them in registers (largest |re|+|im|, the earlier candidate
-//          on a tie, as izamax) and keeps its w pivot rows in pivot order; rounds repeat until
-//          one workgroup remains, which also writes the packed LU of its pivot rows: the
-//          factored w x w top block of the panel;
+//   k_calu_round (complex double), k_calu_round_wave (complex float): each workgroup takes up
+//          to CaluRows candidate rows (a contiguous block of the panel, or the winners of the
+//          previous round, from their original entries), runs LU with partial pivoting on them
+//          in registers (largest |re|+|im|, the earlier candidate on a tie, as izamax) and keeps
+//          its w pivot rows in pivot order; rounds repeat until one workgroup remains, which
+//          also writes the pivot magnitudes;
 //   k_calu_ipiv: LAPACK-style interchanges ipiv[j0 + c] that bring the winners to the top;
 //   k_laswp on the panel's own columns;
-//   k_calu_l21: the factored top block into place and L21 = A21 U11^{-1}, one row per lane.
+//   k_calu_top: the w x w top block (the winners' rows) eliminated in pivot order;
+//   k_calu_l21: L21 = A21 U11^{-1}, one row per lane.
 // The panel's interchanges and factors keep the layout of the other panel kernels, so the rest
 // of the blocked LU is unchanged. Pivots differ from partial pivoting's; no workgroup waits for
-// another, and every elimination step of a round costs one workgroup barrier: each wave
-// publishes its best candidate row in LDS (double-buffered by step parity) and every wave then
-// picks the same winner from those.
+// another. In k_calu_round every elimination step costs one workgroup barrier (each wave
+// publishes its best candidate row in LDS, double-buffered by step parity, and every wave then
+// picks the same winner from those); k_calu_round_wave is a single wave and needs none.
 
 // wave-wide maxima of the device library (DPP reductions; every lane gets the result)
 extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_max_u32(unsigned int);
@@ -587,13 +588,12 @@ struct PivKey<double> {
 };
 
 // One round. in == nullptr: candidates are rows j0 + c CP_T + t (< n); else in[b nin + c CP_T + t]
-// (< nin; -1 = none). out[(b G + c) w + k]: k-th winner (global row, -1 = none). A single
-// workgroup (the last round) also writes best[b w + k] (|pivot|) and the packed factor row k of
-// its winners to top[(b CP_W + k) CP_W + u].
+// (< nin; -1 = none). out[(b G + c) w + k]: k-th winner (global row, -1 = none); best (the last
+// round, a single workgroup; else null): best[b w + k] = |pivot| of step k.
 template <typename T>
 __global__ __launch_bounds__(CaluRows<T>::value) void k_calu_round(int n, int j0, int w, const cx<T>* A, long lda,
                                                                    long sA, const int* in, int nin, int* out,
-                                                                   T* best, cx<T>* top) {
+                                                                   T* best) {
     constexpr int CP_T = CaluRows<T>::value;
     constexpr int NW = CP_T / 64;
     using K = typename PivKey<T>::type;
@@ -651,16 +651,151 @@ __global__ __launch_bounds__(CaluRows<T>::value) void k_calu_round(int n, int j0
                 if (live) x[k] = l;
 #pragma unroll
                 for (int u = k + 1; u < CP_W; ++u) x[u] = fms(x[u], l, prow[buf][bw][u]);
-                if (t == 0) o[k] = win;
-                if (top != nullptr) {
-                    if (t == 0) best[(long)b * w + k] = PivKey<T>::value(bk);
-                    if (t < w) top[((long)b * CP_W + k) * CP_W + t] = prow[buf][bw][t];
+                if (t == 0) {
+                    o[k] = win;
+                    if (best != nullptr) best[(long)b * w + k] = PivKey<T>::value(bk);
                 }
             } else {
                 if (t == 0) o[k] = -1;
-                if (top != nullptr && t == 0) best[(long)b * w + k] = (T)0;
+                if (best != nullptr && t == 0) best[(long)b * w + k] = (T)0;
             }
         }
+    }
+}
+
+extern "C" __device__ __attribute__((const)) unsigned int __ockl_wfred_min_u32(unsigned int);
+
+// One round for complex float in a single wave: candidate q of the block (q < 64 R) sits in
+// lane q % 64, slot q / 64, R rows per lane in registers. The loop over the w elimination steps
+// is not unrolled (the fully unrolled form is larger than the instruction cache and ran bound by
+// instruction fetch): the rows are kept as shift registers, whose element 0 is the current pivot
+// column; after each step the remaining columns move one place down, so every register index is
+// a constant. A step needs no barrier: the wave maximum of the lanes' best |re|+|im| (the
+// earlier slot on a tie inside a lane), the wave minimum of the candidate index among the lanes
+// that hold it (the earlier candidate on a tie, as izamax), the winner's row through LDS to
+// every lane, and the rank-1 update of the live rows on the columns still in the panel. Only the
+// winners are kept (and, for the last round, the pivot magnitudes): the factored top block is
+// formed afterwards from the winners' rows (k_calu_top). Pivot rule as k_calu_round, blocks of
+// 64 R rows.
+template <int R>
+__global__ __launch_bounds__(64) void k_calu_round_wave(int n, int j0, int w, const cx<float>* A, long lda, long sA,
+                                                        const int* in, int nin, int* out, float* best) {
+    constexpr int CP = 64 * R;
+    using K = PivKey<float>;
+    __shared__ cx<float> prow[CP_W];
+    __shared__ int prow_id;
+    const int G = gridDim.x, c = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
+    A += (long)b * sA;
+    int row[R];
+    bool live[R];
+    cx<float> x[R][CP_W];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int q = c * CP + r * 64 + lane;
+        if (in == nullptr) {
+            const int g = j0 + q;
+            row[r] = g < n ? g : -1;
+        } else {
+            row[r] = q < nin ? in[(long)b * nin + q] : -1;
+        }
+        live[r] = row[r] >= 0;
+#pragma unroll
+        for (int u = 0; u < CP_W; ++u)
+            x[r][u] = (live[r] && u < w) ? A[row[r] + (long)(j0 + u) * lda] : mk<float>(0, 0);
+    }
+    int* o = out + ((long)b * G + c) * w;
+#pragma nounroll
+    for (int k = 0; k < w; ++k) {
+        unsigned key = 0, idx = 0xffffffffu;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const unsigned kr = live[r] ? K::of(cabs1(x[r][0])) : 0u;
+            if (kr > key) {
+                key = kr;
+                idx = (unsigned)(r * 64 + lane);
+            }
+        }
+        const unsigned wmax = K::wave_max(key);
+        if (wmax == 0u) {
+            // no live candidate left
+            if (lane == 0) {
+                o[k] = -1;
+                if (best != nullptr) best[(long)b * w + k] = 0.0f;
+            }
+        } else {
+            const unsigned win = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_u32(key == wmax ? idx : 0xffffffffu));
+            const int wl = (int)(win & 63u), ws = (int)(win >> 6);
+            if (lane == wl) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (ws == r) {
+#pragma unroll
+                        for (int u = 0; u < CP_W; ++u) prow[u] = x[r][u];
+                        prow_id = row[r];
+                        live[r] = false;
+                    }
+            }
+            __syncthreads();
+            cx<float> pr[CP_W];
+#pragma unroll
+            for (int u = 0; u < CP_W; ++u) pr[u] = prow[u];
+            const int wrow = prow_id;
+            const bool zero = pr[0].re == 0 && pr[0].im == 0;
+            const cx<float> rp = zero ? mk<float>(0, 0) : crecip(pr[0]);
+            // columns k + 1 .. w - 1 of the panel sit at 1 .. w - 1 - k; groups of 8 past them
+            // are skipped (a branch the whole wave takes)
+            const int left = w - k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                // a zero pivot leaves every live x[0] at zero: no update, as partial pivoting
+                const cx<float> l = live[r] ? x[r][0] * rp : mk<float>(0, 0);
+#pragma unroll
+                for (int g = 0; g < CP_W / 8; ++g)
+                    if (8 * g < left)
+#pragma unroll
+                        for (int u = 8 * g; u < 8 * g + 8; ++u)
+                            if (u > 0) x[r][u] = fms(x[r][u], l, pr[u]);
+#pragma unroll
+                for (int u = 0; u + 1 < CP_W; ++u) x[r][u] = x[r][u + 1];
+                x[r][CP_W - 1] = mk<float>(0, 0);
+            }
+            if (lane == 0) {
+                o[k] = wrow;
+                if (best != nullptr) best[(long)b * w + k] = K::value(wmax);
+            }
+            __syncthreads();      // prow is rewritten at the next step
+        }
+    }
+}
+
+// The factored w x w top block of a tournament panel: the winners' rows, moved to rows
+// j0 .. j0 + w by the interchanges, eliminated in pivot order without further pivoting (the
+// same arithmetic as the rounds: l = x_k / pivot through the reciprocal, x_u -= l p_u, and a
+// zero pivot leaves its column unscaled), in place, with a row-major copy in top[] for
+// k_calu_l21. One wave per matrix, the block in LDS, lane r owns row r.
+template <typename T>
+__global__ __launch_bounds__(64) void k_calu_top(int j0, int w, cx<T>* A, long lda, long sA, cx<T>* top) {
+    __shared__ cx<T> t_[CP_W][CP_W + 1];
+    const int b = blockIdx.x, r = threadIdx.x;
+    A += (long)b * sA;
+    top += (long)b * CP_W * CP_W;
+    for (int e = r; e < w * w; e += 64) t_[e % w][e / w] = A[j0 + e % w + (long)(j0 + e / w) * lda];
+    __syncthreads();
+    for (int k = 0; k < w; ++k) {
+        if (r > k && r < w) {
+            const cx<T> piv = t_[k][k];
+            const bool zero = piv.re == 0 && piv.im == 0;
+            const cx<T> rp = zero ? mk<T>(0, 0) : crecip(piv);
+            const cx<T> l = t_[r][k] * rp;
+            t_[r][k] = l;
+            for (int u = k + 1; u < w; ++u) t_[r][u] = fms(t_[r][u], l, t_[k][u]);
+        }
+        __syncthreads();
+    }
+    for (int e = r; e < w * w; e += 64) {
+        const int i = e % w, jj = e / w;
+        A[j0 + i + (long)(j0 + jj) * lda] = t_[i][jj];
+        top[i * CP_W + jj] = t_[i][jj];
     }
 }
 
@@ -711,39 +846,31 @@ __global__ __launch_bounds__(64) void k_calu_ipiv(int j0, int w, const int* sel,
     if (q == 0 && first_zero != 0 && info[b] == 0) info[b] = first_zero;
 }
 
-// The factored top block (rows j0 .. j0 + w) from top[] into place (workgroup x = 0), and
-// L21 = A21 U11^{-1} for the rows j0 + w .. n, one lane per row.
-template <typename T>
-__global__ __launch_bounds__(256) void k_calu_l21(int n, int j0, int w, cx<T>* A, long lda, long sA,
-                                                  const cx<T>* top) {
-    __shared__ cx<T> u[CP_W][CP_W + 1];
+// L21 = A21 U11^{-1} for the rows j0 + w .. n, one lane per row, NT rows per workgroup. U11's
+// entries are the same for every lane: they are read from top[] with uniform addresses (scalar
+// loads), so that only the row itself lives in vector registers; the substitution runs
+// right-looking (after x[k] is final, the later columns' updates are independent).
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_calu_l21(int n, int j0, int w, cx<T>* A, long lda, long sA,
+                                                 const cx<T>* __restrict__ top) {
     const int t = threadIdx.x, b = blockIdx.y;
     A += (long)b * sA;
     top += (long)b * CP_W * CP_W;
-    __shared__ cx<T> dinv[CP_W];
-    // U11 padded to CP_W x CP_W with the identity, so that columns from w on stay zero; the
-    // reciprocals of its diagonal (1 for a zero pivot: that column is left unscaled)
-    for (int e = t; e < CP_W * CP_W; e += 256) {
-        const int r = e / CP_W, c = e % CP_W;
-        u[r][c] = (r < w && c < w) ? top[r * CP_W + c] : mk<T>(r == c ? 1 : 0, 0);
-    }
-    if (t < CP_W) {
-        const cx<T> d = t < w ? top[t * CP_W + t] : mk<T>(1, 0);
-        dinv[t] = (d.re == 0 && d.im == 0) ? mk<T>(1, 0) : crecip(d);
-    }
-    __syncthreads();
-    if (blockIdx.x == 0)
-        for (int e = t; e < w * w; e += 256) A[j0 + e % w + (long)(j0 + e / w) * lda] = u[e % w][e / w];
-    const int row = j0 + w + blockIdx.x * 256 + t;
+    const int row = j0 + w + blockIdx.x * NT + t;
     if (row >= n) return;
     cx<T> x[CP_W];
 #pragma unroll
     for (int c = 0; c < CP_W; ++c) x[c] = c < w ? A[row + (long)(j0 + c) * lda] : mk<T>(0, 0);
 #pragma unroll
-    for (int c = 0; c < CP_W; ++c) {
+    for (int k = 0; k < CP_W; ++k) {
+        if (k < w) {
+            const cx<T> d = top[k * CP_W + k];
+            // a zero pivot leaves its column unscaled
+            if (d.re != 0 || d.im != 0) x[k] = x[k] * crecip(d);
 #pragma unroll
-        for (int k = 0; k < c; ++k) x[c] = fms(x[c], x[k], u[k][c]);
-        x[c] = x[c] * dinv[c];
+            for (int c = k + 1; c < CP_W; ++c)
+                if (c < w) x[c] = fms(x[c], x[k], top[k * CP_W + c]);
+        }
     }
 #pragma unroll
     for (int c = 0; c < CP_W; ++c)
@@ -836,6 +963,17 @@ static void laswp(int r0, int r1, int a0, int a1, int b0, int b1, cx<T>* a, long
                        sA, ipiv, sP);
 }
 
+static void round_launch(int G, int batch, int n, int j0, int w, const cx<float>* a, long lda, long sA, const int* in,
+                         int nin, int* out, float* best, hipStream_t s) {
+    static_assert(CaluRows<float>::value == 128, "two rows per lane of one wave");
+    hipLaunchKernelGGL(k_calu_round_wave<2>, dim3(G, batch), dim3(64), 0, s, n, j0, w, a, lda, sA, in, nin, out, best);
+}
+static void round_launch(int G, int batch, int n, int j0, int w, const cx<double>* a, long lda, long sA,
+                         const int* in, int nin, int* out, double* best, hipStream_t s) {
+    hipLaunchKernelGGL(k_calu_round<double>, dim3(G, batch), dim3(CaluRows<double>::value), 0, s, n, j0, w, a, lda, sA,
+                       in, nin, out, best);
+}
+
 // Tall panel by tournament pivoting (see k_calu_round); false when the scratch does not fit.
 template <typename T>
 static bool panel_calu(int n, int j0, int w, cx<T>* a, long lda, long sA, int* ipiv, long sP, int* info, int batch,
@@ -855,8 +993,8 @@ static bool panel_calu(int n, int j0, int w, cx<T>* a, long lda, long sA, int* i
     int nin = 0;
     for (;;) {
         const bool last = G == 1;
-        hipLaunchKernelGGL(k_calu_round<T>, dim3(G, batch), dim3(CP_T), 0, s, n, j0, w, (const cx<T>*)a, lda, sA, in,
-                           nin, last ? sel : lst[cur], best, last ? top : (cx<T>*)nullptr);
+        round_launch(G, batch, n, j0, w, (const cx<T>*)a, lda, sA, in, nin, last ? sel : lst[cur],
+                     last ? best : (T*)nullptr, s);
         if (last) break;
         in = lst[cur];
         nin = G * w;
@@ -866,8 +1004,10 @@ static bool panel_calu(int n, int j0, int w, cx<T>* a, long lda, long sA, int* i
     hipLaunchKernelGGL(k_calu_ipiv<T>, dim3(batch), dim3(64), 0, s, j0, w, (const int*)sel, (const T*)best, ipiv, sP,
                        info);
     laswp<T>(j0, j0 + w, j0, j0 + w, 0, 0, a, lda, sA, ipiv, sP, batch, s);
-    const int gx = m > w ? (m - w + 255) / 256 : 1;
-    hipLaunchKernelGGL(k_calu_l21<T>, dim3(gx, batch), dim3(256), 0, s, n, j0, w, a, lda, sA, (const cx<T>*)top);
+    hipLaunchKernelGGL(k_calu_top<T>, dim3(batch), dim3(64), 0, s, j0, w, a, lda, sA, top);
+    // 64 rows per workgroup: a 25k-row panel of one matrix spreads over every CU
+    const int gx = m > w ? (m - w + 63) / 64 : 1;
+    hipLaunchKernelGGL((k_calu_l21<T, 64>), dim3(gx, batch), dim3(64), 0, s, n, j0, w, a, lda, sA, (const cx<T>*)top);
     return true;
 }
 

@@ -220,6 +220,7 @@ class RetSolverBatch:
     def _factor_assembled(self):
         """LU chain of the assembled G1, G2, H1, H2 of the first self.nlam matrices."""
         g, f, nl = self.geo, self.fac, self.nlam
+        self.explicit = False
         from .autotune import knobs
         with knobs(self.tuning):
             call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
@@ -235,6 +236,27 @@ class RetSolverBatch:
             warn_singular("BEMRet ({})".format(self.precision), info)
         return self
 
+    def invert(self):
+        """Replaces the LU factors of G1, G2 and Sigma by their explicit inverses (right-side
+        solves of the identity, 16/3 n^3 complex flops each), so that every later solve applies
+        them as matrix-vector products: for a factorisation reused by many solves (a fixed GMRES
+        preconditioner), whose triangular substitutions are latency-bound."""
+        if getattr(self, "explicit", False):
+            return self
+        g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
+        pb = nl * n
+        qinv = {0: 1, 1: 3, 2: 7}              # perm layout: (pf, qinv) pairs of G1, G2, Delta, Sigma
+        eye, out = f.H1, f.H2                   # scratch after the factorisation
+        for i, M in enumerate((f.G1, f.G2, self.S2)):
+            eye[:nl].zero_()
+            eye[:nl].diagonal(dim1=1, dim2=2).fill_(1)
+            call("mnp_rsolve_batched", self.prec, n, n, nl, _p(M), f.ld, f.stride,
+                 self.perm.data_ptr() + 4 * qinv[i] * pb, n, _p(eye), f.ld, f.stride, _p(out), f.ld, f.stride,
+                 self.nb, stream_handle())
+            M[:nl].copy_(out[:nl])
+        self.explicit = True
+        return self
+
     def check_info(self):
         """(4, nlam) LU info of G1, G2, Delta, Sigma: 0 = ok, j = zero pivot in column j-1.
         The factor call lays pivots and info out for the wavelengths it was given."""
@@ -244,8 +266,9 @@ class RetSolverBatch:
     def _chain(self, rhs, out):
         """out = M rhs with the factor precision (rhs, out: (B, 8, n) of that dtype)."""
         g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
+        perm = None if getattr(self, "explicit", False) else _p(self.perm)
         call("mnp_bemret_solve", self.prec, n, nl, _p(f.G1), _p(f.G2), _p(self.S1), _p(self.D), _p(self.S2),
-             f.ld, f.stride, _p(self.perm), _p(g.nvec), g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), _p(rhs),
+             f.ld, f.stride, perm, _p(g.nvec), g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), _p(rhs),
              8 * n,
              _p(out), 8 * n, _p(self.work), 24 * n, stream_handle())
 

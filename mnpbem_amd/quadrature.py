@@ -6,6 +6,13 @@
 * ``triangle_rule18``: the published 28-point, degree-11 triangle rule
   (Burkardt's triangle rule table, rule 18; reference utils/quadrature.py:182-223),
   weights normalised to sum 1.
+* ``triangle_rule(rule)``: the reference's other rule numbers whose points follow from closed
+  forms (utils/quadrature.py:141-181): 1 (centroid, degree 1), 2 (vertices, degree 1),
+  3 (edge midpoints, degree 2), 4 (centroid and three interior points, degree 3), 7 (Radon's
+  seven-point rule, degree 5, with the sqrt(15) orbits); 18 as above.
+* ``subdivide``: a rule repeated on the refine x refine congruent sub-triangles of the unit
+  triangle (upward and downward ones), weights divided evenly (the reference's ``refine``
+  option, utils/quadface.py:21-44).
 * ``polar_triangle`` / ``polar_square``: polar rules around the element centre
   used for the singular self-term integrals (reference utils/quadface.py:106-249):
   LGL nodes in radius and angle over one 60 (45) degree sector, replicated
@@ -60,6 +67,53 @@ def triangle_rule18():
     return arr[:, 0].copy(), arr[:, 1].copy(), w
 
 
+def triangle_rule(rule=18):
+    """Points (x, y) and weights (sum 1) of triangle rule number ``rule`` on (0,0),(1,0),(0,1)."""
+    if rule == 18:
+        return triangle_rule18()
+    if rule == 1:
+        x, y, w = [1 / 3], [1 / 3], [1.0]
+    elif rule == 2:
+        x, y, w = [0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1 / 3] * 3
+    elif rule == 3:
+        x, y, w = [0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [1 / 3] * 3
+    elif rule == 4:
+        # centroid with weight -27/48 and the orbit of (3/5, 1/5, 1/5) with 25/48 each
+        x, y, w = [1 / 3, 0.6, 0.2, 0.2], [1 / 3, 0.2, 0.6, 0.2], [-27 / 48] + [25 / 48] * 3
+    elif rule == 7:
+        # Radon: centroid (9/40) and the orbits of barycentric (b, a, a) with
+        # a = (6 -+ sqrt 15) / 21, b = 1 - 2a, weights (155 -+ sqrt 15) / 1200
+        r = np.sqrt(15.0)
+        a1, a2 = (6 - r) / 21, (6 + r) / 21
+        b1, b2 = 1 - 2 * a1, 1 - 2 * a2
+        w1, w2 = (155 - r) / 1200, (155 + r) / 1200
+        x = [1 / 3, b1, a1, a1, b2, a2, a2]
+        y = [1 / 3, a1, b1, a1, a2, b2, a2]
+        w = [9 / 40, w1, w1, w1, w2, w2, w2]
+    else:
+        raise NotImplementedError("triangle rule {} is not provided (rules 1, 2, 3, 4, 7, 18)".format(rule))
+    return np.asarray(x, float), np.asarray(y, float), np.asarray(w, float)
+
+
+def subdivide(x, y, w, refine):
+    """The rule (x, y, w) on each of the refine^2 sub-triangles of the unit triangle: upward
+    triangles (i, j) + (x, y) and downward ones (i + 1, j + 1) - (x, y), scaled by 1 / refine;
+    weights w / refine^2."""
+    refine = int(refine)
+    if refine <= 1:
+        return x, y, w
+    xs, ys = [], []
+    for i in range(refine):
+        for j in range(refine - i):
+            xs.append(i + x)
+            ys.append(j + y)
+            if j < refine - 1 - i:
+                xs.append(i + 1 - x)
+                ys.append(j + 1 - y)
+    k = len(xs)
+    return np.concatenate(xs) / refine, np.concatenate(ys) / refine, np.tile(w, k) / k
+
+
 def _polar_grid(npol, half_angle_deg, centre_deg):
     n_rad, n_ang = npol
     xr, wr = lgl_nodes(n_rad)
@@ -104,13 +158,12 @@ class QuadFace:
     """Bundle of the rules a boundary element uses (reference utils/quadface.py:48-104)."""
 
     def __init__(self, rule=18, npol=(7, 5), refine=None):
-        if rule != 18:
-            raise NotImplementedError("only the default 28-point triangle rule (rule=18) is provided")
-        if refine not in (None, 0, 1):
-            raise NotImplementedError("quadrature subdivision (refine>1) is not provided")
         if np.isscalar(npol):
             npol = (int(npol), int(npol))
+        self.rule, self.refine = int(rule), refine
         self.npol = tuple(npol)
-        self.x, self.y, self.w = triangle_rule18()
+        self.x, self.y, self.w = triangle_rule(self.rule)
+        if refine is not None and int(refine) > 1:
+            self.x, self.y, self.w = subdivide(self.x, self.y, self.w, refine)
         self.x3, self.y3, self.w3 = polar_triangle(self.npol)
         self.x4, self.y4, self.w4 = polar_square(self.npol)

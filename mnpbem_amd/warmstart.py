@@ -211,19 +211,28 @@ class WarmStartMLP:
         return Y.double().cpu().numpy() * self.y_std + self.y_mean
 
 
-def training_set(cps, lams, pol, direc, precision="mixed"):
-    """Features and targets from direct solves (BEMRet) of every ComParticle in ``cps`` at
-    every wavelength in ``lams`` for one plane wave: (X (S, 12), T (S, 4)) numpy."""
+def training_set(cps, lams, pol, direc, precision="mixed", batch=50):
+    """Features and targets from direct solves of every ComParticle in ``cps`` over its
+    wavelengths for one plane wave: ``lams`` is one array for all particles or one array per
+    particle. Each particle's wavelengths run as device sweeps (BEMRet.solve_spectrum, ``batch``
+    wavelengths per factorisation, solutions copied back). Returns (X (S, 12), T (S, 4))."""
     from .bem import BEMRet
     from .planewave import PlaneWaveRet
     exc = PlaneWaveRet(np.asarray(pol, float), np.asarray(direc, float))
+    p = np.asarray(pol, float)
+    p = p / np.linalg.norm(p)
+    per = isinstance(lams, (list, tuple)) or np.ndim(lams) == 2
     Xs, Ts = [], []
-    for cp in cps:
+    for i, cp in enumerate(cps):
+        el = np.atleast_1d(np.asarray(lams[i] if per else lams, float))
         H = mean_curvature(cp)
-        bem = BEMRet(cp, precision=precision)
-        for lam in lams:
-            sig, _ = bem.solve(exc(cp, lam))
+        bem = BEMRet(cp, precision=precision, batch=min(batch, len(el)), autotune=False)
+        sol = bem.solve_spectrum(exc, el, solutions=True)["solutions"]
+        for j, lam in enumerate(el):
+            x = sol[j, 0]
+            hp = x[1:4].T @ p
             Xs.append(features(cp, lam, pol, direc, curvature=H))
-            Ts.append(targets(cp, sig, pol))
+            Ts.append(np.column_stack([x[4].real, x[4].imag, hp.real, hp.imag]))
+        bem.clear()
         del bem
     return np.vstack(Xs), np.vstack(Ts)

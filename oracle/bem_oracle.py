@@ -165,17 +165,75 @@ def _quad9(x, y):
     return N, Nx, Ny
 
 
-class Surface:
-    """Curved-element surface: per-face geometry and quadrature points."""
+def triangle_rule(rule):
+    """Triangle rules with closed-form points (utils/quadrature.py:141-181), sum w = 1, as
+    barycentric orbits: rule 1 centroid; 2 vertices; 3 edge midpoints; 4 centroid (-27/48) and
+    the orbit of (3/5, 1/5, 1/5) (25/48); 7 centroid (9/40) and the orbits of (1 - 2a, a, a),
+    a = (6 -+ sqrt 15)/21, weights (155 -+ sqrt 15)/1200; 18 the tabulated rule."""
+    if rule == 18:
+        return rule18()
 
-    def __init__(self, verts, faces, verts2, faces2, npol=(7, 5)):
+    def orbit(b):
+        # the distinct permutations of barycentric (b0, b1, b2), as (x, y) = (b0, b1)
+        out = []
+        for perm in ((0, 1, 2), (1, 0, 2), (1, 2, 0), (0, 2, 1), (2, 0, 1), (2, 1, 0)):
+            q = (b[perm[0]], b[perm[1]])
+            if not any(abs(q[0] - o[0]) < 1e-15 and abs(q[1] - o[1]) < 1e-15 for o in out):
+                out.append(q)
+        return out
+    pts = []
+    if rule == 1:
+        pts = [((1 / 3, 1 / 3), 1.0)]
+    elif rule == 2:
+        pts = [(q, 1 / 3) for q in orbit((1.0, 0.0, 0.0))]
+    elif rule == 3:
+        pts = [(q, 1 / 3) for q in orbit((0.5, 0.5, 0.0))]
+    elif rule == 4:
+        pts = [((1 / 3, 1 / 3), -27 / 48)] + [(q, 25 / 48) for q in orbit((0.6, 0.2, 0.2))]
+    elif rule == 7:
+        r = np.sqrt(15.0)
+        pts = [((1 / 3, 1 / 3), 9 / 40)]
+        for a, wt in (((6 - r) / 21, (155 - r) / 1200), ((6 + r) / 21, (155 + r) / 1200)):
+            pts += [(q, wt) for q in orbit((1 - 2 * a, a, a))]
+    else:
+        raise NotImplementedError(rule)
+    x = np.array([q[0] for q, _ in pts])
+    y = np.array([q[1] for q, _ in pts])
+    return x, y, np.array([wt for _, wt in pts])
+
+
+def subdivide(x, y, w, refine):
+    """A unit-triangle rule on its refine^2 congruent sub-triangles (utils/quadface.py:21-44):
+    sub-triangle (i, j) upward with corners (i, j) + the unit triangle / refine, downward ones
+    (i + 1, j + 1) - the unit triangle / refine; weights shared evenly."""
+    if refine is None or refine <= 1:
+        return x, y, w
+    X, Y = [], []
+    for i in range(refine):
+        for j in range(refine - i):
+            X.append((i + x) / refine)
+            Y.append((j + y) / refine)
+        for j in range(refine - i - 1):
+            X.append((i + 1 - x) / refine)
+            Y.append((j + 1 - y) / refine)
+    return np.concatenate(X), np.concatenate(Y), np.tile(w, len(X)) / len(X)
+
+
+class Surface:
+    """Curved-element surface: per-face geometry and quadrature points. Element areas use the
+    28-point rule; the integrals of the refined entries use ``rule`` subdivided ``refine`` times
+    (the reference applies ComParticle's rule/refine options to the particles' integration
+    points after their geometry is set up, geometry/comparticle.py:144-161)."""
+
+    def __init__(self, verts, faces, verts2, faces2, npol=(7, 5), rule=18, refine=None):
         self.verts = np.asarray(verts, float)
         self.faces = np.asarray(faces, float)
         self.verts2 = np.asarray(verts2, float)
         self.faces2 = np.asarray(faces2, float)
         self.n = len(self.faces)
         self.istri = np.isnan(self.faces[:, 3])
-        self.qx, self.qy, self.qw = rule18()
+        self.gx, self.gy, self.gw = rule18()
+        self.qx, self.qy, self.qw = subdivide(*triangle_rule(rule), refine)
         (self.x3, self.y3, self.w3), (self.x4, self.y4, self.w4) = polar_rules(npol)
         self._geometry()
 
@@ -186,14 +244,16 @@ class Surface:
             return [self.verts2[f[[0, 1, 2, 4, 5, 6]].astype(int)]]
         return [self.verts2[f[[0, 1, 2, 4, 5, 8]].astype(int)], self.verts2[f[[2, 3, 0, 6, 7, 8]].astype(int)]]
 
-    def quad_points(self, i):
-        """28-point rule per 6-node triangle (particle.py:1520-1575)."""
+    def quad_points(self, i, geometry=False):
+        """Integration points per 6-node triangle (particle.py:1520-1575): the element's rule, or
+        the 28-point rule of its geometry (area) with ``geometry``."""
         P, W = [], []
-        N, Nx, Ny = _tri6(self.qx, self.qy)
+        x, y, w = (self.gx, self.gy, self.gw) if geometry else (self.qx, self.qy, self.qw)
+        N, Nx, Ny = _tri6(x, y)
         for nodes in self._tri_nodes(i):
             P.append(N.T @ nodes)
             J = np.cross(Nx.T @ nodes, Ny.T @ nodes)
-            W.append(0.5 * self.qw * np.sqrt(np.sum(J * J, axis=1)))
+            W.append(0.5 * w * np.sqrt(np.sum(J * J, axis=1)))
         return np.vstack(P), np.concatenate(W)
 
     def polar_points(self, i):
@@ -219,7 +279,7 @@ class Surface:
         self.area = np.zeros(n)
         self.rad = np.zeros(n)
         for i in range(n):
-            self.area[i] = self.quad_points(i)[1].sum()
+            self.area[i] = self.quad_points(i, geometry=True)[1].sum()
             f = self.faces2[i]
             if self.istri[i]:
                 nodes = self.verts2[f[[0, 1, 2, 4, 5, 6]].astype(int)]

@@ -320,3 +320,32 @@ def test_assembly_shapes_all_equal_and_tuner_picks_one(sphere):
                 assert torch.equal(a, b), cfg
         best = autotune.tune_assembly(ref, kmat, len(lams))
         assert {k: best[k] for k in ("asm_rows", "asm_lchunk")} in autotune.asm_candidates(len(lams))
+
+
+@pytest.mark.parametrize("rule,refine", [(7, None), (18, 2), (4, 3)])
+def test_quadrature_options_match_oracle(rule, refine):
+    """ComParticle(..., rule=, refine=) (geometry/comparticle.py:144-186): the integration points
+    of the refined entries follow the rule (subdivided refine x refine), the element areas keep
+    the 28-point rule. Extinction and sigma2 of the complex128 chain vs the oracle built with the
+    same options (1e-10), and the options change the answer (they are not ignored)."""
+    banner("bemret quadrature rule={} refine={}".format(rule, refine))
+    part = M.trisphere(144, 20)
+    eps = [M.EpsConst(1.0), M.EpsDrude(10.0, 9.065, 0.0708)]
+    kw = dict(rule=rule) if refine is None else dict(rule=rule, refine=refine)
+    cp = M.ComParticle(eps, [part], [[2, 1]], 1, interp="curv", **kw)
+    cp0 = M.ComParticle(eps, [M.trisphere(144, 20)], [[2, 1]], 1, interp="curv")
+    assert np.array_equal(cp.area, cp0.area)
+    lam = 560.0
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    sig, _ = M.BEMRet(cp).solve(exc.potential(cp, lam))
+    sig0, _ = M.BEMRet(cp0).solve(exc.potential(cp0, lam))
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2, rule=rule, refine=refine)
+    out = O.solve_spectrum(s, np.zeros(s.n, int), [[2, 1]], [lambda e: O.eps_const(e, 1.0),
+                           lambda e: O.eps_drude(e, 10.0, 9.065, 0.0708)], [lam], np.array([1.0, 0, 0]),
+                           np.array([0, 0, 1.0]), return_fields=True)
+    ext, ext0 = exc.extinction(sig), exc.extinction(sig0)
+    assert abs(ext - out["ext"][0]) / abs(out["ext"][0]) < 1e-10
+    ref = out["fields"][0]["sol"][2]
+    assert np.max(np.abs(sig.sig2 - ref)) / np.max(np.abs(ref)) < 1e-10
+    assert abs(ext - ext0) / abs(ext0) > 1e-9

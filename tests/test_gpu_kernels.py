@@ -83,7 +83,8 @@ def test_gemm_sub(prec, m, n, k, lda, ldb):
         assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < tol
 
 
-@pytest.mark.parametrize("split,tile", [(12, 11), (13, 11)])
+@pytest.mark.parametrize("split,tile", [(12, 11), (13, 11), (12, 21), (13, 21), (12, 22), (13, 22), (16, 11),
+                                        (16, 21), (16, 12)])
 @pytest.mark.parametrize("m,n,k,lda,ldb", [(200, 167, 45, 200, 46), (300, 260, 128, 300, 128), (192, 192, 16, 192, 16),
                                            (331, 270, 37, 332, 38), (171, 165, 37, 171, 37), (150, 70, 37, 150, 38)])
 def test_gemm_sub_split_bf16(split, tile, m, n, k, lda, ldb):
@@ -120,6 +121,79 @@ def test_gemm_sub_split_bf16(split, tile, m, n, k, lda, ldb):
         assert err < (4e-5 if split % 10 == 2 else 1e-6), err
         if split % 10 == 3:           # as accurate as the FP32 kernel (within 2x)
             assert np.max(np.abs(_np(tC, b) - ref)) / np.max(np.abs(ref)) < 4e-6
+
+
+def test_gemm_fp16_pairs_as_accurate_as_fp32():
+    """Scaled fp16 pairs (mode 16, about 22 significant bits per operand, 3 products per real
+    product) against the FP32 matrix cores (mode 0) and three bfloat16 terms (mode 13) on operands
+    whose rows and columns span 12 orders of magnitude: the error of every entry relative to
+    (|A||B|)_ij stays within 4x that of the FP32 kernel."""
+    banner("gemm fp16 pairs vs fp32")
+    rng = np.random.default_rng(21)
+    m, n, k, B_ = 640, 576, 256, 2
+    A = [_rand(rng, m, k) * 10.0 ** rng.uniform(-6, 6, (m, 1)) for _ in range(B_)]
+    Bm = [_rand(rng, k, n) * 10.0 ** rng.uniform(-6, 6, (1, n)) for _ in range(B_)]
+    Cm = [np.zeros((m, n), complex) for _ in range(B_)]
+    tA, tB = _cm(A, torch.complex64), _cm(Bm, torch.complex64)
+    err = {}
+    old = ctypes.c_int(0)
+    _lib().call("mnp_get_tuning", 4, ctypes.addressof(old))
+    try:
+        for mode in (0, 13, 16):
+            _lib().call("mnp_set_tuning", 4, mode)
+            tC = _cm(Cm, torch.complex64)
+            _lib().call("mnp_gemm_sub_batched", 1, m, n, k, tA.data_ptr(), m, m * k, tB.data_ptr(), k, k * n,
+                        tC.data_ptr(), m, m * n, B_, _sh())
+            torch.cuda.synchronize()
+            e = 0.0
+            for b in range(B_):
+                A64 = A[b].astype(np.complex64).astype(complex)
+                B64 = Bm[b].astype(np.complex64).astype(complex)
+                e = max(e, np.max(np.abs(_np(tC, b) + A64 @ B64) / (np.abs(A64) @ np.abs(B64))))
+            err[mode] = e
+    finally:
+        _lib().call("mnp_set_tuning", 4, old.value)
+    print("max error / (|A||B|): fp32 {:.3g}, 3-term bf16 {:.3g}, fp16 pairs {:.3g}".format(err[0], err[13], err[16]))
+    assert err[16] < 4 * err[0]
+    assert err[13] < 4 * err[0]
+
+
+@pytest.mark.parametrize("split", [12, 13])
+@pytest.mark.parametrize("m,n,k", [(700, 650, 256), (331, 270, 37), (1000, 180, 512)])
+def test_gemm_plane_tiles_identical(split, m, n, k):
+    """The plane kernels with direct LDS copies (tiles 111, 21, 12, 22: 64 x 64, 128 x 64,
+    64 x 128, 128 x 128) form the same term products in the same order per element as the
+    64 x 64 kernel staged through registers (tile 11): bit-identical results."""
+    banner("gemm plane tiles 11 vs 22 split={} m={} n={} k={}".format(split, m, n, k))
+    rng = np.random.default_rng(m + n + k)
+    B_ = 3
+    A = [_rand(rng, m, k) for _ in range(B_)]
+    Bm = [_rand(rng, k, n) for _ in range(B_)]
+    Cm = [_rand(rng, m, n) for _ in range(B_)]
+    tA, tB = _cm(A, torch.complex64), _cm(Bm, torch.complex64)
+    out = {}
+    old, old_t = ctypes.c_int(0), ctypes.c_int(0)
+    _lib().call("mnp_get_tuning", 4, ctypes.addressof(old))
+    _lib().call("mnp_get_tuning", 5, ctypes.addressof(old_t))
+    try:
+        _lib().call("mnp_set_tuning", 4, split)
+        for tile in (11, 111, 21, 12, 22):
+            _lib().call("mnp_set_tuning", 5, tile)
+            tC = _cm(Cm, torch.complex64)
+            _lib().call("mnp_gemm_sub_batched", 1, m, n, k, tA.data_ptr(), m, m * k, tB.data_ptr(), k, k * n,
+                        tC.data_ptr(), m, m * n, B_, _sh())
+            torch.cuda.synchronize()
+            out[tile] = tC.cpu()
+    finally:
+        _lib().call("mnp_set_tuning", 4, old.value)
+        _lib().call("mnp_set_tuning", 5, old_t.value)
+    for tile in (111, 21, 12, 22):
+        assert torch.equal(out[11], out[tile]), tile
+    A64, B64 = A[0].astype(np.complex64).astype(complex), Bm[0].astype(np.complex64).astype(complex)
+    ref = Cm[0].astype(np.complex64).astype(complex) - A64 @ B64
+    got = out[22][0].numpy().T
+    err = np.max(np.abs(got - ref) / (np.abs(A64) @ np.abs(B64) + np.abs(ref)))
+    assert err < (4e-5 if split == 12 else 1e-6), err
 
 
 @pytest.mark.parametrize("n,nb", [(200, 32), (97, 32), (64, 16)])

@@ -35,6 +35,38 @@ def test_lgl_and_rule18_exactness():
             assert abs(np.sum(ws * xs ** a * ys ** b) - exact) < 1e-15
 
 
+@pytest.mark.parametrize("rule,degree,npts", [(1, 1, 1), (2, 1, 3), (3, 2, 3), (4, 3, 4), (7, 5, 7)])
+def test_triangle_rules_and_subdivision_exactness(rule, degree, npts):
+    """Closed-form rules integrate x^a y^b exactly up to their degree, 2 a! b! / (a+b+2)! with
+    sum w = 1, and not one degree higher; subdivision keeps the degree and brings a smooth
+    non-polynomial integrand closer with every level. The product's rules (mnpbem_amd/quadrature.py)
+    are the same point sets."""
+    from mnpbem_amd.quadrature import QuadFace
+    x, y, w = O.triangle_rule(rule)
+    assert len(w) == npts and abs(w.sum() - 1) < 1e-15
+
+    def err(x, y, w, a, b):
+        return abs(np.sum(w * x ** a * y ** b) - 2.0 * factorial(a) * factorial(b) / factorial(a + b + 2))
+    assert max(err(x, y, w, a, d - a) for d in range(degree + 1) for a in range(d + 1)) < 1e-14
+    assert max(err(x, y, w, a, degree + 1 - a) for a in range(degree + 2)) > 1e-6
+    for refine in (2, 3):
+        xs, ys, ws = O.subdivide(x, y, w, refine)
+        assert len(ws) == npts * refine ** 2 and abs(ws.sum() - 1) < 1e-14
+        assert np.all(xs >= -1e-15) and np.all(ys >= -1e-15) and np.all(xs + ys <= 1 + 1e-15)
+        assert max(err(xs, ys, ws, a, d - a) for d in range(degree + 1) for a in range(d + 1)) < 1e-14
+    f = lambda x, y: 1.0 / (0.2 + x + y) ** 2
+    exact = 2.0 * (np.log(1.2 / 0.2) - 1.0 + 0.2 / 1.2)      # 2 * integral of f over the triangle
+    e = [abs(np.sum(ws * f(xs, ys)) - exact) for xs, ys, ws in (O.subdivide(x, y, w, r) for r in (1, 2, 4))]
+    assert e[2] < e[1] < e[0]
+    for refine in (None, 2):
+        q = QuadFace(rule=rule, refine=refine)
+        o = O.subdivide(x, y, w, refine)
+        a = np.lexsort((q.w, q.y, q.x))
+        b = np.lexsort((o[2], o[1], o[0]))
+        assert np.allclose(q.x[a], o[0][b], atol=1e-15) and np.allclose(q.y[a], o[1][b], atol=1e-15)
+        assert np.allclose(q.w[a], o[2][b], atol=1e-16)
+
+
 def test_polar_rules_normalised():
     (x3, y3, w3), (x4, y4, w4) = O.polar_rules((7, 5))
     assert len(w3) == 144 and len(w4) == 192
