@@ -8,10 +8,10 @@ wavelength), `--batch` wavelengths per factorisation (default all 201 at once: a
 of HBM for either precision), inputs resident on the GPU.
 
 value = wavelength solves/s of the mixed-precision path (complex64 LU chain whose GEMMs run
-on the BF16 matrix cores with operands split into 3 bfloat16 terms, as accurate as FP32, +
-FP64 iterative refinement on the exact operator); the complex128 LU path is timed in the same
-run, at the same batch, as the reference point, and the 2-term variant is reported beside it.
-Prints one JSON line.
+on the FP16 matrix cores with operands as scaled fp16 pairs, measured as accurate as the FP32
+matrix cores, + FP64 iterative refinement on the exact operator); the complex128 LU path is
+timed in the same run, at the same batch, as the reference point, and the GEMM forms with 3 and
+2 bfloat16 terms are reported beside it. Prints one JSON line.
 """
 import argparse
 import json
@@ -40,7 +40,7 @@ def _gemm_kernel(tuning, split):
     tile = int(tuning.get("plane_tile", 11))
     mt, nt = {11: (1, 1), 111: (1, 1), 21: (2, 1), 12: (1, 2), 22: (2, 2)}[tile]
     if split == 16:
-        return "k_gemm_sub_c64h<{}, {}> (complex GEMM on the FP16 matrix cores, scaled fp16 pairs)".format(mt, nt)
+        return "k_gemm_sub_c64p<2, true> (complex GEMM on the FP16 matrix cores, scaled fp16 pairs)"
     if tile == 11:
         return "k_gemm_sub_c64p<{}> (complex GEMM on the BF16 matrix cores, {} operand terms)".format(split, split)
     return "k_gemm_sub_c64q<{}, {}, {}> (complex GEMM on the BF16 matrix cores, {} operand terms)".format(
@@ -143,11 +143,15 @@ def config1(skip_cpu=False):
     return out
 
 
-def gmres25k():
+def gmres25k(preconditioner="auto"):
     """Rod dimer (two 20x80 nm capped rods end to end, 2 nm gap, ~24.5k faces), gold Drude in
     vacuum, 750 nm, axial polarisation: BEMRetIter with H-matrices (eta 2.5, leaf 64, ACA tol
     1e-6, kmax 100, low-rank factors complex64) and GMRES(50) to 1e-6. time_to_tol = H-matrix
-    build + preconditioner + GMRES; the host geometry set-up is reported separately."""
+    build + preconditioner + GMRES; the wavelength-independent set-up (meshes, near-field pairs,
+    closed-surface terms, cluster tree; for 'particle' also those of each rod alone) is
+    reported separately.
+    preconditioner: 'auto' = the BEMRet chain of the whole dimer (the reference's choice),
+    'particle' = the chains of the two rods alone as a block-diagonal preconditioner."""
     import torch
     import mnpbem_amd as M
     from mnpbem_amd.bem_iter import BEMRetIter
@@ -157,10 +161,10 @@ def gmres25k():
     g.geo
     exc = M.PlaneWaveRet([0, 0, 1.0], [1.0, 0, 0])
     e = exc(cp, 750.0)
+    it = BEMRetIter(cp, tol=1e-6, maxit=500, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5,
+                    green=g, preconditioner=preconditioner)
     torch.cuda.synchronize()
     setup = time.perf_counter() - t0
-    it = BEMRetIter(cp, tol=1e-6, maxit=500, restart=50, hmatrix=True, htol=1e-6, kmax=100, cleaf=64, eta=2.5,
-                    green=g)
     t0 = time.perf_counter()
     sig, _ = it.solve(e)
     torch.cuda.synchronize()
@@ -168,8 +172,9 @@ def gmres25k():
     flag, relres, niter = it.info()
     st = it._hm.stats
     out = dict(workload="BEMRetIter gold-Drude rod dimer 2x(20x80 nm), 2 nm gap, n={}, 750 nm, axial; H-matrix "
-                        "eta 2.5 leaf 64 htol 1e-6; GMRES(50) to 1e-6, BEMRet-chain preconditioner "
-                        "(complex64)".format(cp.nfaces),
+                        "eta 2.5 leaf 64 htol 1e-6; GMRES(50) to 1e-6, {} (complex64)".format(
+                            cp.nfaces, "BEMRet chain of each rod alone as block-diagonal preconditioner"
+                            if preconditioner == "particle" else "BEMRet-chain preconditioner"),
                faces=cp.nfaces, time_to_tol_s=round(total, 3), hmatrix_build_s=round(it.timings.get("hmatrix", 0), 3),
                precond_s=round(it.timings.get("precond", 0), 3), gmres_s=round(it.timings.get("gmres", 0), 3),
                iterations=int(niter), relres=float(relres), converged=flag == 0, ext=float(exc.extinction(sig)),
@@ -407,9 +412,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=0, help="wavelengths per factorisation (0: all 201 at once)")
-    ap.add_argument("--gemm-split", type=int, default=3, choices=(0, 2, 3, 16),
-                    help="bfloat16 terms per operand in the complex-float GEMMs (0: FP32 matrix cores)")
-    ap.add_argument("--no-mixed2", action="store_true", help="skip the extra 2-term mixed sweep")
+    ap.add_argument("--gemm-split", type=int, default=16, choices=(0, 2, 3, 16),
+                    help="operand form of the complex-float GEMMs: 16 scaled fp16 pairs (FP16 matrix cores), "
+                         "2 or 3 bfloat16 terms (BF16 matrix cores), 0 the FP32 matrix cores")
+    ap.add_argument("--no-mixed2", action="store_true", help="skip the extra mixed sweeps with the other GEMM forms")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="copy one 256 MiB buffer first (a kernel of known traffic for counter scaling)")
     ap.add_argument("--nb", type=int, default=32)
@@ -499,7 +505,7 @@ def main():
     # the same sweep with the other GEMM operand forms: 2 bfloat16 terms (about 16 significant
     # bits) and scaled fp16 pairs (about 22 bits); the main solver's buffers go first (two would
     # not fit at batch 201)
-    for other, tag in ((2, "mixed_2term"), (16, "mixed_f16pair")):
+    for other, tag in ((3, "mixed_3term"), (2, "mixed_2term"), (16, "mixed_f16pair")):
         if args.no_mixed2 or args.gemm_split == other or "double" not in results:
             continue
         solvers["mixed"]._solver = None
@@ -546,7 +552,7 @@ def main():
         "vs_baseline": None,
         "dtype": "c64 LU ({}) + c128 refinement".format(
             {0: "FP32 MFMA", 2: "2-term split-BF16 MFMA", 3: "3-term split-BF16 MFMA, FP32-equivalent",
-             16: "scaled fp16-pair MFMA"}[args.gemm_split]),
+             16: "scaled fp16-pair MFMA, FP32-equivalent"}[args.gemm_split]),
         "data": "synthetic",
         "config": {"workload": "BEMRet gold-Drude capped rod 10x40 nm, n={} curved triangles, eps_out=1.77, "
                                "axial plane wave, 201 wavelengths 500-1000 nm, batch {}".format(n, args.batch),
@@ -557,7 +563,7 @@ def main():
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4),
                      "work": "algorithmic 8 m n k real flops per launch (complex FP32-equivalent)",
-                     "executed_bf16_tflops": round(achieved * products, 3) if args.gemm_split else None,
+                     "executed_16bit_tflops": round(achieved * products, 3) if args.gemm_split else None,
                      "executed_frac": round(achieved * products / peak, 4) if args.gemm_split else None,
                      "frac_of_fp32_peak": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": GEMM_TRAFFIC_BYTES, "traffic_source": GEMM_TRAFFIC_SOURCE,
@@ -579,6 +585,8 @@ def main():
     if not args.no_gmres25k:
         line["gmres25k"] = gmres25k()
         _progress("gmres25k done")
+        line["gmres25k_particle_precond"] = gmres25k("particle")
+        _progress("gmres25k with the per-particle preconditioner done")
     if not args.no_warmstart:
         line["warmstart25k"] = warmstart25k()
         _progress("warmstart25k done")

@@ -280,7 +280,7 @@ def solve_particles(cps, enei_list, pol, direc, precision="mixed", greens=None, 
         t0 = time.perf_counter()
         n = max(nfaces[i] for i in grp)
         gg = GroupGeometry([geos[i] for i in grp], nl, n, geos[grp[0]].dev)
-        tuning = dict(gemm_split=3 if precision != "double" else 0)
+        tuning = dict(gemm_split=16 if precision != "double" else 0)
         if autotune:
             from . import autotune as AT
             fdt = C128 if precision == "double" else C64

@@ -71,11 +71,11 @@ class BEMRet:
         self.max_refine = int(op.get("max_refine", 12))
         # LU shape autotuning (autotune.py): default on for batched sweeps (batch >= 8)
         self.autotune = op.get("autotune", None)
-        # complex-float GEMMs of the mixed chain: operands split into 3 bfloat16 terms (as
-        # accurate as FP32, on the BF16 matrix cores; default), 2 terms (about 16 bits; faster,
-        # the FP64 refinement takes the accuracy back), 16 (scaled fp16 pairs on the FP16 matrix
-        # cores, about 22 bits), or 0 (FP32 matrix cores)
-        self.gemm_split = int(op.get("gemm_split", 3))
+        # complex-float GEMMs of the mixed chain: 16 (default) operands as scaled fp16 pairs on the
+        # FP16 matrix cores (3 products per real product; measured as accurate as the FP32 matrix
+        # cores), 3 bfloat16 terms (6 products, as accurate as FP32), 2 bfloat16 terms (about 16
+        # bits; the FP64 refinement takes the accuracy back), or 0 (the FP32 matrix cores)
+        self.gemm_split = int(op.get("gemm_split", 16))
         if self.gemm_split not in (0, 2, 3, 16):
             raise ValueError("BEMRet: gemm_split must be 0, 2, 3 or 16")
         self.tuning = None

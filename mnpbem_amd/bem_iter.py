@@ -257,6 +257,12 @@ class BEMRetIter:
         self.maxit = int(op.get("maxit", 200))
         self.restart = op.get("restart", None)
         pc = op.get("preconditioner", "auto")
+        if pc not in ("auto", "none", "dense", "hlu_dense", "tree", "hlu_tree", "particle"):
+            raise ValueError("BEMRetIter: unknown preconditioner {!r}".format(pc))
+        # 'particle' (not in the reference): block diagonal over the particles, each block the
+        # BEMRet chain of one particle alone in its media; factored for all particles in one
+        # batch. Cheaper to build (sum of n_p^3 instead of n^3), GMRES resolves the coupling.
+        self.precond_mode = "particle" if pc == "particle" else "chain"
         pr = op.get("precond", None if self.hmatrix else "hmat")
         # with hmatrix=True the reference preconditions with the dense BEMRet chain unless
         # preconditioner='none' (bem_ret_iter.py:1688-1692, :1745-1774)
@@ -275,6 +281,8 @@ class BEMRetIter:
         self.g = op.get("green") or CompGreenRet(p, p, **{k: v for k, v in op.items()
                                                            if k in ("RelCutoff", "AbsCutoff", "order")})
         p._mnp_geo = self.g.geo
+        if self.precond_mode == "particle" and self.use_precond:
+            self._particle_blocks()       # wavelength-independent set-up of the blocks
         self.enei = None
         self.timings = {}     # seconds of the last H-matrix build, preconditioner build and GMRES run
         self._flag = self._relres = self._iter = None
@@ -322,9 +330,12 @@ class BEMRetIter:
                     tuning["plane_tile"] = 12
             elif geo.n >= 4000:
                 tuning["nbo"] = 256
-            self._pre = engine.RetSolverBatch(geo, 1, "single" if single else "double", tuning=tuning)
             self._pre_enei = enei if self.precond_wavelength is None else float(self.precond_wavelength)
             t0 = _now(geo.dev)
+            if self.precond_mode == "particle":
+                self._pre = self._particle_preconditioner(single, tuning)
+            else:
+                self._pre = engine.RetSolverBatch(geo, 1, "single" if single else "double", tuning=tuning)
             self._pre.factor([self._pre_enei])
             if self.precond_inverse:
                 self._pre.invert()
@@ -345,11 +356,59 @@ class BEMRetIter:
              x.data_ptr(), 8 * n, rhs.data_ptr() if rhs is not None else None, 8 * n, y.data_ptr(), 8 * n,
              self._opwork.data_ptr(), 32 * n, None, engine.stream_handle())
 
+    def _particle_blocks(self):
+        """Host and device geometry of each particle of p alone in its media (near-field pairs,
+        closed-surface terms), and the gather indices between the packed 8 N vector and the
+        padded (P, 8, n) batch entries. Wavelength independent: built once."""
+        from .comparticle import ComParticle
+        cps = []
+        for i, part in enumerate(self.p.p):
+            cp = ComParticle(self.p.eps, [part], [self.p.inout[i]], 1)
+            cp.pc.quad_rule = self.p.pc.quad_rule
+            cps.append(cp)
+        greens = [CompGreenRet(cp, cp) for cp in cps]
+        for g in greens:
+            g.geo
+        off = self.p._offsets
+        n = max(cp.nfaces for cp in cps)
+        idx = np.full((len(greens), n), -1, np.int64)
+        for i in range(len(greens)):
+            idx[i, : off[i + 1] - off[i]] = np.arange(off[i], off[i + 1])
+        self._pidx = torch.tensor(idx, device=self.g.geo.dev)
+        self._pmask = self._pidx >= 0
+        self._blocks = (greens, n)
+
+    def _particle_preconditioner(self, single, tuning):
+        """GroupSolver over the particles of p, each alone in its own media (one batch entry per
+        particle, padded to the largest)."""
+        from .batch import GroupGeometry, GroupSolver
+        geo = self.g.geo
+        greens, n = self._blocks
+        tuning = dict(tuning)
+        tuning.pop("plane_tile", None)
+        if n < 12000:
+            tuning.pop("nbo", None)
+            if n >= 4000:
+                tuning["nbo"] = 256
+        gg = GroupGeometry([g.geo for g in greens], 1, n, geo.dev)
+        return GroupSolver(gg, "single" if single else "double", tuning=tuning)
+
     def _apply_M(self, v, z):
         if self._pre is None:
             z.copy_(v)
             return
         n = self.g.geo.n
+        if self.precond_mode == "particle":
+            V = v.view(8, n)
+            P, m = self._pidx.shape
+            rhs = torch.zeros((P, 8, m), dtype=v.dtype, device=v.device)
+            sel = self._pidx.clamp(min=0)
+            rhs[:] = V[:, sel].permute(1, 0, 2)
+            rhs.mul_(self._pmask[:, None, :])
+            x = self._pre.solve(rhs)
+            Z = z.view(8, n)
+            Z[:, self._pidx[self._pmask]] = x.permute(1, 0, 2)[:, self._pmask]
+            return
         z.copy_(self._pre.solve(v.view(1, 8, n)).view(-1))
 
     def _best_start(self, b, guesses):

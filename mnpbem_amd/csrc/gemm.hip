@@ -374,10 +374,12 @@ __device__ inline void xcd_tile(int gm, int& bx, int& by, int& bz) {
 // padding: 48-byte rows, conflict-free 16-byte reads); the next slice is loaded into registers
 // while the current one is multiplied. +Ai (for the -im im product) is one sign flip per
 // fragment in registers.
-template <int NS>
+template <int NS, bool F16 = false>
 __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
                                                        long sPA, const __bf16* __restrict__ PB, int RB, long sPB,
-                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
+                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm,
+                                                       const float* __restrict__ scA = nullptr,
+                                                       const float* __restrict__ scB = nullptr) {
     constexpr int TM = 64, TN = 64;
     constexpr int CA = 4 * NS * TM, CB = 4 * NS * TN;          // 16-byte chunks per k slice
     constexpr int UA = CA / 256, UB = CB / 256;
@@ -442,10 +444,22 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
         for (int p = 0; p < NS; ++p)
 #pragma unroll
             for (int q = 0; q < NS - p; ++q) {
-                cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[p], qr[q], cre, 0, 0, 0);
-                cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[p], qr[q], cim, 0, 0, 0);
-                cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[p], qp[q], cre, 0, 0, 0);
-                cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[p], qi[q], cim, 0, 0, 0);
+                if constexpr (F16) {
+                    // the same bits as fp16 pairs (k_split_f16)
+                    typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+                    const h8 br = __builtin_bit_cast(h8, pr[p]), bi = __builtin_bit_cast(h8, pi[p]);
+                    const h8 ar = __builtin_bit_cast(h8, qr[q]), ai = __builtin_bit_cast(h8, qi[q]);
+                    const h8 ap = __builtin_bit_cast(h8, qp[q]);
+                    cre = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, ar, cre, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, ar, cim, 0, 0, 0);
+                    cre = __builtin_amdgcn_mfma_f32_32x32x16_f16(bi, ap, cre, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f32_32x32x16_f16(br, ai, cim, 0, 0, 0);
+                } else {
+                    cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[p], qr[q], cre, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[p], qr[q], cim, 0, 0, 0);
+                    cre = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[p], qp[q], cre, 0, 0, 0);
+                    cim = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[p], qi[q], cim, 0, 0, 0);
+                }
             }
         __syncthreads();
     }
@@ -456,8 +470,15 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
         if (gi < m && gj < n) {
             cx<float>* c = C + gi + (long)gj * ldc;
             cx<float> v = *c;
-            v.re += cre[r];
-            v.im += cim[r];
+            if constexpr (F16) {
+                // undo the row and column scales (powers of two: exact)
+                const float u = 1.f / (scA[(long)bz * RA + gi] * scB[(long)bz * RB + gj]);
+                v.re += cre[r] * u;
+                v.im += cim[r] * u;
+            } else {
+                v.re += cre[r];
+                v.im += cim[r];
+            }
             *c = v;
         }
     }
@@ -594,16 +615,17 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64q(int m, int n, int K, cons
 
 // ---------------------------------------------------------------------------------------------
 // Complex-float GEMM on the FP16 matrix cores with operands as scaled pairs of half floats
-// (mode 16; the splitting of Ootomo and Yokota, "Recovering single precision accuracy from Tensor
-// Cores while surpassing the FP32 theoretical peak performance", IJHPCA 2022). Each row i of A and
-// column j of B gets a power-of-two scale s (its largest |re|, |im| brought into [2^14, 2^15)),
-// and every scaled value v = s x is held as hi = fp16(v) and lo = fp16((v - hi) 2^11): hi carries
-// 11 significant bits, lo the next 11, so the pair keeps about 22 bits, and the scales keep both
-// in the normal range of fp16 for everything within 2^-28 of the row's (column's) largest entry.
-// A real product keeps hi hi + (hi lo + lo hi) 2^-11 (the dropped lo lo term is 2^-22 of the
-// product): 3 MFMAs instead of the 6 of three bfloat16 terms. hi hi and the cross terms go to
-// separate FP32 accumulators, combined and unscaled in the epilogue (multiplications by powers
-// of two). Planes: the layout of the two-term bfloat16 planes, term 0 = hi, term 1 = lo, fp16.
+// (mode 16; a two-term splitting in the manner of Ootomo and Yokota, "Recovering single
+// precision accuracy from Tensor Cores while surpassing the FP32 theoretical peak performance",
+// IJHPCA 2022, with fp16 terms). Each row i of A and column j of B gets a power-of-two scale s
+// (its largest |re|, |im| brought into [2^14, 2^15)), and every scaled value v = s x is held as
+// hi = fp16(v) and lo = fp16(v - hi): hi carries 11 significant bits and lo the next 11, so the
+// pair keeps about 22 bits; with the scale, lo is a normal fp16 number for every entry down to
+// 2^-17 of its row's (column's) largest, and below that its absolute error stays under 2^-39 of
+// that largest entry. A real product keeps hi hi + hi lo + lo hi (the dropped lo lo is 2^-22 of
+// it): 3 MFMAs, as two bfloat16 terms and half the 6 of three, with FP32 accumulation; the
+// epilogue divides by the two scales (powers of two, exact). The planes have the layout of the
+// two-term bfloat16 planes (term 0 = hi, term 1 = lo) and are read by k_gemm_sub_c64p<2, true>.
 
 // Scales of the rows of A (m x K, column-major): one thread per row. sc[r] = 2^(15 - e) with
 // max_k max(|re|, |im|) in [2^(e-1), 2^e); 1 for a zero row and for the padding rows.
@@ -652,7 +674,7 @@ __global__ __launch_bounds__(256) void k_scale_cols(int cols, int R, int K, cons
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 // Scaled fp16 pair planes: as k_split_planes<2, TRANS> with v = sgn sc[r] x, term 0 = fp16(v),
-// term 1 = fp16((v - term 0) 2^11).
+// term 1 = fp16(v - term 0).
 template <bool TRANS>
 __global__ __launch_bounds__(256) void k_split_f16(int rows, int K, const cx<float>* __restrict__ X, long ldx, long sX,
                                                    _Float16* __restrict__ P, int R, long sP, float sgn,
@@ -699,136 +721,12 @@ __global__ __launch_bounds__(256) void k_split_f16(int rows, int K, const cx<flo
             for (int e = 0; e < 8; ++e) {
                 const float x = f * v[c][8 * hf + e];
                 hi[e] = (_Float16)x;
-                lo[e] = (_Float16)((x - (float)hi[e]) * 2048.f);
+                lo[e] = (_Float16)(x - (float)hi[e]);
             }
             *(h8v*)(P + (((long)(0 * 2 + c) * KS + s) * 2 + hf) * R * 8 + (long)r * 8) = hi;
             *(h8v*)(P + (((long)(1 * 2 + c) * KS + s) * 2 + hf) * R * 8 + (long)r * 8) = lo;
         }
     }
-}
-
-// GEMM on the scaled fp16 pairs: the k_gemm_sub_c64q structure (tiles of (64 MT) x (64 NT), direct
-// LDS copies, two stages) with v_mfma_f32_32x32x16_f16, hi hi and cross-term accumulators, and the
-// epilogue C += (hh + x 2^-11) / (sA_i sB_j).
-template <int MT, int NT>
-__global__ __launch_bounds__(256) void k_gemm_sub_c64h(int m, int n, int K, const _Float16* __restrict__ PA, int RA,
-                                                       long sPA, const _Float16* __restrict__ PB, int RB, long sPB,
-                                                       const float* __restrict__ scA, const float* __restrict__ scB,
-                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
-    constexpr int NS = 2;
-    constexpr int TM = 64 * MT, TN = 64 * NT;
-    constexpr int CA = 2 * NS * 2 * MT, CB = 2 * NS * 2 * NT;
-    constexpr int CH = CA + CB;
-    static_assert(CH % 4 == 0, "chunks split evenly over the four waves");
-    __shared__ __attribute__((aligned(1024))) unsigned char lds[2][CH * 1024];
-    int tx, ty, bz;
-    xcd_tile(gm, tx, ty, bz);
-    PA += (long)bz * sPA;
-    PB += (long)bz * sPB;
-    scA += (long)bz * RA;
-    scB += (long)bz * RB;
-    C += (long)bz * sC;
-    const int KS = (K + 15) / 16;
-    const int row0 = tx * TM, col0 = ty * TN;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave & 1) * 32 * MT, wn = (wave >> 1) * 32 * NT;
-    f16acc hre[MT][NT], him[MT][NT], xre[MT][NT], xim[MT][NT];
-#pragma unroll
-    for (int a = 0; a < MT; ++a)
-#pragma unroll
-        for (int b = 0; b < NT; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) { hre[a][b][r] = 0; him[a][b][r] = 0; xre[a][b][r] = 0; xim[a][b][r] = 0; }
-    auto issue = [&](int s, int st) {
-#pragma unroll
-        for (int i = 0; i < CH / 4; ++i) {
-            const int ch = wave + 4 * i;
-            const bool isb = ch >= CA;
-            const int c = isb ? ch - CA : ch;
-            const int nbk = isb ? NT : MT;
-            const int qc = c / (2 * nbk), h = (c / nbk) & 1, rb = c % nbk;
-            const _Float16* src = isb ? PB + ((((long)qc * KS + s) * 2 + h) * RB + col0 + 64 * rb) * 8
-                                      : PA + ((((long)qc * KS + s) * 2 + h) * RA + row0 + 64 * rb) * 8;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + 8 * lane),
-                                             (__attribute__((address_space(3))) void*)&lds[st][ch * 1024], 16, 0, 0);
-        }
-    };
-    auto fragA = [&](int st, int qc, int tr) {
-        const int r = tr + (lane & 31), h = lane >> 5;
-        return *(const h8v*)&lds[st][(((qc * 2 + h) * MT + (r >> 6)) * 64 + (r & 63)) * 16];
-    };
-    auto fragB = [&](int st, int qc, int tr) {
-        const int r = tr + (lane & 31), h = lane >> 5;
-        return *(const h8v*)&lds[st][((CA + (qc * 2 + h) * NT + (r >> 6)) * 64 + (r & 63)) * 16];
-    };
-    issue(0, 0);
-    for (int s = 0; s < KS; ++s) {
-        const int st = s & 1;
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        if (s + 1 < KS) issue(s + 1, st ^ 1);
-        h8v pr[NT][NS], pi[NT][NS], qr[MT][NS], qi[MT][NS], qp[MT][NS];
-#pragma unroll
-        for (int q = 0; q < NS; ++q) {
-#pragma unroll
-            for (int u = 0; u < NT; ++u) {
-                pr[u][q] = fragB(st, 2 * q, wn + 32 * u);
-                pi[u][q] = fragB(st, 2 * q + 1, wn + 32 * u);
-            }
-#pragma unroll
-            for (int u = 0; u < MT; ++u) {
-                qr[u][q] = fragA(st, 2 * q, wm + 32 * u);
-                qi[u][q] = fragA(st, 2 * q + 1, wm + 32 * u);
-                u4v w = __builtin_bit_cast(u4v, qi[u][q]);
-                w ^= 0x80008000u;
-                qp[u][q] = __builtin_bit_cast(h8v, w);
-            }
-        }
-        // hi hi
-#pragma unroll
-        for (int a = 0; a < MT; ++a)
-#pragma unroll
-            for (int b = 0; b < NT; ++b) {
-                hre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][0], qr[a][0], hre[a][b], 0, 0, 0);
-                him[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][0], qr[a][0], him[a][b], 0, 0, 0);
-            }
-#pragma unroll
-        for (int a = 0; a < MT; ++a)
-#pragma unroll
-            for (int b = 0; b < NT; ++b) {
-                hre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][0], qp[a][0], hre[a][b], 0, 0, 0);
-                him[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][0], qi[a][0], him[a][b], 0, 0, 0);
-            }
-        // cross terms hi lo + lo hi
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int a = 0; a < MT; ++a)
-#pragma unroll
-                for (int b = 0; b < NT; ++b) {
-                    xre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][p], qr[a][1 - p], xre[a][b], 0, 0, 0);
-                    xim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][p], qr[a][1 - p], xim[a][b], 0, 0, 0);
-                    xre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pi[b][p], qp[a][1 - p], xre[a][b], 0, 0, 0);
-                    xim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pr[b][p], qi[a][1 - p], xim[a][b], 0, 0, 0);
-                }
-    }
-#pragma unroll
-    for (int a = 0; a < MT; ++a)
-#pragma unroll
-        for (int b = 0; b < NT; ++b)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int gi = row0 + wm + 32 * a + (lane & 31);
-                const int gj = col0 + wn + 32 * b + Mfma<float>::drow(lane, r);
-                if (gi < m && gj < n) {
-                    const float u = 1.f / (scA[gi] * scB[gj]);
-                    cx<float>* c = C + gi + (long)gj * ldc;
-                    cx<float> v = *c;
-                    v.re += (hre[a][b][r] + xre[a][b][r] * (1.f / 2048.f)) * u;
-                    v.im += (him[a][b][r] + xim[a][b][r] * (1.f / 2048.f)) * u;
-                    *c = v;
-                }
-            }
 }
 
 // Plane workspace of the calling thread's handle for stream s, grown on demand.
@@ -891,16 +789,9 @@ static bool gemm_f16(int m, int n, int k, const void* A, long lda, long sA, cons
     hipLaunchKernelGGL((k_split_f16<true>), dim3((RB + 255) / 256, KS, batch), dim3(256), 0, s, n, k,
                        (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, scB, vb);
     if (rec) hipEventRecord(rec->a, s);
-    const int tile = ctx().plane_tile;
-    if (tile == 21)
-        hipLaunchKernelGGL((k_gemm_sub_c64h<2, 1>), dim3((m + 127) / 128, (n + 63) / 64, batch), dim3(256), 0, s, m, n,
-                           k, PA, RA, sPA, PB, RB, sPB, scA, scB, (cx<float>*)C, ldc, sC, ctx().plane_group);
-    else if (tile == 12)
-        hipLaunchKernelGGL((k_gemm_sub_c64h<1, 2>), dim3((m + 63) / 64, (n + 127) / 128, batch), dim3(256), 0, s, m, n,
-                           k, PA, RA, sPA, PB, RB, sPB, scA, scB, (cx<float>*)C, ldc, sC, ctx().plane_group);
-    else
-        hipLaunchKernelGGL((k_gemm_sub_c64h<1, 1>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n,
-                           k, PA, RA, sPA, PB, RB, sPB, scA, scB, (cx<float>*)C, ldc, sC, ctx().plane_group);
+    hipLaunchKernelGGL((k_gemm_sub_c64p<2, true>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n, k,
+                       (const __bf16*)PA, RA, sPA, (const __bf16*)PB, RB, sPB, (cx<float>*)C, ldc, sC,
+                       ctx().plane_group, scA, scB);
     return true;
 }
 

@@ -148,10 +148,11 @@ def test_drude_sphere_sweep_vs_oracle(sphere):
     assert np.all(np.isfinite(ext))
 
 
-@pytest.mark.parametrize("split", [0, 2, 3])
+@pytest.mark.parametrize("split", [0, 2, 3, 16])
 def test_mixed_precision_refinement(sphere, split):
-    """complex64 factorisation (GEMMs on the FP32 matrix cores, or on the BF16 ones with 2 or 3
-    operand terms) + FP64 refinement reaches the complex128 answer (1e-8 on ext)."""
+    """complex64 factorisation (GEMMs on the FP32 matrix cores, on the BF16 ones with 2 or 3
+    operand terms, or on the FP16 ones with scaled pairs) + FP64 refinement reaches the
+    complex128 answer (1e-8 on ext)."""
     cp, s, fp, closed = sphere
     lams = np.linspace(400, 800, 9)
     banner("mixed precision sphere gemm_split={}".format(split))

@@ -78,7 +78,7 @@ def test_rod2496_sweep_ext_matches_oracle(rod2496):
         assert abs(ext[i] - ref["ext"][q]) / abs(ref["ext"][q]) < 1e-8
 
 
-@pytest.mark.parametrize("split", [3, 2])
+@pytest.mark.parametrize("split", [16, 3, 2])
 def test_rod2496_mixed_solution_matches_oracle(rod2496, split):
     """Per-wavelength API in mixed precision: sig1, h1, sig2, h2 and ext/sca vs the oracle."""
     cp, exc, lams, ext, sel, ref = rod2496
