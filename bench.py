@@ -418,6 +418,8 @@ def main():
     ap.add_argument("--no-mixed2", action="store_true", help="skip the extra mixed sweeps with the other GEMM forms")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="copy one 256 MiB buffer first (a kernel of known traffic for counter scaling)")
+    ap.add_argument("--refine-tol", type=float, default=1e-12,
+                    help="relative residual at which the FP64 refinement of the mixed sweep stops")
     ap.add_argument("--nb", type=int, default=32)
     ap.add_argument("--cpu-sample", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -454,7 +456,7 @@ def main():
         return bem.solve_spectrum(exc, lams, precision=prec)
 
     solvers = {"mixed": M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g,
-                                 gemm_split=args.gemm_split)}
+                                 gemm_split=args.gemm_split, refine_tol=args.refine_tol)}
     if not args.skip_fp64:
         solvers["double"] = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="double", green=g)
 
@@ -470,15 +472,19 @@ def main():
             out = sweep(prec)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        results[prec] = (dt, out)
         _progress("{} sweep: {:.1f} solves/s".format(prec, args.steps * len(lams) / dt))
+        if prec == "double":
+            # one more sweep, untimed, for the solutions themselves (copied to the host)
+            out = solvers[prec].solve_spectrum(exc, lams, precision=prec, solutions=True)
+        results[prec] = (dt, out)
         solvers_tuning[prec] = solvers[prec].tuning
         solvers[prec]._solver = None          # the next precision gets the memory (large batches)
         torch.cuda.empty_cache()
 
     # GEMM (dominant kernel) timing with HIP events over one extra mixed sweep
+    # (its solutions are kept for the surface charge / current error against the FP64 sweep)
     _lib.call("mnp_prof_enable", 1)
-    sweep("mixed")
+    sol_mixed = solvers["mixed"].solve_spectrum(exc, lams, precision="mixed", solutions=True)["solutions"]
     torch.cuda.synchronize()
     prof = np.zeros(4)
     _lib.call("mnp_prof_collect", prof.ctypes.data)
@@ -502,6 +508,15 @@ def main():
         rel = np.max(np.abs(out_mixed["ext"] - out_d["ext"]) / np.abs(out_d["ext"]))
         extra = dict(fp64_solves_per_s=round(v_d, 3), speedup_vs_fp64=round(value / v_d, 3),
                      ext_rel_err_vs_fp64=float(rel))
+        # per wavelength |x_mixed - x_fp64| / |x_fp64| over each of sigma1, h1, sigma2, h2 (the
+        # packed rows 0, 1-3, 4, 5-7), largest over the sweep
+        xd = out_d["solutions"]
+        errs = {}
+        for name, rows in (("sig1", [0]), ("h1", [1, 2, 3]), ("sig2", [4]), ("h2", [5, 6, 7])):
+            num = np.linalg.norm((sol_mixed[:, :, rows] - xd[:, :, rows]).reshape(len(lams), -1), axis=1)
+            den = np.linalg.norm(xd[:, :, rows].reshape(len(lams), -1), axis=1)
+            errs[name] = float(np.max(num / den))
+        extra["surface_rel_err_vs_fp64"] = errs
     # the same sweep with the other GEMM operand forms: 2 bfloat16 terms (about 16 significant
     # bits) and scaled fp16 pairs (about 22 bits); the main solver's buffers go first (two would
     # not fit at batch 201)
@@ -510,7 +525,8 @@ def main():
             continue
         solvers["mixed"]._solver = None
         torch.cuda.empty_cache()
-        b2 = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g, gemm_split=other)
+        b2 = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g, gemm_split=other,
+                      refine_tol=args.refine_tol)
         b2.solve_spectrum(exc, lams, precision="mixed")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -532,6 +548,8 @@ def main():
     flops_per_solve = 34.67 * n ** 3
     extra["path_tflops_mixed"] = round(value * flops_per_solve / 1e12, 3)
     extra["refine_iterations_max"] = int(max(out_mixed.get("refine_iterations", [0]) or [0]))
+    extra["refine_tol"] = args.refine_tol
+    extra["refine_relres_history_max"] = [float("{:.3g}".format(v)) for v in out_mixed.get("refine_history", [])]
     extra["fp64_fallbacks"] = len(out_mixed.get("fallback", []))
 
     cpu = None

@@ -366,17 +366,22 @@ int mnp_probe_mfma_bf16(float* out, void* stream);
  * reference). key 0: outer block width of the blocked LU and right-side solves (16..1024
  * columns, default 128); 1: lookahead (trailing update split over two streams) on/off; 2:
  * register-resident panel for complex float on/off; 3: panel height from which the
- * multi-workgroup panel is used (>= 256, default 4096); 4: complex-float GEMM on the FP32 matrix cores (0,
- * default) or on the BF16 matrix cores with operands split once into planes of 2 or 3 bfloat16
- * terms (12, 13; 2 terms keep about 16 significant bits, 3 about 24, as FP32); 5: workgroup
- * tile of the plane GEMM (11: 64 x 64, the only one); 6: row tiles per group of the plane
- * GEMM's XCD-aware tile order (0: plain grid order); 7: rows per workgroup of the Green-function
+ * multi-workgroup panel is used (>= 256, default 4096); 4: complex-float GEMM on the FP32 matrix
+ * cores (0, default of the handle) or on the 16-bit matrix cores with operands split once into
+ * planes: 2 or 3 bfloat16 terms (12, 13; 2 terms keep about 16 significant bits, 3 about 24, as
+ * FP32) or scaled fp16 pairs (16: rows of A and columns of B scaled by powers of two, about 22
+ * bits, 3 products per real product); 5: workgroup tile of the bfloat16 plane GEMM (11: 64 x 64
+ * staged through registers; 111, 21, 12, 22: 64 x 64, 128 x 64, 64 x 128, 128 x 128 with direct
+ * LDS copies; all give bit-identical results); 6: row tiles per group of the plane GEMM's
+ * XCD-aware tile order (0: plain grid order); 7: rows per workgroup of the Green-function
  * assembly (64, 128, 256; the workgroup covers 256 / rows columns); 8: wavelengths per assembly
  * thread (0: all of the launch; shorter chunks put more workgroups on small matrices); 9:
  * multi-workgroup panels by tournament pivoting (1, default: each panel's pivot rows chosen by
- * a reduction tree of partial-pivoting eliminations over blocks of 512 (complex float) or 256
+ * a reduction tree of partial-pivoting eliminations over blocks of 128 (complex float) or 256
  * (complex double) rows, no barrier between workgroups) or by partial pivoting with a
- * cooperative launch (0: LAPACK's pivots, one grid barrier per column).
+ * cooperative launch (0: LAPACK's pivots, one grid barrier per column); 10: outer blocks
+ * factored by recursive halving (1, default: the trailing columns of the block are updated by
+ * GEMMs of depth W/2, W/4, ..., nb) or panel by panel (0: W/nb GEMMs of depth nb).
  * mnpbem_amd.autotune picks them per problem shape by timing. */
 int mnp_set_tuning(int key, int value);
 int mnp_get_tuning(int key, int* value);

@@ -20,7 +20,8 @@ import torch
 from ._lib import call
 
 _CACHE = {}
-_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4, "plane_tile": 5, "asm_rows": 7, "asm_lchunk": 8}
+_KEYS = {"nbo": 0, "lookahead": 1, "gemm_split": 4, "plane_tile": 5, "asm_rows": 7, "asm_lchunk": 8,
+         "lu_recursive": 10}
 
 
 def _value(key, v):
@@ -42,7 +43,7 @@ def _get(key):
 @contextlib.contextmanager
 def knobs(cfg):
     """Set the library's knobs from cfg (keys 'nbo', 'lookahead', 'gemm_split', 'plane_tile',
-    'asm_rows', 'asm_lchunk') for the duration."""
+    'asm_rows', 'asm_lchunk', 'lu_recursive') for the duration."""
     if not cfg:
         yield
         return
@@ -58,13 +59,21 @@ def knobs(cfg):
 
 
 def candidates(n=0):
-    """(nb, nbo, lookahead) shapes to time. Wider outer blocks halve the passes of the trailing
-    update over the matrix and deepen its K (split-BF16 GEMM at 64 x 2240^2: K 128 -> 132,
-    K 256 -> 158 TFLOP/s complex-equivalent) at the price of more in-block work; for large n the
-    16-column panel never wins."""
+    """(nb, nbo, lookahead, lu_recursive) shapes to time. Wider outer blocks halve the passes of
+    the trailing update over the matrix and deepen its K (split-BF16 GEMM at 64 x 2240^2: K 128 ->
+    132, K 256 -> 158 TFLOP/s complex-equivalent) at the price of more in-block work, which the
+    recursive order of the outer block (GEMMs of depth W/2, W/4, ... instead of W/nb GEMMs of
+    depth nb) keeps small; for large n the 16-column panel never wins."""
     nbs = (32,) if n >= 1024 else (16, 32)
     nbos = (64, 128, 256, 384, 512) if n >= 1024 else (64, 128, 256)
-    return [dict(nb=nb, nbo=nbo, lookahead=la) for nb in nbs for nbo in nbos for la in (0, 1) if nbo >= nb]
+    out = []
+    for nb in nbs:
+        for nbo in nbos + ((768, 1024) if n >= 2048 else ()):
+            for rec in (0, 1):
+                if nbo < nb or (nbo > 512 and not rec) or (rec and nbo < 2 * nb):
+                    continue
+                out += [dict(nb=nb, nbo=nbo, lookahead=la, lu_recursive=rec) for la in (0, 1)]
+    return out
 
 
 def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):

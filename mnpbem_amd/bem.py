@@ -183,7 +183,7 @@ class BEMRet:
         npol = exc.pol.shape[0]
         ext = np.zeros((len(enei_list), npol))
         sca = np.zeros((len(enei_list), npol)) if scattering else None
-        refine_iters, relres, fallback = [], [], []
+        refine_iters, relres, fallback, history = [], [], [], []
         sols = np.zeros((len(enei_list), npol, 8, geo.n), complex) if solutions else None
         sp = exc.spec
         for b0 in range(0, len(enei_list), batch):
@@ -200,6 +200,7 @@ class BEMRet:
                 if precision == "mixed":
                     refine_iters.append(s.stats["iterations"])
                     relres.append(s.stats["relres"])
+                    history.append(s.stats["history"])
                     if len(s.stats["fallback"]):
                         fb = s.stats["fallback"]
                         fallback.extend(list(el[fb]))
@@ -219,7 +220,10 @@ class BEMRet:
         if solutions:
             out["solutions"] = sols
         if precision == "mixed":
-            out.update(refine_iterations=refine_iters, relres=relres, fallback=fallback)
+            # relres before each refinement step and after the last: the largest over the wavelengths
+            steps = max(len(h) for h in history)
+            hmax = [float(max(np.max(h[i]) for h in history if len(h) > i)) for i in range(steps)]
+            out.update(refine_iterations=refine_iters, relres=relres, fallback=fallback, refine_history=hmax)
         return out
 
     def _redo_double(self, el, idx, rhs, x):

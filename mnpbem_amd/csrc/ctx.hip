@@ -24,7 +24,8 @@ Ctx::Ctx()
       plane_group(env_int("MNP_GEMM_GROUP", 8)),
       asm_rows(env_int("MNP_ASM_ROWS", 64)),
       asm_lchunk(env_int("MNP_ASM_LCHUNK", 0)),
-      tall_panel(env_int("MNP_TALL_PANEL", 1)) {}
+      tall_panel(env_int("MNP_TALL_PANEL", 1)),
+      lu_recursive(env_int("MNP_LU_RECURSIVE", 1)) {}
 
 hipEvent_t Ctx::event() {
     if (!pool.empty()) {

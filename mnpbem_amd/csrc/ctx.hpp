@@ -37,6 +37,7 @@ struct Ctx {
     int asm_rows;          // 7: rows per workgroup of the Green-function assembly
     int asm_lchunk;        // 8: wavelengths per assembly thread (0: all)
     int tall_panel;        // 9: tall panels by tournament pivoting (1) or cooperative partial pivoting (0)
+    int lu_recursive;      // 10: outer panels factored by recursive halving (1) or panel by panel (0)
     // cooperative-panel scratch (candidates, diagonal rows, barrier counters)
     void* coop_ws = nullptr;
     size_t coop_bytes = 0;

@@ -1023,13 +1023,14 @@ static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, 
 
 // Two-level blocking: outer blocks of NBO columns carry the large trailing update (a GEMM with
 // k = NBO, so C is read and written once per NBO columns), inner panels of nb columns are
-// factored by k_panel inside the outer block.
+// factored inside the outer block, by recursive halving (default) or one after the other.
 // Tuning knobs of the calling thread's handle: 0 = outer block width (columns, >= nb), 1 =
 // lookahead on/off, 2 = register-resident panel on/off, 3 = rows from which panels use the
 // cooperative kernel, 4 = complex-float GEMM mode (0: FP32 matrix cores; 12, 13: split-BF16
-// planes of 2 or 3 terms), 5 = workgroup tile of the plane GEMM (11 only), 6 = row tiles per
-// group of its XCD-aware tile order (0: grid order), 7 = rows per workgroup of the Green-function
-// assembly (64, 128, 256), 8 = wavelengths per assembly thread (0: all).
+// planes of 2 or 3 terms; 16: scaled fp16 pairs), 5 = workgroup tile of the plane GEMM (11,
+// 111, 21, 12, 22), 6 = row tiles per group of its XCD-aware tile order (0: grid order), 7 =
+// rows per workgroup of the Green-function assembly (64, 128, 256), 8 = wavelengths per assembly
+// thread (0: all), 9 = tall panels by tournament pivoting, 10 = recursive outer blocks.
 int set_gemm_split(int v);
 int get_gemm_split();
 int set_plane_tile(int v);
@@ -1043,6 +1044,7 @@ int set_tuning(int key, int value) {
         case 7: return set_asm_rows(value);
         case 8: return set_asm_lchunk(value);
         case 9: if (value != 0 && value != 1) return -2; ctx().tall_panel = value; return 0;
+        case 10: if (value != 0 && value != 1) return -2; ctx().lu_recursive = value; return 0;
         case 4: return set_gemm_split(value);
         case 5: return set_plane_tile(value);
         case 6: return set_plane_group(value);
@@ -1065,6 +1067,7 @@ int get_tuning(int key) {
         case 7: return ctx().asm_rows;
         case 8: return ctx().asm_lchunk;
         case 9: return ctx().tall_panel;
+        case 10: return ctx().lu_recursive;
         default: return -1;
     }
 }
@@ -1116,8 +1119,9 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
         const int W = (n - J0 < NB) ? (n - J0) : NB;
         const int JE = J0 + W;
         // factor the tall outer panel A[J0:n, J0:JE]
-        for (int j0 = J0; j0 < JE; j0 += nb) {
-            int w = (JE - j0 < nb) ? (JE - j0) : nb;
+        // one panel of w <= nb columns at j0, its interchanges applied to the other columns of the
+        // outer panel
+        auto panel = [&](int j0, int w) {
             const bool tall = coop && (h.tall_panel == 1 ? panel_calu<T>(n, j0, w, a, lda, sA, ipiv, sP, info,
                                                                           batch, ws, s)
                                                          : panel_coop<T>(n, j0, w, a, lda, sA, ipiv, sP, info,
@@ -1132,11 +1136,47 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
                                        sP, info);
             }
             laswp<T>(j0, j0 + w, J0, j0, j0 + w, JE, a, lda, sA, ipiv, sP, batch, s);
-            int rest = JE - j0 - w;
-            if (rest > 0) {
-                trsm_lln<T>(j0 + w, JE, j0, w, nb, a, lda, sA, batch, s);
-                gemm_sub<T>(n - j0 - w, rest, w, a + (j0 + w) + (long)j0 * lda, lda, sA, a + j0 + (long)(j0 + w) * lda,
-                            lda, sA, a + (j0 + w) + (long)(j0 + w) * lda, lda, sA, batch, s);
+        };
+        // U12 = L11^{-1} A12 for the rows [r0, r1) and columns [c0, c1), nb rows at a time
+        auto solve_u12 = [&](int r0, int r1, int c0, int c1) {
+            for (int j0 = r0; j0 < r1; j0 += nb) {
+                int w = (r1 - j0 < nb) ? (r1 - j0) : nb;
+                trsm_lln<T>(c0, c1, j0, w, nb, a, lda, sA, batch, s);
+                if (j0 + w < r1)
+                    gemm_sub<T>(r1 - j0 - w, c1 - c0, w, a + (j0 + w) + (long)j0 * lda, lda, sA,
+                                a + j0 + (long)c0 * lda, lda, sA, a + (j0 + w) + (long)c0 * lda, lda, sA, batch, s);
+            }
+        };
+        if (h.lu_recursive) {
+            // columns [c0, c1): the left half recursively, then the right half's rows c0 .. mid by
+            // L11 and its rows below by one GEMM of depth mid - c0, then the right half. Each
+            // trailing column of the outer panel is read and written log2(W / nb) times by
+            // GEMMs of depth W / 2, W / 4, ... instead of W / nb times by GEMMs of depth nb.
+            auto rec = [&](auto&& self, int c0, int c1) -> void {
+                const int np = (c1 - c0 + nb - 1) / nb;
+                if (np <= 1) {
+                    panel(c0, c1 - c0);
+                    return;
+                }
+                const int mid = c0 + ((np + 1) / 2) * nb;
+                self(self, c0, mid);
+                solve_u12(c0, mid, mid, c1);
+                gemm_sub<T>(n - mid, c1 - mid, mid - c0, a + mid + (long)c0 * lda, lda, sA, a + c0 + (long)mid * lda,
+                            lda, sA, a + mid + (long)mid * lda, lda, sA, batch, s);
+                self(self, mid, c1);
+            };
+            rec(rec, J0, JE);
+        } else {
+            for (int j0 = J0; j0 < JE; j0 += nb) {
+                int w = (JE - j0 < nb) ? (JE - j0) : nb;
+                panel(j0, w);
+                int rest = JE - j0 - w;
+                if (rest > 0) {
+                    trsm_lln<T>(j0 + w, JE, j0, w, nb, a, lda, sA, batch, s);
+                    gemm_sub<T>(n - j0 - w, rest, w, a + (j0 + w) + (long)j0 * lda, lda, sA,
+                                a + j0 + (long)(j0 + w) * lda, lda, sA, a + (j0 + w) + (long)(j0 + w) * lda, lda, sA,
+                                batch, s);
+                }
             }
         }
         // interchanges of the outer panel on the columns outside it (after the previous block's
@@ -1147,14 +1187,8 @@ void getrf_batched(int n, int batch, void* A, long lda, long sA, int* ipiv, long
         }
         laswp<T>(J0, JE, 0, J0, JE, n, a, lda, sA, ipiv, sP, batch, s);
         if (JE < n) {
-            // U12 = L11^{-1} A12 for the W x (n - JE) block row, nb rows at a time
-            for (int j0 = J0; j0 < JE; j0 += nb) {
-                int w = (JE - j0 < nb) ? (JE - j0) : nb;
-                trsm_lln<T>(JE, n, j0, w, nb, a, lda, sA, batch, s);
-                if (j0 + w < JE)
-                    gemm_sub<T>(JE - j0 - w, n - JE, w, a + (j0 + w) + (long)j0 * lda, lda, sA,
-                                a + j0 + (long)JE * lda, lda, sA, a + (j0 + w) + (long)JE * lda, lda, sA, batch, s);
-            }
+            // U12 = L11^{-1} A12 for the W x (n - JE) block row
+            solve_u12(J0, JE, JE, n);
             // trailing update with k = W: the next block's columns here, the rest on the side stream
             const int first = lookahead ? ((n - JE < NB) ? (n - JE) : NB) : n - JE;
             gemm_sub<T>(n - JE, first, W, a + JE + (long)J0 * lda, lda, sA, a + J0 + (long)JE * lda, lda, sA,
@@ -1307,16 +1341,47 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
         else
             hipLaunchKernelGGL((k_rtrsm_lower_unit<T, 64>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
     };
+    const bool rec = ctx().lu_recursive;
+    // phase 1 inside an outer block, columns [c0, c1): left half, B[:, right] -= B[:, left]
+    // U[left, right] (one GEMM of depth mid - c0), right half; the diagonal blocks by k_rtrsm
+    auto upper_rec = [&](auto&& self, int c0, int c1) -> void {
+        const int np = (c1 - c0 + nb - 1) / nb;
+        if (np <= 1) {
+            upper(c0, c1 - c0);
+            return;
+        }
+        const int mid = c0 + ((np + 1) / 2) * nb;
+        self(self, c0, mid);
+        gemm_sub<T>(m, c1 - mid, mid - c0, bm + (long)c0 * ldb, ldb, sB, a + c0 + (long)mid * lda, lda, sA,
+                    bm + (long)mid * ldb, ldb, sB, batch, s);
+        self(self, mid, c1);
+    };
+    // phase 2 inside an outer block: right half, B[:, left] -= B[:, right] L[right, left], left half
+    auto lower_rec = [&](auto&& self, int c0, int c1) -> void {
+        const int np = (c1 - c0 + nb - 1) / nb;
+        if (np <= 1) {
+            lower(c0, c1 - c0);
+            return;
+        }
+        const int mid = c0 + ((np + 1) / 2) * nb;
+        self(self, mid, c1);
+        gemm_sub<T>(m, mid - c0, c1 - mid, bm + (long)mid * ldb, ldb, sB, a + mid + (long)c0 * lda, lda, sA,
+                    bm + (long)c0 * ldb, ldb, sB, batch, s);
+        self(self, c0, mid);
+    };
     // phase 1: Y U = B, outer blocks left to right
     for (int J0 = 0; J0 < n; J0 += NB) {
         const int JE = (n - J0 < NB) ? n : J0 + NB;
-        for (int j0 = J0; j0 < JE; j0 += nb) {
-            int w = (JE - j0 < nb) ? (JE - j0) : nb;
-            upper(j0, w);
-            if (j0 + w < JE)
-                gemm_sub<T>(m, JE - j0 - w, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)(j0 + w) * lda, lda, sA,
-                            bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
-        }
+        if (rec)
+            upper_rec(upper_rec, J0, JE);
+        else
+            for (int j0 = J0; j0 < JE; j0 += nb) {
+                int w = (JE - j0 < nb) ? (JE - j0) : nb;
+                upper(j0, w);
+                if (j0 + w < JE)
+                    gemm_sub<T>(m, JE - j0 - w, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)(j0 + w) * lda, lda,
+                                sA, bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
+            }
         if (JE < n)
             gemm_sub<T>(m, n - JE, JE - J0, bm + (long)J0 * ldb, ldb, sB, a + J0 + (long)JE * lda, lda, sA,
                         bm + (long)JE * ldb, ldb, sB, batch, s);
@@ -1326,13 +1391,16 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
     for (int J0 = lastJ; J0 >= 0; J0 -= NB) {
         const int JE = (n - J0 < NB) ? n : J0 + NB;
         const int lastj = J0 + ((JE - J0 - 1) / nb) * nb;
-        for (int j0 = lastj; j0 >= J0; j0 -= nb) {
-            int w = (JE - j0 < nb) ? (JE - j0) : nb;
-            lower(j0, w);
-            if (j0 > J0)
-                gemm_sub<T>(m, j0 - J0, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)J0 * lda, lda, sA,
-                            bm + (long)J0 * ldb, ldb, sB, batch, s);
-        }
+        if (rec)
+            lower_rec(lower_rec, J0, JE);
+        else
+            for (int j0 = lastj; j0 >= J0; j0 -= nb) {
+                int w = (JE - j0 < nb) ? (JE - j0) : nb;
+                lower(j0, w);
+                if (j0 > J0)
+                    gemm_sub<T>(m, j0 - J0, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)J0 * lda, lda, sA,
+                                bm + (long)J0 * ldb, ldb, sB, batch, s);
+            }
         if (J0 > 0)
             gemm_sub<T>(m, J0, JE - J0, bm + (long)J0 * ldb, ldb, sB, a + J0, lda, sA, bm, ldb, sB, batch, s);
     }

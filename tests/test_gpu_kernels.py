@@ -212,6 +212,73 @@ def test_getrf_matches_lapack(n, nb):
         assert np.max(np.abs(_np(t, b) - lu)) < 1e-11 * np.max(np.abs(lu))
 
 
+@pytest.mark.parametrize("rec", [0, 1])
+@pytest.mark.parametrize("n,nbo", [(600, 256), (777, 384), (1100, 512)])
+def test_getrf_outer_block_orders_match_lapack(n, nbo, rec):
+    """Outer blocks factored panel by panel (0) or by recursive halving (1; uneven halves when
+    the block has an odd number of panels) on random matrices, which interchange rows at almost
+    every column: LAPACK's pivots in complex128 and its factors to rounding."""
+    banner("getrf n={} nbo={} recursive={}".format(n, nbo, rec))
+    rng = np.random.default_rng(n + rec)
+    B = 2
+    A = [_rand(rng, n, n) for _ in range(B)]
+    t = _cm(A, torch.complex128)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    old = [ctypes.c_int(0), ctypes.c_int(0)]
+    _lib().call("mnp_get_tuning", 0, ctypes.addressof(old[0]))
+    _lib().call("mnp_get_tuning", 10, ctypes.addressof(old[1]))
+    try:
+        _lib().call("mnp_set_tuning", 0, nbo)
+        _lib().call("mnp_set_tuning", 10, rec)
+        _lib().call("mnp_getrf_batched", 0, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
+                    _sh())
+        torch.cuda.synchronize()
+    finally:
+        _lib().call("mnp_set_tuning", 0, old[0].value)
+        _lib().call("mnp_set_tuning", 10, old[1].value)
+    assert np.all(info.cpu().numpy() == 0)
+    for b in range(B):
+        lu, piv = sla.lu_factor(A[b])
+        assert np.array_equal(ipiv[b].cpu().numpy(), piv)
+        assert np.max(np.abs(_np(t, b) - lu)) < 1e-11 * np.max(np.abs(lu))
+
+
+@pytest.mark.parametrize("rec", [0, 1])
+@pytest.mark.parametrize("m,n,nbo", [(500, 777, 384), (301, 640, 256)])
+def test_rsolve_outer_block_orders(m, n, nbo, rec):
+    """Right-side solve X = B A^{-1} with outer blocks solved panel by panel (0) or by recursive
+    halving (1), B of m rows: numpy's B inv(A) in complex128."""
+    banner("rsolve m={} n={} nbo={} recursive={}".format(m, n, nbo, rec))
+    rng = np.random.default_rng(m + n + rec)
+    A = _rand(rng, n, n)
+    Bm = _rand(rng, m, n)
+    t = _cm([A], torch.complex128)
+    ipiv = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(1, dtype=torch.int32, device=_dev())
+    pf = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    q = torch.zeros((1, n), dtype=torch.int32, device=_dev())
+    tb = _cm([Bm], torch.complex128)
+    tX = torch.zeros_like(tb)
+    old = [ctypes.c_int(0), ctypes.c_int(0)]
+    _lib().call("mnp_get_tuning", 0, ctypes.addressof(old[0]))
+    _lib().call("mnp_get_tuning", 10, ctypes.addressof(old[1]))
+    try:
+        _lib().call("mnp_set_tuning", 0, nbo)
+        _lib().call("mnp_set_tuning", 10, rec)
+        _lib().call("mnp_getrf_batched", 0, n, 1, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
+                    _sh())
+        _lib().call("mnp_perm_batched", n, 1, ipiv.data_ptr(), n, pf.data_ptr(), q.data_ptr(), n, _sh())
+        _lib().call("mnp_rsolve_batched", 0, m, n, 1, t.data_ptr(), n, n * n, q.data_ptr(), n, tb.data_ptr(), m, m * n,
+                    tX.data_ptr(), m, m * n, 32, _sh())
+        torch.cuda.synchronize()
+    finally:
+        _lib().call("mnp_set_tuning", 0, old[0].value)
+        _lib().call("mnp_set_tuning", 10, old[1].value)
+    X = Bm @ np.linalg.inv(A)
+    assert np.max(np.abs(_np(tX, 0) - X)) / np.max(np.abs(X)) < 1e-11
+
+
 def test_getrf_single_backward_error():
     banner("getrf complex64")
     rng = np.random.default_rng(5)
@@ -425,15 +492,15 @@ def test_autotune_picks_a_candidate_and_restores_knobs():
     import ctypes
     banner("lu autotune n=300 batch 4")
     before = []
-    for key in (0, 1):
+    for key in (0, 1, 10):
         v = ctypes.c_int(0)
         _lib().call("mnp_get_tuning", key, ctypes.addressof(v))
         before.append(v.value)
     cfg = autotune.tune_lu(300, 4, torch.complex64, _dev())
-    assert {k: cfg[k] for k in ("nb", "nbo", "lookahead")} in autotune.candidates(300)
+    assert {k: cfg[k] for k in ("nb", "nbo", "lookahead", "lu_recursive")} in autotune.candidates(300)
     assert cfg["ms"] > 0
     after = []
-    for key in (0, 1):
+    for key in (0, 1, 10):
         v = ctypes.c_int(0)
         _lib().call("mnp_get_tuning", key, ctypes.addressof(v))
         after.append(v.value)
