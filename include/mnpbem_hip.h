@@ -381,7 +381,9 @@ int mnp_probe_mfma_bf16(float* out, void* stream);
  * (complex double) rows, no barrier between workgroups) or by partial pivoting with a
  * cooperative launch (0: LAPACK's pivots, one grid barrier per column); 10: outer blocks
  * factored by recursive halving (1, default: the trailing columns of the block are updated by
- * GEMMs of depth W/2, W/4, ..., nb) or panel by panel (0: W/nb GEMMs of depth nb).
+ * GEMMs of depth W/2, W/4, ..., nb) or panel by panel (0: W/nb GEMMs of depth nb); 11: fp16-pair
+ * operands with K <= 512 scaled and split in one pass over the operand (1, default) or in two
+ * (0); both write the same bits.
  * mnpbem_amd.autotune picks them per problem shape by timing. */
 int mnp_set_tuning(int key, int value);
 int mnp_get_tuning(int key, int* value);

@@ -685,12 +685,12 @@ int mnp_probe_mfma_bf16(float* out, void* stream) {
 
 // ------------------------------------------------------------------------- tuning knobs
 int mnp_set_tuning(int key, int value) {
-    REQUIRE(key >= 0 && key <= 10, 1);
+    REQUIRE(key >= 0 && key <= 11, 1);
     return set_tuning(key, value) == 0 ? 0 : -2;
 }
 
 int mnp_get_tuning(int key, int* value) {
-    REQUIRE(key >= 0 && key <= 10, 1);
+    REQUIRE(key >= 0 && key <= 11, 1);
     REQUIRE(value, 2);
     *value = get_tuning(key);
     return 0;

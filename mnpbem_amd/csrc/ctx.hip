@@ -25,7 +25,8 @@ Ctx::Ctx()
       asm_rows(env_int("MNP_ASM_ROWS", 64)),
       asm_lchunk(env_int("MNP_ASM_LCHUNK", 0)),
       tall_panel(env_int("MNP_TALL_PANEL", 1)),
-      lu_recursive(env_int("MNP_LU_RECURSIVE", 1)) {}
+      lu_recursive(env_int("MNP_LU_RECURSIVE", 1)),
+      f16_fused(env_int("MNP_F16_FUSED", 1)) {}
 
 hipEvent_t Ctx::event() {
     if (!pool.empty()) {

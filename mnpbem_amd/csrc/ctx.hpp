@@ -38,6 +38,7 @@ struct Ctx {
     int asm_lchunk;        // 8: wavelengths per assembly thread (0: all)
     int tall_panel;        // 9: tall panels by tournament pivoting (1) or cooperative partial pivoting (0)
     int lu_recursive;      // 10: outer panels factored by recursive halving (1) or panel by panel (0)
+    int f16_fused;         // 11: fp16-pair operands scaled and split in one pass (1) or two (0)
     // cooperative-panel scratch (candidates, diagonal rows, barrier counters)
     void* coop_ws = nullptr;
     size_t coop_bytes = 0;

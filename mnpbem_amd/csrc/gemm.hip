@@ -729,6 +729,103 @@ __global__ __launch_bounds__(256) void k_split_f16(int rows, int K, const cx<flo
     }
 }
 
+// Scales and fp16-pair planes in one pass over the operand, for K <= 64 HPT: a workgroup of 256
+// lanes takes 32 plane rows (rows of A; TRANS: columns of B, K x n column-major) and all of K.
+// Lane t holds row t % 32 and the half slices (8 consecutive k) h = t / 32 + 8 j, j < HPT, in
+// registers; the row's largest |re|, |im| is reduced over its 8 lanes in LDS; then the scale is
+// written as k_scale_rows / k_scale_cols write it and the planes as k_split_f16 writes them
+// (bit-identical), so the operand is read once instead of twice.
+template <bool TRANS, int HPT>
+__global__ __launch_bounds__(256) void k_scale_split_f16(int rows, int K, const cx<float>* __restrict__ X, long ldx,
+                                                         long sX, _Float16* __restrict__ P, int R, long sP, float sgn,
+                                                         float* __restrict__ sc, int vec) {
+    __shared__ float red[8][32];
+    const int t = threadIdx.x, rl = t & 31, g = t >> 5;
+    const int r = blockIdx.x * 32 + rl;          // < R: the grid covers R / 32 blocks, R a multiple of 128
+    X += (long)blockIdx.y * sX;
+    P += (long)blockIdx.y * sP;
+    sc += (long)blockIdx.y * R;
+    const int KS = (K + 15) / 16, NH = 2 * KS;
+    float v[HPT][2][8];
+    float mx = 0.f;
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+        const int h = g + 8 * j, k0 = 8 * h;
+        if (TRANS && vec && r < rows && k0 + 8 <= K) {
+            const float4* p = (const float4*)(X + k0 + (long)r * ldx);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float4 w = p[e];
+                v[j][0][2 * e] = w.x;
+                v[j][1][2 * e] = w.y;
+                v[j][0][2 * e + 1] = w.z;
+                v[j][1][2 * e + 1] = w.w;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + e;
+                v[j][0][e] = 0.f;
+                v[j][1][e] = 0.f;
+                if (h < NH && r < rows && k < K) {
+                    const cx<float> w = TRANS ? X[k + (long)r * ldx] : X[r + (long)k * ldx];
+                    v[j][0][e] = w.re;
+                    v[j][1][e] = w.im;
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fmaxf(fabsf(v[j][0][e]), fabsf(v[j][1][e])));
+    }
+    red[g][rl] = mx;
+    __syncthreads();
+    float m = red[0][rl];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) m = fmaxf(m, red[i][rl]);
+    int ex = 0;
+    frexpf(m, &ex);
+    ex = ex < -100 ? -100 : ex;
+    const float scale = m > 0.f ? ldexpf(1.f, 15 - ex) : 1.f;
+    if (g == 0) sc[r] = scale;
+    const float f = sgn * scale;
+#pragma unroll
+    for (int j = 0; j < HPT; ++j) {
+        const int h = g + 8 * j;
+        if (h >= NH) break;
+        const int sl = h >> 1, hf = h & 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            h8v hi, lo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float x = f * v[j][c][e];
+                hi[e] = (_Float16)x;
+                lo[e] = (_Float16)(x - (float)hi[e]);
+            }
+            *(h8v*)(P + (((long)(0 * 2 + c) * KS + sl) * 2 + hf) * R * 8 + (long)r * 8) = hi;
+            *(h8v*)(P + (((long)(1 * 2 + c) * KS + sl) * 2 + hf) * R * 8 + (long)r * 8) = lo;
+        }
+    }
+}
+
+template <bool TRANS>
+static bool scale_split_f16(int rows, int R, int K, const cx<float>* X, long ldx, long sX, _Float16* P, long sP,
+                            float sgn, float* sc, int vec, int batch, hipStream_t s) {
+    const int NH = 2 * ((K + 15) / 16);
+    const dim3 g(R / 32, batch);
+    if (NH <= 8)
+        hipLaunchKernelGGL((k_scale_split_f16<TRANS, 1>), g, dim3(256), 0, s, rows, K, X, ldx, sX, P, R, sP, sgn, sc, vec);
+    else if (NH <= 16)
+        hipLaunchKernelGGL((k_scale_split_f16<TRANS, 2>), g, dim3(256), 0, s, rows, K, X, ldx, sX, P, R, sP, sgn, sc, vec);
+    else if (NH <= 32)
+        hipLaunchKernelGGL((k_scale_split_f16<TRANS, 4>), g, dim3(256), 0, s, rows, K, X, ldx, sX, P, R, sP, sgn, sc, vec);
+    else if (NH <= 64)
+        hipLaunchKernelGGL((k_scale_split_f16<TRANS, 8>), g, dim3(256), 0, s, rows, K, X, ldx, sX, P, R, sP, sgn, sc, vec);
+    else
+        return false;
+    return true;
+}
+
 // Plane workspace of the calling thread's handle for stream s, grown on demand.
 static void* plane_ws(hipStream_t s, size_t bytes) {
     Ctx& c = ctx();
@@ -780,14 +877,21 @@ static bool gemm_f16(int m, int n, int k, const void* A, long lda, long sA, cons
     float* scA = (float*)(PB + sPB * batch);
     float* scB = scA + (long)RA * batch;
     const int vb = ((ldb | sB) & 1) == 0 && ((uintptr_t)B & 15) == 0;
-    hipLaunchKernelGGL(k_scale_rows, dim3((RA + 255) / 256, batch), dim3(256), 0, s, m, RA, k, (const cx<float>*)A, lda,
-                       sA, scA);
-    hipLaunchKernelGGL(k_scale_cols, dim3((RB + 3) / 4, batch), dim3(256), 0, s, n, RB, k, (const cx<float>*)B, ldb,
-                       sB, scB);
-    hipLaunchKernelGGL((k_split_f16<false>), dim3((RA + 255) / 256, KS, batch), dim3(256), 0, s, m, k,
-                       (const cx<float>*)A, lda, sA, PA, RA, sPA, -1.0f, scA, 0);
-    hipLaunchKernelGGL((k_split_f16<true>), dim3((RB + 255) / 256, KS, batch), dim3(256), 0, s, n, k,
-                       (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, scB, vb);
+    // one pass per operand while its K fits the registers (K <= 512), else scales then planes
+    if (!(ctx().f16_fused && scale_split_f16<false>(m, RA, k, (const cx<float>*)A, lda, sA, PA, sPA, -1.0f, scA, 0,
+                                                     batch, s))) {
+        hipLaunchKernelGGL(k_scale_rows, dim3((RA + 255) / 256, batch), dim3(256), 0, s, m, RA, k,
+                           (const cx<float>*)A, lda, sA, scA);
+        hipLaunchKernelGGL((k_split_f16<false>), dim3((RA + 255) / 256, KS, batch), dim3(256), 0, s, m, k,
+                           (const cx<float>*)A, lda, sA, PA, RA, sPA, -1.0f, scA, 0);
+    }
+    if (!(ctx().f16_fused && scale_split_f16<true>(n, RB, k, (const cx<float>*)B, ldb, sB, PB, sPB, 1.0f, scB, vb,
+                                                    batch, s))) {
+        hipLaunchKernelGGL(k_scale_cols, dim3((RB + 3) / 4, batch), dim3(256), 0, s, n, RB, k,
+                           (const cx<float>*)B, ldb, sB, scB);
+        hipLaunchKernelGGL((k_split_f16<true>), dim3((RB + 255) / 256, KS, batch), dim3(256), 0, s, n, k,
+                           (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, scB, vb);
+    }
     if (rec) hipEventRecord(rec->a, s);
     hipLaunchKernelGGL((k_gemm_sub_c64p<2, true>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n, k,
                        (const __bf16*)PA, RA, sPA, (const __bf16*)PB, RB, sPB, (cx<float>*)C, ldc, sC,

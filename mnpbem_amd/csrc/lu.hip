@@ -1030,7 +1030,8 @@ static void trsm_lln(int c0, int c1, int j0, int w, int nb, cx<T>* a, long lda, 
 // planes of 2 or 3 terms; 16: scaled fp16 pairs), 5 = workgroup tile of the plane GEMM (11,
 // 111, 21, 12, 22), 6 = row tiles per group of its XCD-aware tile order (0: grid order), 7 =
 // rows per workgroup of the Green-function assembly (64, 128, 256), 8 = wavelengths per assembly
-// thread (0: all), 9 = tall panels by tournament pivoting, 10 = recursive outer blocks.
+// thread (0: all), 9 = tall panels by tournament pivoting, 10 = recursive outer blocks, 11 =
+// one-pass scale and split of fp16-pair operands.
 int set_gemm_split(int v);
 int get_gemm_split();
 int set_plane_tile(int v);
@@ -1045,6 +1046,7 @@ int set_tuning(int key, int value) {
         case 8: return set_asm_lchunk(value);
         case 9: if (value != 0 && value != 1) return -2; ctx().tall_panel = value; return 0;
         case 10: if (value != 0 && value != 1) return -2; ctx().lu_recursive = value; return 0;
+        case 11: if (value != 0 && value != 1) return -2; ctx().f16_fused = value; return 0;
         case 4: return set_gemm_split(value);
         case 5: return set_plane_tile(value);
         case 6: return set_plane_group(value);
@@ -1068,6 +1070,7 @@ int get_tuning(int key) {
         case 8: return ctx().asm_lchunk;
         case 9: return ctx().tall_panel;
         case 10: return ctx().lu_recursive;
+        case 11: return ctx().f16_fused;
         default: return -1;
     }
 }

@@ -157,6 +157,41 @@ def test_gemm_fp16_pairs_as_accurate_as_fp32():
     assert err[13] < 4 * err[0]
 
 
+@pytest.mark.parametrize("m,n,k,lda,ldb", [(300, 260, 37, 301, 37), (640, 576, 128, 640, 128), (331, 270, 256, 332, 258),
+                                           (500, 300, 500, 500, 501), (260, 200, 600, 260, 600), (200, 170, 64, 200, 64)])
+def test_gemm_fp16_pairs_one_pass_identical(m, n, k, lda, ldb):
+    """Scaled fp16 pairs: scales and planes formed in one pass over each operand (K <= 512) or in
+    two (scale, then split) give bit-identical products; K = 600 takes the two-pass form in both."""
+    banner("gemm fp16 pairs one pass m={} n={} k={}".format(m, n, k))
+    rng = np.random.default_rng(k)
+    A = [_rand(rng, lda, k) * 10.0 ** rng.uniform(-4, 4, (lda, 1)) for _ in range(2)]
+    Bm = [_rand(rng, ldb, n) * 10.0 ** rng.uniform(-4, 4, (1, n)) for _ in range(2)]
+    Cm = [_rand(rng, m, n) for _ in range(2)]
+    tA, tB = _cm(A, torch.complex64), _cm(Bm, torch.complex64)
+    old = [ctypes.c_int(0), ctypes.c_int(0)]
+    _lib().call("mnp_get_tuning", 4, ctypes.addressof(old[0]))
+    _lib().call("mnp_get_tuning", 11, ctypes.addressof(old[1]))
+    out = {}
+    try:
+        _lib().call("mnp_set_tuning", 4, 16)
+        for fused in (0, 1):
+            _lib().call("mnp_set_tuning", 11, fused)
+            tC = _cm(Cm, torch.complex64)
+            _lib().call("mnp_gemm_sub_batched", 1, m, n, k, tA.data_ptr(), lda, lda * k, tB.data_ptr(), ldb, ldb * n,
+                        tC.data_ptr(), m, m * n, 2, _sh())
+            torch.cuda.synchronize()
+            out[fused] = tC.cpu()
+    finally:
+        _lib().call("mnp_set_tuning", 4, old[0].value)
+        _lib().call("mnp_set_tuning", 11, old[1].value)
+    assert torch.equal(out[0], out[1])
+    A64 = A[0][:m].astype(np.complex64).astype(complex)
+    B64 = Bm[0][:k].astype(np.complex64).astype(complex)
+    ref = Cm[0].astype(np.complex64).astype(complex) - A64 @ B64
+    err = np.max(np.abs(out[1][0].numpy().T - ref) / (np.abs(A64) @ np.abs(B64) + np.abs(ref)))
+    assert err < 1e-6, err
+
+
 @pytest.mark.parametrize("split", [12, 13])
 @pytest.mark.parametrize("m,n,k", [(700, 650, 256), (331, 270, 37), (1000, 180, 512)])
 def test_gemm_plane_tiles_identical(split, m, n, k):
@@ -244,21 +279,24 @@ def test_getrf_outer_block_orders_match_lapack(n, nbo, rec):
         assert np.max(np.abs(_np(t, b) - lu)) < 1e-11 * np.max(np.abs(lu))
 
 
+@pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("rec", [0, 1])
 @pytest.mark.parametrize("m,n,nbo", [(500, 777, 384), (301, 640, 256)])
-def test_rsolve_outer_block_orders(m, n, nbo, rec):
+def test_rsolve_outer_block_orders(m, n, nbo, rec, prec):
     """Right-side solve X = B A^{-1} with outer blocks solved panel by panel (0) or by recursive
-    halving (1), B of m rows: numpy's B inv(A) in complex128."""
-    banner("rsolve m={} n={} nbo={} recursive={}".format(m, n, nbo, rec))
+    halving (1), B of m rows: numpy's B inv(A) (complex128 to 1e-11, complex64 to 5e-4 on these
+    random matrices)."""
+    banner("rsolve m={} n={} nbo={} recursive={} prec={}".format(m, n, nbo, rec, prec))
     rng = np.random.default_rng(m + n + rec)
-    A = _rand(rng, n, n)
+    A = _rand(rng, n, n) + 0.1 * n * np.eye(n) * prec
     Bm = _rand(rng, m, n)
-    t = _cm([A], torch.complex128)
+    dt = torch.complex128 if prec == 0 else torch.complex64
+    t = _cm([A], dt)
     ipiv = torch.zeros((1, n), dtype=torch.int32, device=_dev())
     info = torch.zeros(1, dtype=torch.int32, device=_dev())
     pf = torch.zeros((1, n), dtype=torch.int32, device=_dev())
     q = torch.zeros((1, n), dtype=torch.int32, device=_dev())
-    tb = _cm([Bm], torch.complex128)
+    tb = _cm([Bm], dt)
     tX = torch.zeros_like(tb)
     old = [ctypes.c_int(0), ctypes.c_int(0)]
     _lib().call("mnp_get_tuning", 0, ctypes.addressof(old[0]))
@@ -266,17 +304,17 @@ def test_rsolve_outer_block_orders(m, n, nbo, rec):
     try:
         _lib().call("mnp_set_tuning", 0, nbo)
         _lib().call("mnp_set_tuning", 10, rec)
-        _lib().call("mnp_getrf_batched", 0, n, 1, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
+        _lib().call("mnp_getrf_batched", prec, n, 1, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32,
                     _sh())
         _lib().call("mnp_perm_batched", n, 1, ipiv.data_ptr(), n, pf.data_ptr(), q.data_ptr(), n, _sh())
-        _lib().call("mnp_rsolve_batched", 0, m, n, 1, t.data_ptr(), n, n * n, q.data_ptr(), n, tb.data_ptr(), m, m * n,
-                    tX.data_ptr(), m, m * n, 32, _sh())
+        _lib().call("mnp_rsolve_batched", prec, m, n, 1, t.data_ptr(), n, n * n, q.data_ptr(), n, tb.data_ptr(), m,
+                    m * n, tX.data_ptr(), m, m * n, 32, _sh())
         torch.cuda.synchronize()
     finally:
         _lib().call("mnp_set_tuning", 0, old[0].value)
         _lib().call("mnp_set_tuning", 10, old[1].value)
     X = Bm @ np.linalg.inv(A)
-    assert np.max(np.abs(_np(tX, 0) - X)) / np.max(np.abs(X)) < 1e-11
+    assert np.max(np.abs(_np(tX, 0) - X)) / np.max(np.abs(X)) < (1e-11 if prec == 0 else 5e-4)
 
 
 def test_getrf_single_backward_error():
