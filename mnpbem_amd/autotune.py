@@ -110,15 +110,15 @@ def tune_lu(n, batch, dtype, dev, reps=1, seed=0, gemm_split=0):
     tried = []
     for cfg in candidates(n):
         cfg = dict(cfg, gemm_split=gemm_split)
-        if gemm_split in (2, 3) and prec == 1:
+        if gemm_split in (2, 3, 16) and prec == 1:
             cfg["plane_tile"] = 11
         ms = timed(cfg)
         tried.append((ms, cfg))
         if best is None or ms < best["ms"]:
             best = dict(cfg, ms=round(ms, 3))
-    if gemm_split in (2, 3) and prec == 1:
+    if gemm_split in (2, 3, 16) and prec == 1:
         # the plane kernels with direct LDS copies (64 x 64, 128 x 64, 64 x 128 tiles) on the two
-        # fastest shapes (the fp16 pairs have one kernel)
+        # fastest shapes
         for _, cfg in sorted(tried, key=lambda e: e[0])[:2]:
             for tile in (111, 21, 12):
                 c2 = dict(cfg, plane_tile=tile)

@@ -30,7 +30,7 @@ bool lu_reserve(int, int, int, hipStream_t);
 void perm_batched(int, int, const int*, long, int*, int*, long, hipStream_t);
 template <typename T>
 void rsolve_batched(int, int, int, const void*, long, long, const int*, long, void*, long, long, void*, long, long, int,
-                    hipStream_t);
+                    hipStream_t, bool identity = false);
 template <typename T>
 void getrs_batched(int, int, int, const void*, long, long, const int*, long, const void*, long, long, void*, long,
                    long, hipStream_t);
@@ -366,7 +366,7 @@ int mnp_bemret_factor(int prec, int n, int batch, void* G1, void* G2, void* H1, 
         getrf_batched<T>(n, batch, H1, ld, stride, ipD, sP, info + 2 * batch, nb, s);
         perm_batched(n, batch, ipD, sP, pfD, qD, sQ, s);
         identity_batched<T>(n, batch, H2, ld, stride, s);
-        rsolve_batched<T>(n, n, batch, H1, ld, stride, qD, sQ, H2, ld, stride, D, ld, stride, nb, s);
+        rsolve_batched<T>(n, n, batch, H1, ld, stride, qD, sQ, H2, ld, stride, D, ld, stride, nb, s, true);
         sigma_batched<T>(n, batch, S1, S2, D, ld, stride, nvec, sface, L1, L2, kvac, s);
         getrf_batched<T>(n, batch, S2, ld, stride, ipS, sP, info + 3 * batch, nb, s);
         perm_batched(n, batch, ipS, sP, pfS, qS, sQ, s);

@@ -496,10 +496,12 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64p(int m, int n, int K, cons
 // (conflict-free). Each accumulator receives the term products of k_gemm_sub_c64p in the same
 // order: results are identical. The MFMA tiles take turns, so that an accumulator is written
 // again only MT NT MFMAs later.
-template <int NS, int MT, int NT>
+template <int NS, int MT, int NT, bool F16 = false>
 __global__ __launch_bounds__(256) void k_gemm_sub_c64q(int m, int n, int K, const __bf16* __restrict__ PA, int RA,
                                                        long sPA, const __bf16* __restrict__ PB, int RB, long sPB,
-                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm) {
+                                                       cx<float>* __restrict__ C, long ldc, long sC, int gm,
+                                                       const float* __restrict__ scA = nullptr,
+                                                       const float* __restrict__ scB = nullptr) {
     constexpr int TM = 64 * MT, TN = 64 * NT;
     constexpr int CA = 2 * NS * 2 * MT, CB = 2 * NS * 2 * NT;    // 1 KB chunks per slice: plane x half x block
     constexpr int CH = CA + CB;
@@ -569,6 +571,17 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64q(int m, int n, int K, cons
                 qp[u][q] = __builtin_bit_cast(bf8v, w);
             }
         }
+        // one real product of two operand planes into an accumulator: bfloat16 terms, or the
+        // same bits read as fp16 pairs (k_split_f16)
+        auto mfma = [](bf8v x, bf8v y, f16acc acc) {
+            if constexpr (F16) {
+                typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+                return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(h8, x), __builtin_bit_cast(h8, y),
+                                                              acc, 0, 0, 0);
+            } else {
+                return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, acc, 0, 0, 0);
+            }
+        };
 #pragma unroll
         for (int p = 0; p < NS; ++p)
 #pragma unroll
@@ -577,22 +590,22 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64q(int m, int n, int K, cons
                 for (int a = 0; a < MT; ++a)
 #pragma unroll
                     for (int b = 0; b < NT; ++b)
-                        cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[b][p], qr[a][q], cre[a][b], 0, 0, 0);
+                        cre[a][b] = mfma(pr[b][p], qr[a][q], cre[a][b]);
 #pragma unroll
                 for (int a = 0; a < MT; ++a)
 #pragma unroll
                     for (int b = 0; b < NT; ++b)
-                        cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[b][p], qr[a][q], cim[a][b], 0, 0, 0);
+                        cim[a][b] = mfma(pi[b][p], qr[a][q], cim[a][b]);
 #pragma unroll
                 for (int a = 0; a < MT; ++a)
 #pragma unroll
                     for (int b = 0; b < NT; ++b)
-                        cre[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pi[b][p], qp[a][q], cre[a][b], 0, 0, 0);
+                        cre[a][b] = mfma(pi[b][p], qp[a][q], cre[a][b]);
 #pragma unroll
                 for (int a = 0; a < MT; ++a)
 #pragma unroll
                     for (int b = 0; b < NT; ++b)
-                        cim[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pr[b][p], qi[a][q], cim[a][b], 0, 0, 0);
+                        cim[a][b] = mfma(pr[b][p], qi[a][q], cim[a][b]);
             }
     }
 #pragma unroll
@@ -606,8 +619,15 @@ __global__ __launch_bounds__(256) void k_gemm_sub_c64q(int m, int n, int K, cons
                 if (gi < m && gj < n) {
                     cx<float>* c = C + gi + (long)gj * ldc;
                     cx<float> v = *c;
-                    v.re += cre[a][b][r];
-                    v.im += cim[a][b][r];
+                    if constexpr (F16) {
+                        // undo the row and column scales (powers of two: exact)
+                        const float u = 1.f / (scA[(long)bz * RA + gi] * scB[(long)bz * RB + gj]);
+                        v.re += cre[a][b][r] * u;
+                        v.im += cim[a][b][r] * u;
+                    } else {
+                        v.re += cre[a][b][r];
+                        v.im += cim[a][b][r];
+                    }
                     *c = v;
                 }
             }
@@ -893,9 +913,30 @@ static bool gemm_f16(int m, int n, int k, const void* A, long lda, long sA, cons
                            (const cx<float>*)B, ldb, sB, PB, RB, sPB, 1.0f, scB, vb);
     }
     if (rec) hipEventRecord(rec->a, s);
-    hipLaunchKernelGGL((k_gemm_sub_c64p<2, true>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s, m, n, k,
-                       (const __bf16*)PA, RA, sPA, (const __bf16*)PB, RB, sPB, (cx<float>*)C, ldc, sC,
-                       ctx().plane_group, scA, scB);
+    const __bf16 *pa = (const __bf16*)PA, *pb = (const __bf16*)PB;
+    const int gm = ctx().plane_group;
+    // the plane tile knob (11: 64 x 64 through registers; 111, 21, 12, 22: direct LDS copies)
+    switch (ctx().plane_tile) {
+        case 111:
+            hipLaunchKernelGGL((k_gemm_sub_c64q<2, 1, 1, true>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256),
+                               0, s, m, n, k, pa, RA, sPA, pb, RB, sPB, (cx<float>*)C, ldc, sC, gm, scA, scB);
+            break;
+        case 21:
+            hipLaunchKernelGGL((k_gemm_sub_c64q<2, 2, 1, true>), dim3((m + 127) / 128, (n + 63) / 64, batch),
+                               dim3(256), 0, s, m, n, k, pa, RA, sPA, pb, RB, sPB, (cx<float>*)C, ldc, sC, gm, scA, scB);
+            break;
+        case 12:
+            hipLaunchKernelGGL((k_gemm_sub_c64q<2, 1, 2, true>), dim3((m + 63) / 64, (n + 127) / 128, batch),
+                               dim3(256), 0, s, m, n, k, pa, RA, sPA, pb, RB, sPB, (cx<float>*)C, ldc, sC, gm, scA, scB);
+            break;
+        case 22:
+            hipLaunchKernelGGL((k_gemm_sub_c64q<2, 2, 2, true>), dim3((m + 127) / 128, (n + 127) / 128, batch),
+                               dim3(256), 0, s, m, n, k, pa, RA, sPA, pb, RB, sPB, (cx<float>*)C, ldc, sC, gm, scA, scB);
+            break;
+        default:
+            hipLaunchKernelGGL((k_gemm_sub_c64p<2, true>), dim3((m + 63) / 64, (n + 63) / 64, batch), dim3(256), 0, s,
+                               m, n, k, pa, RA, sPA, pb, RB, sPB, (cx<float>*)C, ldc, sC, gm, scA, scB);
+    }
     return true;
 }
 

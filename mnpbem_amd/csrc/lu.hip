@@ -130,7 +130,6 @@ __global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>*
     __shared__ cx<T> ublk[RP_MAXW][CW];
     __shared__ int sw_p[RP_MAXW];          // pivot row chosen for panel column c (global index)
     __shared__ int aff[2 * CW], src_s[2 * CW], naff_s;
-    __shared__ int piv_s;
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     A += (long)b * sA;
     ipiv += (long)b * sP;
@@ -205,18 +204,24 @@ __global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>*
                 }
                 if (lane == 0) { red_v[wv] = best; red_i[wv] = bi; }
                 __syncthreads();
-                if (t == 0) {
-                    T bv = red_v[0];
-                    int bx = red_i[0];
-                    for (int q = 1; q < NW; ++q)
-                        if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) { bv = red_v[q]; bx = red_i[q]; }
-                    if (bx >= n) bx = col;
-                    piv_s = bx;
-                    sw_p[c0 + cc] = bx;
-                    ipiv[col] = bx;
+                // every wave reduces the NW wave maxima itself (lanes 0 .. NW-1, same comparison),
+                // so the pivot needs no second barrier to reach the other waves
+                {
+                    T bv = lane < NW ? red_v[lane] : (T)-1;
+                    int bx = lane < NW ? red_i[lane] : n;
+                    for (int off = NW / 2; off > 0; off >>= 1) {
+                        T ov = __shfl_xor(bv, off, 64);
+                        int oi = __shfl_xor(bx, off, 64);
+                        if (ov > bv || (ov == bv && oi < bx)) { bv = ov; bx = oi; }
+                    }
+                    bi = __builtin_amdgcn_readfirstlane(bx);
                 }
-                __syncthreads();
-                const int p = piv_s;
+                if (bi >= n) bi = col;
+                if (t == 0) {
+                    sw_p[c0 + cc] = bi;
+                    ipiv[col] = bi;
+                }
+                const int p = bi;
                 // rows col and p trade places (chunk columns)
 #pragma unroll
                 for (int q = 0; q < RP_RPT; ++q) {
@@ -255,7 +260,9 @@ __global__ __launch_bounds__(RP_T) void k_panel_reg(int n, int j0, int w, cx<T>*
                         }
                     }
                 }
-                __syncthreads();
+                // no barrier here: the next column writes red_v / red_i only after the barrier
+                // above (every read of them is before it) and urow / crow only after its own
+                // first barrier (every read of this column's is before that)
             }
         }
         // 4. write back; then this chunk's interchanges on the earlier chunks' columns
@@ -1326,17 +1333,23 @@ __global__ __launch_bounds__(256) void k_gather_cols(int m, int n, const cx<T>* 
 
 template <typename T>
 void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, const int* qinv, long sQ, void* B,
-                    long ldb, long sB, void* X, long ldx, long sX, int nb, hipStream_t s) {
+                    long ldb, long sB, void* X, long ldx, long sX, int nb, hipStream_t s, bool identity) {
     const cx<T>* a = (const cx<T>*)LU;
     cx<T>* bm = (cx<T>*)B;
     dim3 g((m + 127) / 128, batch);
     const int nbo = ctx().nbo;
     const int NB = (nbo > nb) ? (nbo / nb) * nb : nb;
+    // identity: B is the n x n identity (the explicit inverse), so in phase 1 the column block
+    // [c0, c1) is zero in every row from c1 on (Y = U^{-1} is upper triangular): its solve and
+    // the updates it feeds touch rows < c1 only, a third of phase 1's work
+    auto rows = [&](int c1) { return identity && c1 < m ? c1 : m; };
     auto upper = [&](int j0, int w) {
+        const int mr = rows(j0 + w);
+        const dim3 gu((mr + 127) / 128, batch);
         if (nb <= 32)
-            hipLaunchKernelGGL((k_rtrsm_upper<T, 32>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
+            hipLaunchKernelGGL((k_rtrsm_upper<T, 32>), gu, dim3(128), 0, s, mr, j0, w, a, lda, sA, bm, ldb, sB);
         else
-            hipLaunchKernelGGL((k_rtrsm_upper<T, 64>), g, dim3(128), 0, s, m, j0, w, a, lda, sA, bm, ldb, sB);
+            hipLaunchKernelGGL((k_rtrsm_upper<T, 64>), gu, dim3(128), 0, s, mr, j0, w, a, lda, sA, bm, ldb, sB);
     };
     auto lower = [&](int j0, int w) {
         if (nb <= 32)
@@ -1355,7 +1368,7 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
         }
         const int mid = c0 + ((np + 1) / 2) * nb;
         self(self, c0, mid);
-        gemm_sub<T>(m, c1 - mid, mid - c0, bm + (long)c0 * ldb, ldb, sB, a + c0 + (long)mid * lda, lda, sA,
+        gemm_sub<T>(rows(mid), c1 - mid, mid - c0, bm + (long)c0 * ldb, ldb, sB, a + c0 + (long)mid * lda, lda, sA,
                     bm + (long)mid * ldb, ldb, sB, batch, s);
         self(self, mid, c1);
     };
@@ -1382,11 +1395,11 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
                 int w = (JE - j0 < nb) ? (JE - j0) : nb;
                 upper(j0, w);
                 if (j0 + w < JE)
-                    gemm_sub<T>(m, JE - j0 - w, w, bm + (long)j0 * ldb, ldb, sB, a + j0 + (long)(j0 + w) * lda, lda,
-                                sA, bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
+                    gemm_sub<T>(rows(j0 + w), JE - j0 - w, w, bm + (long)j0 * ldb, ldb, sB,
+                                a + j0 + (long)(j0 + w) * lda, lda, sA, bm + (long)(j0 + w) * ldb, ldb, sB, batch, s);
             }
         if (JE < n)
-            gemm_sub<T>(m, n - JE, JE - J0, bm + (long)J0 * ldb, ldb, sB, a + J0 + (long)JE * lda, lda, sA,
+            gemm_sub<T>(rows(JE), n - JE, JE - J0, bm + (long)J0 * ldb, ldb, sB, a + J0 + (long)JE * lda, lda, sA,
                         bm + (long)JE * ldb, ldb, sB, batch, s);
     }
     // phase 2: Z L = Y, outer blocks right to left
@@ -1412,9 +1425,9 @@ void rsolve_batched(int m, int n, int batch, const void* LU, long lda, long sA, 
 }
 
 template void rsolve_batched<double>(int, int, int, const void*, long, long, const int*, long, void*, long, long,
-                                     void*, long, long, int, hipStream_t);
+                                     void*, long, long, int, hipStream_t, bool);
 template void rsolve_batched<float>(int, int, int, const void*, long, long, const int*, long, void*, long, long,
-                                    void*, long, long, int, hipStream_t);
+                                    void*, long, long, int, hipStream_t, bool);
 
 // ---------------------------------------------------------------------------------------------
 // Left solve with a few right-hand sides: X = A^{-1} B = U^{-1} L^{-1} (B[pf]).

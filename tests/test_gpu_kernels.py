@@ -192,12 +192,13 @@ def test_gemm_fp16_pairs_one_pass_identical(m, n, k, lda, ldb):
     assert err < 1e-6, err
 
 
-@pytest.mark.parametrize("split", [12, 13])
+@pytest.mark.parametrize("split", [12, 13, 16])
 @pytest.mark.parametrize("m,n,k", [(700, 650, 256), (331, 270, 37), (1000, 180, 512)])
 def test_gemm_plane_tiles_identical(split, m, n, k):
     """The plane kernels with direct LDS copies (tiles 111, 21, 12, 22: 64 x 64, 128 x 64,
     64 x 128, 128 x 128) form the same term products in the same order per element as the
-    64 x 64 kernel staged through registers (tile 11): bit-identical results."""
+    64 x 64 kernel staged through registers (tile 11), for bfloat16 terms and for scaled fp16
+    pairs: bit-identical results."""
     banner("gemm plane tiles 11 vs 22 split={} m={} n={} k={}".format(split, m, n, k))
     rng = np.random.default_rng(m + n + k)
     B_ = 3
@@ -315,6 +316,37 @@ def test_rsolve_outer_block_orders(m, n, nbo, rec, prec):
         _lib().call("mnp_set_tuning", 10, old[1].value)
     X = Bm @ np.linalg.inv(A)
     assert np.max(np.abs(_np(tX, 0) - X)) / np.max(np.abs(X)) < (1e-11 if prec == 0 else 5e-4)
+
+
+@pytest.mark.parametrize("n", [700, 2100])
+def test_getrf_single_register_panel_pivots(n):
+    """Complex-float LU through the register-resident panel (every panel below 3072 rows) on
+    matrices whose pivot in each column is unambiguous (a dominant entry at a random row):
+    LAPACK's pivots exactly, so every interchange the panel and k_laswp make is checked, and
+    P A = L U to complex64 rounding."""
+    banner("getrf complex64 register panel pivots n={}".format(n))
+    rng = np.random.default_rng(n)
+    B = 2
+    A = []
+    for _ in range(B):
+        M_ = _rand(rng, n, n)
+        M_[rng.permutation(n), np.arange(n)] += 40.0 * np.sqrt(n)
+        A.append(M_)
+    t = _cm(A, torch.complex64)
+    ipiv = torch.zeros((B, n), dtype=torch.int32, device=_dev())
+    info = torch.zeros(B, dtype=torch.int32, device=_dev())
+    _lib().call("mnp_getrf_batched", 1, n, B, t.data_ptr(), n, n * n, ipiv.data_ptr(), n, info.data_ptr(), 32, _sh())
+    assert np.all(info.cpu().numpy() == 0)
+    for b in range(B):
+        _, piv = sla.lu_factor(A[b])
+        assert np.array_equal(ipiv[b].cpu().numpy(), piv)
+        LU = _np(t, b).astype(complex)
+        L = np.tril(LU, -1) + np.eye(n)
+        U = np.triu(LU)
+        PA = A[b].copy()
+        for i, p in enumerate(piv):
+            PA[[i, p]] = PA[[p, i]]
+        assert np.linalg.norm(PA - L @ U) / np.linalg.norm(A[b]) < 1e-6
 
 
 def test_getrf_single_backward_error():
