@@ -40,7 +40,10 @@ def _gemm_kernel(tuning, split):
     tile = int(tuning.get("plane_tile", 11))
     mt, nt = {11: (1, 1), 111: (1, 1), 21: (2, 1), 12: (1, 2), 22: (2, 2)}[tile]
     if split == 16:
-        return "k_gemm_sub_c64p<2, true> (complex GEMM on the FP16 matrix cores, scaled fp16 pairs)"
+        if tile == 11:
+            return "k_gemm_sub_c64p<2, true> (complex GEMM on the FP16 matrix cores, scaled fp16 pairs)"
+        return "k_gemm_sub_c64q<2, {}, {}, true> (complex GEMM on the FP16 matrix cores, scaled fp16 pairs)".format(
+            mt, nt)
     if tile == 11:
         return "k_gemm_sub_c64p<{}> (complex GEMM on the BF16 matrix cores, {} operand terms)".format(split, split)
     return "k_gemm_sub_c64q<{}, {}, {}> (complex GEMM on the BF16 matrix cores, {} operand terms)".format(
@@ -52,10 +55,11 @@ def _progress(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-# HBM bytes per k_gemm_sub_c64p<3,1,1> launch of this sweep (batch 201, 478 launches) from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the counter's reading for a 256 MiB
-# copy in the same run (FETCH x2.0, WRITE x1.0); the commands and raw averages are in the file named.
-GEMM_TRAFFIC_BYTES = int((2639095.59 * 2.0 + 3445227.46 * 1.0) * 1024)
+# HBM bytes per k_gemm_sub_c64q<2, 1, 1, true> launch of this sweep (batch 201, 406 launches,
+# scaled fp16 pairs, outer blocks of 256 by recursive halving) from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, each scaled by the counter's reading for a 256 MiB copy in the same run (FETCH x2.0,
+# WRITE x1.0); the commands and raw averages are in the file named.
+GEMM_TRAFFIC_BYTES = int((907156.5 * 2.0 + 1078164.0 * 1.0) * 1024)
 GEMM_TRAFFIC_SOURCE = "profiles/gemm_pmc_rod2496.md"
 
 
@@ -143,13 +147,13 @@ def config1(skip_cpu=False):
     return out
 
 
-def gmres25k(preconditioner="auto"):
+def gmres25k(preconditioner="particle"):
     """Rod dimer (two 20x80 nm capped rods end to end, 2 nm gap, ~24.5k faces), gold Drude in
     vacuum, 750 nm, axial polarisation: BEMRetIter with H-matrices (eta 2.5, leaf 64, ACA tol
     1e-6, kmax 100, low-rank factors complex64) and GMRES(50) to 1e-6. time_to_tol = H-matrix
-    build + preconditioner + GMRES; the wavelength-independent set-up (meshes, near-field pairs,
-    closed-surface terms, cluster tree; for 'particle' also those of each rod alone) is
-    reported separately.
+    build (with the cluster tree) + preconditioner + GMRES; the wavelength-independent set-up
+    (meshes, near-field pairs, closed-surface terms; for 'particle' also those of each rod alone)
+    is reported separately.
     preconditioner: 'auto' = the BEMRet chain of the whole dimer (the reference's choice),
     'particle' = the chains of the two rods alone as a block-diagonal preconditioner."""
     import torch
@@ -420,6 +424,9 @@ def main():
                     help="copy one 256 MiB buffer first (a kernel of known traffic for counter scaling)")
     ap.add_argument("--refine-tol", type=float, default=1e-12,
                     help="relative residual at which the FP64 refinement of the mixed sweep stops")
+    ap.add_argument("--lu-tuning", type=str, default=None,
+                    help="JSON dict of LU knobs (nb, nbo, lookahead, lu_recursive, plane_tile) for the mixed "
+                         "sweep instead of the autotuner's choice (profiler runs, whose counters distort timing)")
     ap.add_argument("--nb", type=int, default=32)
     ap.add_argument("--cpu-sample", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
@@ -456,7 +463,8 @@ def main():
         return bem.solve_spectrum(exc, lams, precision=prec)
 
     solvers = {"mixed": M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="mixed", green=g,
-                                 gemm_split=args.gemm_split, refine_tol=args.refine_tol)}
+                                 gemm_split=args.gemm_split, refine_tol=args.refine_tol,
+                                 lu_tuning=json.loads(args.lu_tuning) if args.lu_tuning else None)}
     if not args.skip_fp64:
         solvers["double"] = M.BEMRet(cp, batch=args.batch, nb=args.nb, precision="double", green=g)
 
@@ -601,10 +609,12 @@ def main():
         line["batched64"] = batched64()
         _progress("batched64 done")
     if not args.no_gmres25k:
-        line["gmres25k"] = gmres25k()
+        # time-to-1e-6 with the chain of each rod alone as preconditioner (the faster form), and
+        # with the chain of the whole dimer (the reference's preconditioner for hmatrix=True)
+        line["gmres25k"] = gmres25k("particle")
         _progress("gmres25k done")
-        line["gmres25k_particle_precond"] = gmres25k("particle")
-        _progress("gmres25k with the per-particle preconditioner done")
+        line["gmres25k_dimer_precond"] = gmres25k("auto")
+        _progress("gmres25k with the whole-dimer preconditioner done")
     if not args.no_warmstart:
         line["warmstart25k"] = warmstart25k()
         _progress("warmstart25k done")

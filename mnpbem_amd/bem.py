@@ -10,7 +10,8 @@ Options: ``precision`` = 'double' (default, complex128 LU chain as the reference
 'mixed' (complex64 LU chain + FP64 iterative refinement on the exact operator) or
 'single'; ``batch`` = wavelengths factored together; ``nb`` = LU panel width;
 ``refine_tol``, ``max_refine`` for the refinement; ``autotune`` = pick the LU shape (panel
-width, outer block width, lookahead) by timing (default: on when batch >= 8).
+width, outer block width, lookahead) by timing (default: on when batch >= 8); ``lu_tuning`` =
+a fixed LU shape instead (dict with keys of ``autotune.knobs``).
 """
 import numpy as np
 import torch
@@ -71,6 +72,7 @@ class BEMRet:
         self.max_refine = int(op.get("max_refine", 12))
         # LU shape autotuning (autotune.py): default on for batched sweeps (batch >= 8)
         self.autotune = op.get("autotune", None)
+        self.lu_tuning = op.get("lu_tuning", None)
         # complex-float GEMMs of the mixed chain: 16 (default) operands as scaled fp16 pairs on the
         # FP16 matrix cores (3 products per real product; measured as accurate as the FP32 matrix
         # cores), 3 bfloat16 terms (6 products, as accurate as FP32), 2 bfloat16 terms (about 16
@@ -98,7 +100,14 @@ class BEMRet:
             nb = self.nb
             split = self.gemm_split if precision != "double" else 0
             tuning = dict(gemm_split=split)
-            if (self.autotune if self.autotune is not None else batch >= 8):
+            if self.lu_tuning is not None:
+                # an explicit LU shape (keys of autotune.knobs: nb, nbo, lookahead, lu_recursive,
+                # plane_tile) instead of the autotuner's timing
+                tuning = dict(self.lu_tuning, gemm_split=split)
+                if split != 16 and split not in (2, 3):
+                    tuning.pop("plane_tile", None)
+                nb = int(tuning.get("nb", nb))
+            elif (self.autotune if self.autotune is not None else batch >= 8):
                 from . import autotune
                 fdt = torch.complex128 if precision == "double" else torch.complex64
                 tuning = autotune.tune_lu(self.g.geo.n, batch, fdt, self.g.geo.dev, gemm_split=split)
