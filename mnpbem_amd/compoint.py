@@ -102,6 +102,9 @@ class PointGreen:
     """Device data of the Green function between a ComPoint and a ComParticle."""
 
     def __init__(self, pt, p, dev, rel_cutoff=3.0, abs_cutoff=0.0, order=ORDER):
+        io = np.asarray(p.inout)
+        if np.any(io[:, 1][:, None] == io[:, 0][None, :]):
+            raise NotImplementedError("fields at points of nested particles are not provided")
         self.pt, self.p, self.dev, self.order = pt, p, dev, order
         rows, cols = point_refine_pairs(pt, p, rel_cutoff, abs_cutoff)
         g, f = point_coefficients(pt, p, rows, cols, order)

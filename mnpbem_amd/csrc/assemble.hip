@@ -7,7 +7,9 @@
 // diagonal of same-particle blocks. The wavenumber of block (particle a, particle b) is that
 // of the material connecting the two regions, con_in[a][b] for the inside matrices
 // (G1, H1) and con_out[a][b] for the outside ones (G2, H2) (greenfun/compgreen_ret.py:435-505,
-// :739-858); 0 = not connected (entry 0).
+// :739-858); 0 = not connected (entry 0). A negative table entry -m is the cross-region term of
+// nested particles, subtracted (G1 = G(in, in) - G(out, in), bem_ret.py:407-433): the entry is
+// formed with material m and negated.
 //
 // Refined pairs (self terms and near pairs) are overwritten afterwards with the Taylor series
 //   G = sum_n g_n (ik)^n e^{ikd},  F = n_i . sum_n f_n (ik)^n e^{ikd}
@@ -43,8 +45,8 @@ __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, int lb, cx<OT>*
     double nr = a.nvec[3 * i] * rx + a.nvec[3 * i + 1] * ry + a.nvec[3 * i + 2] * rz;
     double aj = a.area[j];
     int pa = a.fpart[i], pb = a.fpart[j];
-    int m1 = a.con_in[pa * a.npart + pb];
-    int m2 = a.con_out[pa * a.npart + pb];
+    int c1 = a.con_in[pa * a.npart + pb];
+    int c2 = a.con_out[pa * a.npart + pb];
     bool diag = (i == j);
     double inv_d = diag ? 0.0 : 1.0 / d;
     long off = (long)i + (long)j * a.ld;
@@ -52,15 +54,17 @@ __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, int lb, cx<OT>*
     for (int l = l0; l < l1; ++l) {
         long o = off + (long)l * a.stride;
         for (int side = 0; side < 2; ++side) {
-            int m = side ? m2 : m1;
+            const int c = side ? c2 : c1;
+            const int m = c < 0 ? -c : c;
+            const double sgn = c < 0 ? -1.0 : 1.0;
             cx<OT> g = mk<OT>(0, 0), f = mk<OT>(0, 0);
             if (m > 0 && !diag) {
                 cx<double> k = kk[l * a.nmat + m - 1];
                 cx<double> ph = expikd(k, d);
-                cx<double> gg = (aj * inv_d) * ph;
+                cx<double> gg = (sgn * aj * inv_d) * ph;
                 // (i k - 1/d) / d^2 * nr * a_j
                 cx<double> fac = mk(-k.im - inv_d, k.re);
-                cx<double> ff = (nr * aj * inv_d * inv_d) * (fac * ph);
+                cx<double> ff = (sgn * nr * aj * inv_d * inv_d) * (fac * ph);
                 g = conv<OT>(gg);
                 f = conv<OT>(ff);
             }
@@ -92,16 +96,18 @@ __global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int lb, int
                 nz * rf[((long)p * 3 + 2) * no + q];
     }
     int pa = a.fpart[i], pb = a.fpart[j];
-    int m1 = a.con_in[pa * a.npart + pb];
-    int m2 = a.con_out[pa * a.npart + pb];
+    int c1 = a.con_in[pa * a.npart + pb];
+    int c2 = a.con_out[pa * a.npart + pb];
     bool diag = (i == j);
     long off = (long)i + (long)j * a.ld;
     const cx<double>* kk = reinterpret_cast<const cx<double>*>(a.kmat);
     for (int l = l0; l < l1; ++l) {
         long o = off + (long)l * a.stride;
         for (int side = 0; side < 2; ++side) {
-            int m = side ? m2 : m1;
-            if (m <= 0) continue;
+            const int c = side ? c2 : c1;
+            const int m = c < 0 ? -c : c;
+            const double sgn = c < 0 ? -1.0 : 1.0;
+            if (m == 0) continue;
             cx<double> k = kk[l * a.nmat + m - 1];
             cx<double> ik = mk(-k.im, k.re);
             // Horner in (ik)
@@ -111,7 +117,7 @@ __global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int lb, int
                 sf = ik * sf + mk(fn[q], 0.0);
             }
             cx<double> ph = expikd(k, d);
-            cx<double> g = sg * ph, f = sf * ph;
+            cx<double> g = sgn * (sg * ph), f = sgn * (sf * ph);
             if (diag) f.re += side ? -TWO_PI : TWO_PI;
             if (side == 0) { G1[o] = conv<OT>(g); H1[o] = conv<OT>(f); }
             else { G2[o] = conv<OT>(g); H2[o] = conv<OT>(f); }

@@ -25,8 +25,10 @@ constexpr double HM_TWO_PI = 6.283185307179586476925286766559;
 
 __device__ cx<double> hm_entry(const GreenGeom& a, int kind, int i, int j) {
     int pa = a.fpart[i], pb = a.fpart[j];
-    int m = (kind & 1) ? a.con_out[pa * a.npart + pb] : a.con_in[pa * a.npart + pb];
-    if (m <= 0) return mk(0.0, 0.0);
+    // signed material: -m is the subtracted cross-region term of nested particles
+    const int c = (kind & 1) ? a.con_out[pa * a.npart + pb] : a.con_in[pa * a.npart + pb];
+    const int m = c < 0 ? -c : c;
+    if (m == 0) return mk(0.0, 0.0);
     cx<double> k = reinterpret_cast<const cx<double>*>(a.kmat)[m - 1];
     double rx = a.pos[3 * i] - a.pos[3 * j];
     double ry = a.pos[3 * i + 1] - a.pos[3 * j + 1];
@@ -70,6 +72,7 @@ __device__ cx<double> hm_entry(const GreenGeom& a, int kind, int i, int j) {
             v = (nr * aj * inv_d * inv_d) * (mk(-k.im - inv_d, k.re) * ph);
         }
     }
+    if (c < 0) v = -1.0 * v;
     if (!isG && i == j) v.re += (kind == 2) ? HM_TWO_PI : -HM_TWO_PI;
     return v;
 }

@@ -76,12 +76,18 @@ class DeviceGeometry:
         self.area = torch.tensor(np.ascontiguousarray(cp.area), dtype=f64, device=d)
         self.fpart = torch.tensor(cp.face_particle.astype(np.int32), device=d)
         con = connectivity(cp.inout)
-        if np.any(con[1][0]) or np.any(con[0][1]):
-            raise NotImplementedError("particles whose inside and outside media connect (nested or touching "
-                                      "regions) are not supported by the device BEM path")
+        # nested particles (a core inside a shell: one particle's outside medium is another's
+        # inside) add the cross-region blocks G1 = G(in, in) - G(out, in), G2 = G(out, out) -
+        # G(in, out) (bem_ret.py:407-433). For a pair of particles at most one of the two terms of
+        # each side exists (both would need a particle whose inside and outside media agree), so
+        # the tables hold one signed material per pair: +m for G(in, in) / G(out, out), -m for the
+        # subtracted cross term, 0 for none.
+        self.nested = bool(np.any(con[1][0]) or np.any(con[0][1]))
+        if np.any((con[0][0] != 0) & (con[1][0] != 0)) or np.any((con[1][1] != 0) & (con[0][1] != 0)):
+            raise ValueError("a particle's inside and outside media must differ")
         self.npart = cp.inout.shape[0]
-        self.con_in = torch.tensor(con[0][0].ravel(), device=d)
-        self.con_out = torch.tensor(con[1][1].ravel(), device=d)
+        self.con_in = torch.tensor((con[0][0] - con[1][0]).ravel().astype(np.int32), device=d)
+        self.con_out = torch.tensor((con[1][1] - con[0][1]).ravel().astype(np.int32), device=d)
         self.nmat = len(cp.eps)
         nf = nearfield
         if nf is not None:   # None: far-field use only (no matrix assembly)

@@ -108,9 +108,13 @@ class CompGreenRet:
             raise ValueError("Unknown key: {}".format(key))
         if not np.any(self.con[i][j]):
             return False
-        if i != j:
-            raise NotImplementedError("cross-region Green functions are not provided")
+        if i != j or self._nested():
+            raise NotImplementedError("cross-region Green functions are not provided (BEMRet and BEMRetIter solve "
+                                      "nested particles; their Green function blocks are not returned)")
         return True
+
+    def _nested(self):
+        return bool(np.any(self.con[1][0]) or np.any(self.con[0][1]))
 
     def eval(self, i, j, key, enei, ind=None):
         """Region-pair matrix, or 0 when the regions do not connect (compgreen_ret.py:507-553).
@@ -191,6 +195,8 @@ class CompGreenRet:
         from ._lib import call
         from .engine import stream_handle
         from .hmatrix import C_addr
+        if self._nested():
+            raise NotImplementedError("surface potentials and fields of nested particles are not provided")
         if inout not in (1, 2):
             raise ValueError("inout must be 1 (inside) or 2 (outside)")
         geo, enei = self.geo, float(sig.enei)

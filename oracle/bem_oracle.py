@@ -24,6 +24,9 @@ the reference algorithm of MNPBEM's retarded direct solver:
     surface fields        greenfun/compgreen_ret.py:911-1040 (Gp, H1p, H2p: greenret_refined.py:812-866)
     fields at points      geometry/compoint.py:262-297 (media), greenfun/compgreen_ret.py:911-1034,
                           refine_utils.py:89-159 and greenret_refined.py:287-420 (point-face pairs)
+    nested particles      region pairs of compgreen_ret.py:435-505 in bem_ret.py:407-433, the
+                          general L1 = G1 eps1 G1^-1 of bem_ret.py:450-465 and the source-side eps
+                          of the operator (bem_ret_iter.py:1362-1501)
 
 Parity pins (see tests/test_oracle.py): the reference's MATLAB validation
 tables for trisphere(144, 20) in gold (ext/sca/abs over 41 wavelengths), the
@@ -32,7 +35,8 @@ closed-surface identity, brute-force integration of the refined
 coefficients on fine rules, LU backward error, and ext = sca + abs >= 0; surface
 fields by n.Hp = H and the boundary conditions of a BEM solution; fields at points
 by the far-field limit of the pinned far-field amplitude and by continuity with the
-surface fields.
+surface fields; nested particles by the reference's own solution of a core-shell sphere
+(tests/golden/reference_coreshell.npz) and by the coated-sphere Mie series.
 
 Inputs are plain arrays: corner ``verts``/``faces`` (NaN in column 3 for
 triangles) and curved ``verts2``/``faces2`` (9 columns), e.g. from
@@ -474,9 +478,13 @@ def matrices(green, ks):
 
 
 class BEMRetOracle:
-    """Dense LU chain of bem_ret.py:438-523 (CPU path: explicit inverses)."""
+    """Dense LU chain of bem_ret.py:438-523 (CPU path: explicit inverses). ``cross``: the
+    inside of one particle and the outside of another share a medium (nested particles,
+    con{1,2} != 0); with non-uniform eps the chain then uses L1 = G1 eps1 G1^-1 and
+    L2 = G2 eps2 G2^-1 (Eq. 22 of Garcia de Abajo and Howie, bem_ret.py:450-465) instead of
+    the diagonal eps."""
 
-    def __init__(self, G1, G2, H1, H2, nvec, eps1, eps2, k):
+    def __init__(self, G1, G2, H1, H2, nvec, eps1, eps2, k, cross=False):
         self.k, self.nvec = k, nvec
         n = len(nvec)
         self.G1_lu = sla.lu_factor(G1, check_finite=False)
@@ -485,8 +493,13 @@ class BEMRetOracle:
         G2i = sla.lu_solve(self.G2_lu, np.eye(n), check_finite=False)
         e1, e2 = np.asarray(eps1), np.asarray(eps2)
         uniform = np.allclose(e1, e1[0]) and np.allclose(e2, e2[0])
-        self.L1 = e1[0] if uniform else np.diag(e1)
-        self.L2 = e2[0] if uniform else np.diag(e2)
+        if uniform:
+            self.L1, self.L2 = e1[0], e2[0]
+        elif cross:
+            self.L1 = G1 @ (e1[:, None] * G1i)
+            self.L2 = G2 @ (e2[:, None] * G2i)
+        else:
+            self.L1, self.L2 = np.diag(e1), np.diag(e2)
         self.Sigma1 = H1 @ G1i
         Sigma2 = H2 @ G2i
         self.Delta_lu = sla.lu_factor(self.Sigma1 - Sigma2, check_finite=False)
@@ -606,16 +619,22 @@ def stat_cross_sections(pos, area, sig, pol, k):
 # ------------------------------------------------------------------ 8N operator
 def afun(G1, G2, H1, H2, nvec, eps1, eps2, k, x):
     """Exact operator on x = [sig1, h1, sig2, h2] packed column-major
-    (bem_ret_iter.py:1196-1232, :1362-1501): returns [phi, a, De, alpha]."""
+    (bem_ret_iter.py:1196-1232, :1362-1501): returns [phi, a, De, alpha]. The dielectric
+    function multiplies the sources before the Green function and its derivative,
+    alpha = H1 h1 - H2 h2 - i k n (G1 (eps1 sig1) - G2 (eps2 sig2)),
+    De = H1 (eps1 sig1) - H2 (eps2 sig2) - i k n . (G1 (eps1 h1) - G2 (eps2 h2)), which is the
+    chain's L = G eps G^-1 applied to G sig, and equals eps (G sig) when eps is uniform."""
     n = len(nvec)
     X = x.reshape(8, n).T
     v1, v2 = X[:, 0:4], X[:, 4:8]
+    e1 = np.broadcast_to(np.asarray(eps1), (n,))[:, None]
+    e2 = np.broadcast_to(np.asarray(eps2), (n,))[:, None]
     G1v, G2v, H1v, H2v = G1 @ v1, G2 @ v2, H1 @ v1, H2 @ v2
-    e1, e2 = np.asarray(eps1), np.asarray(eps2)
+    G1e, G2e, H1e, H2e = G1 @ (e1 * v1), G2 @ (e2 * v2), H1 @ (e1 * v1), H2 @ (e2 * v2)
     phi = G1v[:, 0] - G2v[:, 0]
     a = G1v[:, 1:4] - G2v[:, 1:4]
-    alpha = H1v[:, 1:4] - H2v[:, 1:4] - 1j * k * nvec * (e1 * G1v[:, 0] - e2 * G2v[:, 0])[:, None]
-    De = e1 * H1v[:, 0] - e2 * H2v[:, 0] - 1j * k * np.sum(nvec * (e1[:, None] * G1v[:, 1:4] - e2[:, None] * G2v[:, 1:4]), axis=1)
+    alpha = H1v[:, 1:4] - H2v[:, 1:4] - 1j * k * nvec * (G1e[:, 0] - G2e[:, 0])[:, None]
+    De = H1e[:, 0] - H2e[:, 0] - 1j * k * np.sum(nvec * (G1e[:, 1:4] - G2e[:, 1:4]), axis=1)
     return np.concatenate([phi, a.T.ravel(), De, alpha.T.ravel()])
 
 
@@ -644,7 +663,8 @@ def solve_spectrum(surface, face_particle, inout, eps_funcs, enei_list, pol, dir
         e1 = np.array([eps_k[m - 1][0] for m in inout[:, 0]])[fp]
         e2 = np.array([eps_k[m - 1][0] for m in inout[:, 1]])[fp]
         k0 = TWO_PI / enei
-        bem = BEMRetOracle(G1, G2, H1, H2, surface.nvec, e1, e2, k0)
+        cross = bool(np.any(inout[:, 0][:, None] == inout[:, 1][None, :]))
+        bem = BEMRetOracle(G1, G2, H1, H2, surface.nvec, e1, e2, k0, cross=cross)
         epsm, km = eps_k[medium - 1]
         a2, a2p = planewave(surface.pos, surface.nvec, pol, direc, enei, epsm, exc2)
         a1, a1p = planewave(surface.pos, surface.nvec, pol, direc, enei, epsm, exc1)
@@ -690,6 +710,53 @@ def mie_sphere(enei, diameter, eps_in, eps_out, lmax=20):
     xx, dxx = xi(x)
     an = (m * pmx * dpx - px * dpmx) / (m * pmx * dxx - xx * dpmx)
     bn = (pmx * dpx - m * px * dpmx) / (pmx * dxx - m * xx * dpmx)
+    ext = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * np.real(an + bn))
+    sca = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * (np.abs(an) ** 2 + np.abs(bn) ** 2))
+    return float(ext), float(sca)
+
+
+def mie_coated(enei, d_core, d_shell, eps_core, eps_shell, eps_out, lmax=20):
+    """Extinction and scattering cross sections (nm^2) of a coated sphere (core diameter
+    d_core, shell diameter d_shell) from the Aden-Kerker series as written by Bohren and Huffman
+    (ch. 8.1): x = k a, y = k b, m1, m2 = core and shell index over the medium's,
+    A_n = (m2 psi_n(m2 x) psi_n'(m1 x) - m1 psi_n'(m2 x) psi_n(m1 x)) /
+          (m2 chi_n(m2 x) psi_n'(m1 x) - m1 chi_n'(m2 x) psi_n(m1 x)),
+    B_n = (m2 psi_n(m1 x) psi_n'(m2 x) - m1 psi_n(m2 x) psi_n'(m1 x)) /
+          (m2 chi_n'(m2 x) psi_n(m1 x) - m1 psi_n'(m1 x) chi_n(m2 x)),
+    a_n = (psi(y) [psi'(m2 y) - A chi'(m2 y)] - m2 psi'(y) [psi(m2 y) - A chi(m2 y)]) /
+          (xi(y) [psi'(m2 y) - A chi'(m2 y)] - m2 xi'(y) [psi(m2 y) - A chi(m2 y)]),
+    b_n likewise with B and m2 on the other terms; chi_n(z) = -z y_n(z). Reduces to mie_sphere
+    when the core and shell media agree or the core vanishes. A check of the nested-particle
+    path (no counterpart in the reference)."""
+    from scipy.special import spherical_jn, spherical_yn
+    nb = np.sqrt(complex(eps_out))
+    k = 2 * np.pi / enei * nb.real
+    m1, m2 = np.sqrt(complex(eps_core)) / nb, np.sqrt(complex(eps_shell)) / nb
+    x, y = k * 0.5 * d_core, k * 0.5 * d_shell
+    n = np.arange(1, lmax + 1)
+
+    def psi(z):
+        return z * spherical_jn(n, z), spherical_jn(n, z) + z * spherical_jn(n, z, derivative=True)
+
+    def chi(z):
+        return -z * spherical_yn(n, z), -(spherical_yn(n, z) + z * spherical_yn(n, z, derivative=True))
+
+    def xi(z):
+        h = spherical_jn(n, z) + 1j * spherical_yn(n, z)
+        dh = spherical_jn(n, z, derivative=True) + 1j * spherical_yn(n, z, derivative=True)
+        return z * h, h + z * dh
+
+    p1x, dp1x = psi(m1 * x)
+    p2x, dp2x = psi(m2 * x)
+    c2x, dc2x = chi(m2 * x)
+    A = (m2 * p2x * dp1x - m1 * dp2x * p1x) / (m2 * c2x * dp1x - m1 * dc2x * p1x)
+    B = (m2 * p1x * dp2x - m1 * p2x * dp1x) / (m2 * dc2x * p1x - m1 * dp1x * c2x)
+    py, dpy = psi(y)
+    xy, dxy = xi(y)
+    p2y, dp2y = psi(m2 * y)
+    c2y, dc2y = chi(m2 * y)
+    an = (py * (dp2y - A * dc2y) - m2 * dpy * (p2y - A * c2y)) / (xy * (dp2y - A * dc2y) - m2 * dxy * (p2y - A * c2y))
+    bn = (m2 * py * (dp2y - B * dc2y) - dpy * (p2y - B * c2y)) / (m2 * xy * (dp2y - B * dc2y) - dxy * (p2y - B * c2y))
     ext = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * np.real(an + bn))
     sca = 2 * np.pi / k ** 2 * np.sum((2 * n + 1) * (np.abs(an) ** 2 + np.abs(bn) ** 2))
     return float(ext), float(sca)

@@ -1,0 +1,123 @@
+"""Nested particles: a gold core inside a dielectric shell. The device assembly forms the
+cross-region blocks of G1 = G(in, in) - G(out, in) and G2 = G(out, out) - G(in, out) (H1, H2
+alike; bem_ret.py:407-433, materials joining the regions from compgreen_ret.py:435-505); checked
+against the oracle's region-pair matrices, then the BEMRet solutions (complex128 and mixed
+precision) against the oracle's chain, and BEMRetIter (dense and H-matrix operators) against
+BEMRet."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import banner
+from oracle import bem_oracle as O
+import mnpbem_amd as M
+from mnpbem_amd.bem_iter import BEMRetIter
+from mnpbem_amd.engine import GreenBatch
+
+pytestmark = pytest.mark.gpu
+
+DRUDE = (10.0, 9.065, 0.0708)
+INOUT = [[2, 1], [3, 2]]          # shell surface: shell | vacuum; core surface: gold | shell
+
+
+@pytest.fixture(scope="module")
+def coreshell():
+    shell = M.trisphere(144, 20)
+    core = M.trisphere(144, 12)
+    eps = [M.EpsConst(1.0), M.EpsConst(2.1), M.EpsDrude(*DRUDE)]
+    cp = M.ComParticle(eps, [shell, core], INOUT, 1, 2, interp="curv")
+    p = cp.pc
+    s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
+    fp = cp.face_particle
+    closed = [(np.nonzero(fp == a)[0], np.nonzero(fp == a)[0], 1.0) for a in range(2)]
+    return cp, s, fp, closed
+
+
+@pytest.fixture(scope="module")
+def coreshell_oracle(coreshell):
+    cp, s, fp, closed = coreshell
+    return O.Green(s, fp, INOUT, closed)
+
+
+def _eps_funcs(cp):
+    return [lambda lam, e=e: e(lam) for e in cp.eps]
+
+
+def test_nested_assembly_matches_oracle(coreshell, coreshell_oracle):
+    cp, s, fp, closed = coreshell
+    bem = M.BEMRet(cp)
+    geo = bem.g.geo
+    assert geo.nested
+    lams = np.array([520.0, 700.0])
+    banner("nested assembly n={}".format(s.n))
+    gb = GreenBatch(geo, 2, torch.complex128)
+    gb.assemble(geo.media(lams)[0], 2)
+    core = fp == 1
+    shell = fp == 0
+    for b, lam in enumerate(lams):
+        ks = [e(lam)[1] for e in cp.eps]
+        for name, t, R in zip(("G1", "G2", "H1", "H2"), (gb.G1, gb.G2, gb.H1, gb.H2), O.matrices(coreshell_oracle, ks)):
+            got = GreenBatch.to_numpy(t, b)
+            assert np.max(np.abs(got - R)) / np.max(np.abs(R)) < 1e-9, name
+        # the cross blocks are there: G1 (core rows, shell columns) is minus the shell-medium
+        # Green function, G2 (shell rows, core columns) likewise; the other off-diagonal blocks
+        # vanish (vacuum does not reach the core, gold not the shell)
+        G1, G2 = GreenBatch.to_numpy(gb.G1, b), GreenBatch.to_numpy(gb.G2, b)
+        assert np.max(np.abs(G1[np.ix_(core, shell)])) > 0 and np.max(np.abs(G2[np.ix_(shell, core)])) > 0
+        assert np.max(np.abs(G1[np.ix_(shell, core)])) == 0 and np.max(np.abs(G2[np.ix_(core, shell)])) == 0
+
+
+@pytest.mark.parametrize("precision,tol", [("double", 1e-10), ("mixed", 1e-8)])
+def test_nested_bemret_matches_oracle(coreshell, precision, tol):
+    cp, s, fp, closed = coreshell
+    lams = np.array([480.0, 530.0, 620.0])
+    banner("nested bemret {} spectrum".format(precision))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    out = M.BEMRet(cp, batch=3, precision=precision, autotune=False).solve_spectrum(exc, lams, scattering=True)
+    ref = O.solve_spectrum(s, fp, INOUT, _eps_funcs(cp), lams, np.array([1.0, 0, 0]), np.array([0, 0, 1.0]),
+                           pinf_nvec=exc.spec.nvec, pinf_area=exc.spec.area)
+    assert np.max(np.abs(out["ext"][:, 0] - ref["ext"]) / np.abs(ref["ext"])) < tol
+    assert np.max(np.abs(out["sca"][:, 0] - ref["sca"]) / np.abs(ref["sca"])) < tol
+
+
+def test_nested_bemret_solution_and_per_wavelength_api(coreshell, coreshell_oracle):
+    """sig1, h1, sig2, h2 of BEMRet.solve (the per-wavelength API) against the oracle's chain."""
+    cp, s, fp, closed = coreshell
+    lam = 530.0
+    banner("nested bemret solution lam={}".format(lam))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    sig, _ = M.BEMRet(cp).solve(exc.potential(cp, lam))
+    ks = [e(lam)[1] for e in cp.eps]
+    G1, G2, H1, H2 = O.matrices(coreshell_oracle, ks)
+    e1, e2 = cp.eps1(lam), cp.eps2(lam)
+    ob = O.BEMRetOracle(G1, G2, H1, H2, s.nvec, e1, e2, 2 * np.pi / lam, cross=True)
+    eps_out, k_out = cp.eps[0](lam)
+    io = np.asarray(INOUT)[fp]
+    a2, a2p = O.planewave(s.pos, s.nvec, np.array([1.0, 0, 0]), np.array([0, 0, 1.0]), lam, eps_out,
+                          np.nonzero(io[:, 1] == 1)[0])
+    a1, a1p = O.planewave(s.pos, s.nvec, np.array([1.0, 0, 0]), np.array([0, 0, 1.0]), lam, eps_out,
+                          np.nonzero(io[:, 0] == 1)[0])
+    phi, a, alpha, De = O.excitation(s.nvec, 2 * np.pi / lam, e1, e2, a1, a1p, a2, a2p)
+    s1, h1, s2, h2 = ob.solve(phi, a, alpha, De)
+    for got, want in ((sig.sig1, s1), (sig.h1, h1), (sig.sig2, s2), (sig.h2, h2)):
+        got = np.asarray(got).reshape(want.shape)
+        assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
+
+
+@pytest.mark.parametrize("hmatrix", [False, True])
+def test_nested_bemretiter_equals_bemret(coreshell, hmatrix):
+    cp, s, fp, closed = coreshell
+    lam = 530.0
+    banner("nested bemretiter hmatrix={}".format(hmatrix))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    pot = exc.potential(cp, lam)
+    ref, _ = M.BEMRet(cp).solve(pot)
+    op = dict(tol=1e-10, maxit=400, restart=60)
+    if hmatrix:
+        op.update(hmatrix=True, htol=1e-10, kmax=200, cleaf=32)
+    it = BEMRetIter(cp, **op)
+    sig, _ = it.solve(pot)
+    flag, relres, niter = it.info()
+    assert flag == 0
+    e_ref, e_it = exc.extinction(ref), exc.extinction(sig)
+    assert abs(e_it - e_ref) / abs(e_ref) < (1e-6 if hmatrix else 1e-8)
