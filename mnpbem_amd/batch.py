@@ -52,6 +52,8 @@ class GroupGeometry:
     particle p at wavelength l (pos, nvec, area, side flags per medium; zeros on padding)."""
 
     def __init__(self, geos, nlam, n, dev):
+        if any(getattr(g, "nested", False) for g in geos):
+            raise NotImplementedError("particle groups hold separate particles; solve nested particles with BEMRet")
         self.geos, self.nlam, self.n, self.dev = geos, nlam, n, dev
         self.sface = n
         P = len(geos)

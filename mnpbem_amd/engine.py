@@ -223,15 +223,31 @@ class RetSolverBatch:
         with knobs(self.asm_cfg.get(key)):
             gb.assemble(self.kmat, nl)
 
+    def _needs_general(self):
+        """Whether the chain needs L1 = G1 eps1 G1^-1, L2 = G2 eps2 G2^-1 as full matrices
+        (bem_ret.py:450-465): the inside of one particle is the outside of another (nested
+        particles) and eps is not one value over all faces. Otherwise L1, L2 are the per-face
+        eps (diagonal)."""
+        if not getattr(self.geo, "nested", False):
+            return False
+        nl = self.nlam
+        e1, e2 = self.L1[:nl], self.L2[:nl]
+        uniform = bool(torch.all(e1 == e1[:, :1]).item()) and bool(torch.all(e2 == e2[:, :1]).item())
+        return not uniform
+
     def _factor_assembled(self):
         """LU chain of the assembled G1, G2, H1, H2 of the first self.nlam matrices."""
         g, f, nl = self.geo, self.fac, self.nlam
         self.explicit = False
+        self.general = self._needs_general()
         from .autotune import knobs
         with knobs(self.tuning):
-            call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
-                 _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info), _p(g.nvec),
-                 g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
+            if self.general:
+                self._factor_general()
+            else:
+                call("mnp_bemret_factor", self.prec, g.n, nl, _p(f.G1), _p(f.G2), _p(f.H1), _p(f.H2), _p(self.S1),
+                     _p(self.S2), _p(self.D), f.ld, f.stride, _p(self.ipiv), _p(self.perm), _p(self.info),
+                     _p(g.nvec), g.sface, _p(self.L1), _p(self.L2), _p(self.kvac), self.nb, stream_handle())
         info = self.check_info()
         if np.any(info < 0):
             raise RuntimeError("LU panel synchronisation timed out (info {}); factors are invalid".format(info))
@@ -241,6 +257,128 @@ class RetSolverBatch:
         if self.precision != "mixed" and np.any(info > 0):
             warn_singular("BEMRet ({})".format(self.precision), info)
         return self
+
+    # -- general chain (nested particles with non-uniform eps) ---------------------------------
+    # Same storage as the diagonal chain: G1, G2 hold their LUs, S1 = Sigma1, D = Delta^-1,
+    # S2 = LU(Sigma); pivots, permutations and info in the layout of mnp_bemret_factor
+    # (factor q of G1, G2, Delta, Sigma at offset q nlam n, permutations (pf, qinv) at 2q, 2q + 1).
+    # L1, L2 are kept as full matrices for the solve. Elementwise steps run on torch device tensors;
+    # the LUs, right-side solves and products run on the library's kernels.
+
+    def _lu(self, q, A):
+        n, nl, pb = self.geo.n, self.nlam, self.nlam * self.geo.n
+        ip = self.ipiv.view(-1)[q * pb:(q + 1) * pb]
+        pm = self.perm.view(-1)
+        call("mnp_getrf_batched", self.prec, n, nl, _p(A), n, n * n, _p(ip), n,
+             self.info.data_ptr() + 4 * q * nl, self.nb, stream_handle())
+        call("mnp_perm_batched", n, nl, _p(ip), n, pm.data_ptr() + 4 * 2 * q * pb,
+             pm.data_ptr() + 4 * (2 * q + 1) * pb, n, stream_handle())
+
+    def _rsolve(self, q, A, B, X):
+        """X = B A^-1 (A: LU number q; B is overwritten)."""
+        n, nl, pb = self.geo.n, self.nlam, self.nlam * self.geo.n
+        call("mnp_rsolve_batched", self.prec, n, n, nl, _p(A), n, n * n,
+             self.perm.data_ptr() + 4 * (2 * q + 1) * pb, n, _p(B), n, n * n, _p(X), n, n * n, self.nb,
+             stream_handle())
+
+    def _gemm_sub(self, C, A, B):
+        """C -= A B for the first nlam matrices."""
+        n = self.geo.n
+        call("mnp_gemm_sub_batched", self.prec, n, n, n, _p(A), n, n * n, _p(B), n, n * n, _p(C), n, n * n,
+             self.nlam, stream_handle())
+
+    def _factor_general(self):
+        """bem_ret.py:438-495 with L1 = G1 eps1 G1^-1, L2 = G2 eps2 G2^-1 (right-side solves of
+        the column-scaled G1, G2) and Sigma = Sigma1 L1 - Sigma2 L2 + k^2 ((L Delta^-1) o n n^T) L."""
+        g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
+        fdt = f.dtype
+        if getattr(self, "_Lm", None) is None:
+            mk = lambda: torch.empty((self.batch, n, n), dtype=fdt, device=g.dev)
+            self._Lm = [mk(), mk(), mk()]
+        L1m, L2m, T = self._Lm
+        G1, G2, H1, H2 = f.G1[:nl], f.G2[:nl], f.H1[:nl], f.H2[:nl]
+        S1, S2, D = self.S1[:nl], self.S2[:nl], self.D[:nl]
+        e1 = self.L1[:nl].to(fdt)
+        e2 = self.L2[:nl].to(fdt)
+        # matrix (i, j) of entry b sits at [b, j, i]: scaling column j is a scaling of row j here
+        torch.mul(G1, e1[:, :, None], out=T[:nl])
+        torch.mul(G2, e2[:, :, None], out=L2m[:nl])
+        self._lu(0, f.G1)
+        self._lu(1, f.G2)
+        self._rsolve(0, f.G1, T, L1m)                # L1 = (G1 eps1) G1^-1
+        self._rsolve(1, f.G2, L2m, T)                # L2 = (G2 eps2) G2^-1, in T
+        L2m, T = T, L2m
+        self._Lm = [L1m, L2m, T]
+        self._rsolve(0, f.G1, f.H1, self.S1)         # Sigma1 = H1 G1^-1
+        self._rsolve(1, f.G2, f.H2, self.S2)         # Sigma2 = H2 G2^-1 (in S2 for now)
+        torch.sub(S1, S2, out=H1)                    # Delta
+        self._lu(2, f.H1)
+        H2.zero_()
+        H2.diagonal(dim1=1, dim2=2).fill_(1)
+        self._rsolve(2, f.H1, f.H2, self.D)          # Delta^-1
+        H1.zero_()
+        self._gemm_sub(f.H1, self.S2, L2m)           # H1 = -Sigma2 L2
+        torch.sub(L2m[:nl], L1m[:nl], out=S2)        # S2 = -L
+        H2.zero_()
+        self._gemm_sub(f.H2, self.S2, self.D)        # H2 = L Delta^-1
+        nv = g.nvec.to(fdt)
+        nn = nv @ nv.transpose(0, 1)                 # symmetric: its layout does not matter
+        k2 = (self.kvac[:nl] ** 2).to(fdt)
+        H2.mul_(nn[None]).mul_(k2[:, None, None])    # k^2 (L Delta^-1) o n n^T
+        S1.neg_()
+        self._gemm_sub(f.H1, self.S1, L1m)           # + Sigma1 L1
+        S1.neg_()
+        self._gemm_sub(f.H1, f.H2, self.S2)          # + k^2 ((L Delta^-1) o n n^T) L
+        S2.copy_(H1)
+        self._lu(3, self.S2)
+
+    def _inv_apply(self, q, M, X, Y, r):
+        """Y = M^-1 X (r columns) from LU number q of M, or M X when M holds the inverse."""
+        n, nl, pb = self.geo.n, self.nlam, self.nlam * self.geo.n
+        if getattr(self, "explicit", False):
+            call("mnp_gemv_batched", self.prec, n, r, nl, _p(M), n, n * n, _p(X), n, r * n, _p(Y), n, r * n,
+                 stream_handle())
+        else:
+            call("mnp_getrs_batched", self.prec, n, r, nl, _p(M), n, n * n, self.perm.data_ptr() + 4 * 2 * q * pb,
+                 n, _p(X), n, r * n, _p(Y), n, r * n, stream_handle())
+
+    def _gemv(self, M, X, r):
+        n, nl = self.geo.n, self.nlam
+        Y = torch.empty((nl, r, n), dtype=X.dtype, device=X.device)
+        call("mnp_gemv_batched", self.prec, n, r, nl, _p(M), n, n * n, _p(X), n, r * n, _p(Y), n, r * n,
+             stream_handle())
+        return Y
+
+    def _chain_general(self, rhs, out):
+        """Eqs. (19)-(20) of Garcia de Abajo and Howie (bem_ret.py:1708-1752) with full L1, L2:
+        rhs = [phi a De alpha], out = [sig1 h1 sig2 h2] (B, 8, n) of the factor precision."""
+        g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
+        L1m, L2m = self._Lm[0], self._Lm[1]
+        x = rhs[:nl]
+        ik = (1j * self.kvac[:nl]).to(x.dtype)[:, None]
+        nv = g.nvec.to(x.dtype).transpose(0, 1)            # (3, n)
+        phi, a, De, alpha = x[:, 0], x[:, 1:4], x[:, 4], x[:, 5:8]
+        La = self._gemv(L1m, x[:, 0:4].contiguous(), 4)     # [L1 phi, L1 a]
+        L1phi, L1a = La[:, 0], La[:, 1:4]
+        W = self._gemv(self.S1, torch.cat([a, L1phi[:, None]], 1).contiguous(), 4)
+        alpha2 = alpha - W[:, 0:3] + ik[:, None] * nv[None] * L1phi[:, None]
+        De2 = De - W[:, 3] + ik * torch.sum(nv[None] * L1a, 1)
+        U = self._gemv(self.D, alpha2.contiguous(), 3)
+        LU_ = self._gemv(L1m, U, 3) - self._gemv(L2m, U, 3)
+        R = (De2 + ik * torch.sum(nv[None] * LU_, 1))[:, None].contiguous()
+        sig2 = torch.empty_like(R)
+        self._inv_apply(3, self.S2, R, sig2, 1)
+        w = self._gemv(L1m, sig2, 1) - self._gemv(L2m, sig2, 1)
+        Tt = (ik[:, None] * nv[None] * w + alpha2).contiguous()
+        h2 = self._gemv(self.D, Tt, 3)
+        P1 = torch.cat([sig2 + phi[:, None], h2 + a], 1).contiguous()
+        P2 = torch.cat([sig2, h2], 1).contiguous()
+        o1 = torch.empty_like(P1)
+        o2 = torch.empty_like(P2)
+        self._inv_apply(0, f.G1, P1, o1, 4)
+        self._inv_apply(1, f.G2, P2, o2, 4)
+        out[:nl, 0:4] = o1
+        out[:nl, 4:8] = o2
 
     def invert(self):
         """Replaces the LU factors of G1, G2 and Sigma by their explicit inverses (right-side
@@ -271,6 +409,9 @@ class RetSolverBatch:
 
     def _chain(self, rhs, out):
         """out = M rhs with the factor precision (rhs, out: (B, 8, n) of that dtype)."""
+        if getattr(self, "general", False):
+            self._chain_general(rhs, out)
+            return
         g, f, nl, n = self.geo, self.fac, self.nlam, self.geo.n
         perm = None if getattr(self, "explicit", False) else _p(self.perm)
         call("mnp_bemret_solve", self.prec, n, nl, _p(f.G1), _p(f.G2), _p(self.S1), _p(self.D), _p(self.S2),
