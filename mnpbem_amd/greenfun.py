@@ -91,15 +91,29 @@ class CompGreenRet:
 
     _KEYS = ("G", "F", "H1", "H2", "Gp", "H1p", "H2p")
 
-    def _struct(self, enei):
-        """Geometry descriptor for entry-wise kernels at one wavelength: (struct, keep-alive)."""
+    def _struct(self, enei, pair=None):
+        """Geometry descriptor for entry-wise kernels at one wavelength: (struct, keep-alive).
+        ``pair`` = (i, j): both connectivity tables are region pair (i, j)'s, so that either
+        side evaluates that pair alone."""
         from ._lib import GreenGeomC
         geo = self.geo
         kmat = geo.media([float(enei)])[0][0].contiguous()
         rowptr, rcol, rg, rf = geo.refined_csr()
+        cin, cout = (geo.con_in, geo.con_out) if pair is None else (self._pair(*pair),) * 2
         g = GreenGeomC(geo.n, geo.npart, geo.order, *[t.data_ptr() for t in (
-            geo.pos, geo.nvec, geo.area, geo.fpart, geo.con_in, geo.con_out, kmat, rowptr, rcol, rg, rf)])
+            geo.pos, geo.nvec, geo.area, geo.fpart, cin, cout, kmat, rowptr, rcol, rg, rf)])
         return g, kmat
+
+    def _pair(self, i, j):
+        """Connectivity table of region pair (i, j) alone (material per particle pair, 0 = not
+        connected) on the device: the tables of the assembly without the subtracted
+        cross-region terms of nested particles (compgreen_ret.py:435-505)."""
+        if not hasattr(self, "_pairs"):
+            self._pairs = {}
+        if (i, j) not in self._pairs:
+            self._pairs[(i, j)] = torch.tensor(np.ascontiguousarray(self.con[i][j]).ravel().astype(np.int32),
+                                               device=self.geo.dev)
+        return self._pairs[(i, j)]
 
     def _check(self, i, j, key):
         if self._points is not None:
@@ -108,9 +122,6 @@ class CompGreenRet:
             raise ValueError("Unknown key: {}".format(key))
         if not np.any(self.con[i][j]):
             return False
-        if i != j or self._nested():
-            raise NotImplementedError("cross-region Green functions are not provided (BEMRet and BEMRetIter solve "
-                                      "nested particles; their Green function blocks are not returned)")
         return True
 
     def _nested(self):
@@ -127,20 +138,22 @@ class CompGreenRet:
             return 0
         if key in ("Gp", "H1p", "H2p"):
             return self.eval_block(i, j, key, enei, 0, self.geo.n)
-        gb = GreenBatch(self.geo, 1, torch.complex128)
-        kmat = self.geo.media([enei])[0]
-        gb.assemble(kmat, 1)
+        # the assembly with region pair (i, j)'s table on both sides: G1 = G2 = G(i, j), and
+        # H1 = F + 2 pi, H2 = F - 2 pi on the diagonal where the pair connects a face to itself
+        geo = self.geo
+        gb = GreenBatch(geo, 1, torch.complex128)
+        kmat = geo.media([enei])[0]
+        tables = geo.con_in, geo.con_out
+        geo.con_in = geo.con_out = self._pair(i, j)
+        try:
+            gb.assemble(kmat, 1)
+        finally:
+            geo.con_in, geo.con_out = tables
         if key == "G":
-            t = gb.G1 if i == 0 else gb.G2
-            return GreenBatch.to_numpy(t, 0)
-        t = gb.H1 if i == 0 else gb.H2
-        m = GreenBatch.to_numpy(t, 0)
+            return GreenBatch.to_numpy(gb.G1, 0)
         if key == "F":
-            # H = F +- 2 pi on the diagonal of same-particle blocks
-            m = m - (2 * np.pi if i == 0 else -2 * np.pi) * np.eye(m.shape[0])
-        elif (key == "H1") != (i == 0):
-            m = m + (4 * np.pi if key == "H1" else -4 * np.pi) * np.eye(m.shape[0])
-        return m
+            return 0.5 * (GreenBatch.to_numpy(gb.H1, 0) + GreenBatch.to_numpy(gb.H2, 0))
+        return GreenBatch.to_numpy(gb.H1 if key == "H1" else gb.H2, 0)
 
     def eval_block(self, i, j, key, enei, col_start, col_stop):
         """Columns [col_start, col_stop) of ``eval(i, j, key, enei)`` without the full matrix
@@ -157,7 +170,7 @@ class CompGreenRet:
         shape = (n, 3, c1 - c0) if vec else (n, c1 - c0)
         if not self._check(i, j, key):
             return np.zeros(shape, complex)
-        g, keep = self._struct(enei)
+        g, keep = self._struct(enei, (i, j))
         out = torch.empty(shape, dtype=torch.complex128, device=self.geo.dev)
         call("mnp_green_block", C_addr(g), i, self._KEYS.index(key), c0, c1 - c0, out.data_ptr(), stream_handle())
         res = out.cpu().numpy()
@@ -177,7 +190,7 @@ class CompGreenRet:
             raise IndexError("linear index out of range for a {} x {} matrix".format(n, n))
         if not self._check(i, j, key) or len(ind) == 0:
             return np.zeros(shape, complex)
-        g, keep = self._struct(enei)
+        g, keep = self._struct(enei, (i, j))
         d_ind = torch.tensor(ind, device=self.geo.dev)
         out = torch.empty(shape, dtype=torch.complex128, device=self.geo.dev)
         call("mnp_green_pick", C_addr(g), i, self._KEYS.index(key), d_ind.data_ptr(), len(ind), out.data_ptr(),

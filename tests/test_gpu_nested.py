@@ -121,3 +121,35 @@ def test_nested_bemretiter_equals_bemret(coreshell, hmatrix):
     assert flag == 0
     e_ref, e_it = exc.extinction(ref), exc.extinction(sig)
     assert abs(e_it - e_ref) / abs(e_ref) < (1e-6 if hmatrix else 1e-8)
+
+
+@pytest.mark.parametrize("key", ["G", "F", "H1", "H2", "Gp", "H1p", "H2p"])
+def test_nested_region_pair_blocks_match_oracle(coreshell, coreshell_oracle, key):
+    """CompGreenRet.eval(i, j, key) of every region pair of the core-shell particle, the full
+    matrix (assembly kernel), column blocks and picked entries (entry-wise kernel), against the
+    oracle's region-pair matrices (compgreen_ret.py:507-737, :860-910). The cross pairs carry the
+    shell medium between the shell's inside and the core's outside; pairs that do not connect
+    are 0."""
+    cp, s, fp, closed = coreshell
+    lam = 560.0
+    banner("nested region-pair blocks key={}".format(key))
+    g = M.CompGreenRet(cp, cp)
+    ks = [e(lam)[1] for e in cp.eps]
+    n = s.n
+    ind = np.random.default_rng(5).integers(0, n * n, 400)
+    vec = key in ("Gp", "H1p", "H2p")
+    for i in range(2):
+        for j in range(2):
+            want = coreshell_oracle.eval(i, j, key, ks)
+            got = g.eval(i, j, key, lam)
+            if np.isscalar(want):
+                assert np.isscalar(got) and got == 0
+                continue
+            scale = np.max(np.abs(want))
+            assert np.max(np.abs(got - want)) / scale < 1e-9, (i, j)
+            blk = g.eval_block(i, j, key, lam, 100, 220)
+            assert np.max(np.abs(blk - want[..., 100:220])) / scale < 1e-9, (i, j)
+            picked = g.eval(i, j, key, lam, ind=ind)
+            r, c = ind // n, ind % n
+            want_p = want[r, :, c] if vec else want[r, c]
+            assert np.max(np.abs(picked - want_p)) / scale < 1e-9, (i, j)
