@@ -62,6 +62,7 @@ class CompGreenRet:
             raise NotImplementedError("only the Cartesian surface derivative (deriv='cart') is provided")
         from .compoint import ComPoint
         self._points = None
+        self._split = None
         if isinstance(p1, ComPoint):
             # observer points against a particle: potentials and fields at points
             from .compoint import PointGreen
@@ -72,13 +73,34 @@ class CompGreenRet:
             self.nearfield, self._geo = None, None
             return
         if p1 is not p2:
-            raise NotImplementedError("CompGreenRet on the device covers p1 is p2 (the BEM solve path) and "
-                                      "p1 = ComPoint with p2 a ComParticle (fields at points)")
-        self.p1 = self.p2 = p1
+            # two different particles: rows are p1's faces (observer centroids and normals), columns
+            # p2's (sources), regions joined by material index (compgreen_ret.py:57-190, :435-505).
+            # The entries are those of the composite of both, whose p1-row / p2-column block is
+            # returned: the same far-field formula, the refinement of face pairs closer than
+            # RelCutoff face radii, and no closed-surface correction (the reference applies it only
+            # for p1 is p2).
+            from .comparticle import ComParticle
+            if not isinstance(p2, ComParticle) or not isinstance(p1, ComParticle):
+                raise NotImplementedError("CompGreenRet(p1, p2) takes a ComParticle or ComPoint p1 and a "
+                                          "ComParticle p2")
+            if len(p1.eps) != len(p2.eps):
+                raise ValueError("CompGreenRet: p1 and p2 must share the table of dielectric functions")
+            comb = ComParticle(p1.eps, list(p1.p) + list(p2.p), np.vstack([p1.inout, p2.inout]))
+            self._split = (p1.nfaces, p2.nfaces, len(p1.p), len(p2.p))
+            src = comb
+        else:
+            src = p1
+        self.p1, self.p2 = p1, p2
         self.deriv = "cart"
-        self.nearfield = NearField(p1, rel_cutoff=op.get("RelCutoff", 3.0), abs_cutoff=op.get("AbsCutoff", 0.0),
+        self.nearfield = NearField(src, rel_cutoff=op.get("RelCutoff", 3.0), abs_cutoff=op.get("AbsCutoff", 0.0),
                                    order=op.get("order", 5), device=device())
-        self.con = [[c for c in row] for row in connectivity(p1.inout)]
+        self._src = src
+        if self._split is None:
+            self.con = [[c for c in row] for row in connectivity(p1.inout)]
+        else:
+            full = connectivity(src.inout)
+            a = self._split[2]
+            self.con = [[full[r1][r2][:a, a:] for r2 in range(2)] for r1 in range(2)]
         self._geo = None
 
     @property
@@ -86,7 +108,7 @@ class CompGreenRet:
         if self._points is not None:
             raise NotImplementedError("for observer points the device path provides potential and field")
         if self._geo is None:
-            self._geo = DeviceGeometry(self.p1, self.nearfield, device())
+            self._geo = DeviceGeometry(self._src, self.nearfield, device())
         return self._geo
 
     _KEYS = ("G", "F", "H1", "H2", "Gp", "H1p", "H2p")
@@ -111,7 +133,14 @@ class CompGreenRet:
         if not hasattr(self, "_pairs"):
             self._pairs = {}
         if (i, j) not in self._pairs:
-            self._pairs[(i, j)] = torch.tensor(np.ascontiguousarray(self.con[i][j]).ravel().astype(np.int32),
+            t = np.asarray(self.con[i][j])
+            if self._split is not None:
+                # p1's particles first, then p2's: only the p1-row / p2-column block connects
+                a, b = self._split[2], self._split[3]
+                full = np.zeros((a + b, a + b), dtype=t.dtype)
+                full[:a, a:] = t
+                t = full
+            self._pairs[(i, j)] = torch.tensor(np.ascontiguousarray(t).ravel().astype(np.int32),
                                                device=self.geo.dev)
         return self._pairs[(i, j)]
 
@@ -125,7 +154,14 @@ class CompGreenRet:
         return True
 
     def _nested(self):
-        return bool(np.any(self.con[1][0]) or np.any(self.con[0][1]))
+        return self._split is None and bool(np.any(self.con[1][0]) or np.any(self.con[0][1]))
+
+    def _block(self, m):
+        """The p1-row / p2-column block of a matrix over the composite (two particles), else m."""
+        if self._split is None:
+            return m
+        n1 = self._split[0]
+        return m[:n1, ..., n1:]
 
     def eval(self, i, j, key, enei, ind=None):
         """Region-pair matrix, or 0 when the regions do not connect (compgreen_ret.py:507-553).
@@ -137,7 +173,7 @@ class CompGreenRet:
         if not self._check(i, j, key):
             return 0
         if key in ("Gp", "H1p", "H2p"):
-            return self.eval_block(i, j, key, enei, 0, self.geo.n)
+            return self.eval_block(i, j, key, enei, 0, self._dims()[1])
         # the assembly with region pair (i, j)'s table on both sides: G1 = G2 = G(i, j), and
         # H1 = F + 2 pi, H2 = F - 2 pi on the diagonal where the pair connects a face to itself
         geo = self.geo
@@ -150,10 +186,10 @@ class CompGreenRet:
         finally:
             geo.con_in, geo.con_out = tables
         if key == "G":
-            return GreenBatch.to_numpy(gb.G1, 0)
+            return self._block(GreenBatch.to_numpy(gb.G1, 0))
         if key == "F":
-            return 0.5 * (GreenBatch.to_numpy(gb.H1, 0) + GreenBatch.to_numpy(gb.H2, 0))
-        return GreenBatch.to_numpy(gb.H1 if key == "H1" else gb.H2, 0)
+            return self._block(0.5 * (GreenBatch.to_numpy(gb.H1, 0) + GreenBatch.to_numpy(gb.H2, 0)))
+        return self._block(GreenBatch.to_numpy(gb.H1 if key == "H1" else gb.H2, 0))
 
     def eval_block(self, i, j, key, enei, col_start, col_stop):
         """Columns [col_start, col_stop) of ``eval(i, j, key, enei)`` without the full matrix
@@ -163,19 +199,29 @@ class CompGreenRet:
         from .engine import stream_handle
         from .hmatrix import C_addr
         c0, c1 = int(col_start), int(col_stop)
-        n = self.geo.n
-        if not (0 <= c0 <= c1 <= n):
-            raise AssertionError("[error] eval_block col range ({}, {}) out of bounds for p2.n={}".format(c0, c1, n))
+        nr, nc = self._dims()
+        if not (0 <= c0 <= c1 <= nc):
+            raise AssertionError("[error] eval_block col range ({}, {}) out of bounds for p2.n={}".format(c0, c1, nc))
         vec = key in ("Gp", "H1p", "H2p")
-        shape = (n, 3, c1 - c0) if vec else (n, c1 - c0)
+        shape = (nr, 3, c1 - c0) if vec else (nr, c1 - c0)
         if not self._check(i, j, key):
             return np.zeros(shape, complex)
+        n, off = self.geo.n, self._col0()
         g, keep = self._struct(enei, (i, j))
-        out = torch.empty(shape, dtype=torch.complex128, device=self.geo.dev)
-        call("mnp_green_block", C_addr(g), i, self._KEYS.index(key), c0, c1 - c0, out.data_ptr(), stream_handle())
-        res = out.cpu().numpy()
+        out = torch.empty((n, 3, c1 - c0) if vec else (n, c1 - c0), dtype=torch.complex128, device=self.geo.dev)
+        call("mnp_green_block", C_addr(g), i, self._KEYS.index(key), off + c0, c1 - c0, out.data_ptr(),
+             stream_handle())
+        res = out[:nr].cpu().numpy()
         del keep
         return res
+
+    def _dims(self):
+        """Rows (p1 faces) and columns (p2 faces) of a region-pair matrix."""
+        return (self._split[0], self._split[1]) if self._split is not None else (self.geo.n, self.geo.n)
+
+    def _col0(self):
+        """First column of p2's faces in the composite geometry."""
+        return self._split[0] if self._split is not None else 0
 
     def _pick(self, i, j, key, enei, ind):
         """Entries at row-major linear indices (compgreen_ret.py:860-910)."""
@@ -183,13 +229,15 @@ class CompGreenRet:
         from .engine import stream_handle
         from .hmatrix import C_addr
         ind = np.asarray(ind, dtype=np.int64).ravel()
-        n = self.geo.n
+        nr, nc = self._dims()
         vec = key in ("Gp", "H1p", "H2p")
         shape = (len(ind), 3) if vec else (len(ind),)
-        if len(ind) and (ind.min() < 0 or ind.max() >= n * n):
-            raise IndexError("linear index out of range for a {} x {} matrix".format(n, n))
+        if len(ind) and (ind.min() < 0 or ind.max() >= nr * nc):
+            raise IndexError("linear index out of range for a {} x {} matrix".format(nr, nc))
         if not self._check(i, j, key) or len(ind) == 0:
             return np.zeros(shape, complex)
+        if self._split is not None:
+            ind = (ind // nc) * self.geo.n + self._col0() + ind % nc
         g, keep = self._struct(enei, (i, j))
         d_ind = torch.tensor(ind, device=self.geo.dev)
         out = torch.empty(shape, dtype=torch.complex128, device=self.geo.dev)
@@ -210,6 +258,9 @@ class CompGreenRet:
         from .hmatrix import C_addr
         if self._nested():
             raise NotImplementedError("surface potentials and fields of nested particles are not provided")
+        if self._split is not None:
+            raise NotImplementedError("potentials and fields of CompGreenRet(p1, p2) for two particles are not "
+                                      "provided (eval, eval_block and picked entries are)")
         if inout not in (1, 2):
             raise ValueError("inout must be 1 (inside) or 2 (outside)")
         geo, enei = self.geo, float(sig.enei)
