@@ -250,45 +250,56 @@ class CompGreenRet:
     # ------------------------------------------------------------ potentials and fields
     def _surface_eval(self, sig, inout):
         """Per polarisation, the 14 complex columns (phi, phip, a, ap, e, h) of region ``inout``
-        on the faces, from the kernel that evaluates G, Gp and H entry-wise (no matrices). With
-        non-nested particles only the sources of the same region contribute (the cross-region
-        connectivities are empty), so inside uses (sig1, h1) and outside (sig2, h2)."""
+        on p1's faces, from the kernel that evaluates G, Gp and H entry-wise (no matrices):
+        phi = G(r, 1) sig1 + G(r, 2) sig2, e = i k0 (G(r, 1) h1 + G(r, 2) h2) - Hp(r, 1) sig1 -
+        Hp(r, 2) sig2 and so on, with r = inout and the self-term sign of H, Hp set by inout
+        (compgreen_ret.py:911-1034). Each source region is one kernel pass with that region
+        pair's table; for non-nested particles only the pair (r, r) connects."""
         from ._lib import call
         from .engine import stream_handle
         from .hmatrix import C_addr
-        if self._nested():
-            raise NotImplementedError("surface potentials and fields of nested particles are not provided")
-        if self._split is not None:
-            raise NotImplementedError("potentials and fields of CompGreenRet(p1, p2) for two particles are not "
-                                      "provided (eval, eval_block and picked entries are)")
         if inout not in (1, 2):
             raise ValueError("inout must be 1 (inside) or 2 (outside)")
         geo, enei = self.geo, float(sig.enei)
-        s = np.asarray(sig.sig1 if inout == 1 else sig.sig2, complex)
-        h = np.asarray(sig.h1 if inout == 1 else sig.h2, complex)
-        n = geo.n
-        if s.shape[0] != n or h.shape[:2] != (n, 3):
-            raise ValueError("surface charges must have n = {} rows and currents shape (n, 3[, npol])".format(n))
-        single = s.ndim == 1
-        s2 = s.reshape(n, -1)
-        h3 = h.reshape(n, 3, -1)
-        npol = s2.shape[1]
-        if h3.shape[2] != npol:
-            raise ValueError("charges and currents have different numbers of polarisations")
+        nr, nc = self._dims()
+        n, off = geo.n, self._col0()
         dev = geo.dev
-        g, kmat = self._struct(enei)
-        sd = torch.tensor(np.ascontiguousarray(s2.T), dtype=torch.complex128, device=dev)
-        hd = torch.tensor(np.ascontiguousarray(np.transpose(h3, (2, 0, 1))), dtype=torch.complex128, device=dev)
-        out = torch.empty((npol, n, 14), dtype=torch.complex128, device=dev)
         st = stream_handle()
-        for q in range(npol):
-            call("mnp_surface_eval", C_addr(g), inout - 1, 2 * np.pi / enei, sd[q].data_ptr(), hd[q].data_ptr(),
-                 out[q].data_ptr(), st)
-        o = np.transpose(out.cpu().numpy(), (1, 2, 0))     # (n, 14, npol)
-        del kmat
+        r = inout - 1
+        total, single = None, None
+        for src, (s, h) in enumerate(((sig.sig1, sig.h1), (sig.sig2, sig.h2))):
+            s = np.asarray(s, complex)
+            h = np.asarray(h, complex)
+            if s.shape[0] != nc or h.shape[:2] != (nc, 3):
+                raise ValueError("surface charges must have n = {} rows and currents shape (n, 3[, npol])"
+                                 .format(nc))
+            single = s.ndim == 1
+            s2 = s.reshape(nc, -1)
+            h3 = h.reshape(nc, 3, -1)
+            npol = s2.shape[1]
+            if h3.shape[2] != npol:
+                raise ValueError("charges and currents have different numbers of polarisations")
+            if not np.any(self.con[r][src]):
+                continue
+            # sources on p2's faces, placed at their columns of the composite (two particles)
+            sd = torch.zeros((npol, n), dtype=torch.complex128, device=dev)
+            hd = torch.zeros((npol, n, 3), dtype=torch.complex128, device=dev)
+            sd[:, off:off + nc] = torch.tensor(np.ascontiguousarray(s2.T), dtype=torch.complex128, device=dev)
+            hd[:, off:off + nc] = torch.tensor(np.ascontiguousarray(np.transpose(h3, (2, 0, 1))),
+                                               dtype=torch.complex128, device=dev)
+            g, kmat = self._struct(enei, (r, src))
+            out = torch.empty((npol, n, 14), dtype=torch.complex128, device=dev)
+            for q in range(npol):
+                call("mnp_surface_eval", C_addr(g), r, 2 * np.pi / enei, sd[q].data_ptr(), hd[q].data_ptr(),
+                     out[q].data_ptr(), st)
+            o = np.transpose(out[:, :nr].cpu().numpy(), (1, 2, 0))     # (nr, 14, npol)
+            del kmat
+            total = o if total is None else total + o
+        if total is None:
+            total = np.zeros((nr, 14, npol), complex)
         if single:
-            o = o[:, :, 0]
-        return o
+            total = total[:, :, 0]
+        return total
 
     def potential(self, sig, inout=1):
         """Potentials and surface derivatives inside (inout=1) or outside (inout=2) of the

@@ -153,3 +153,51 @@ def test_nested_region_pair_blocks_match_oracle(coreshell, coreshell_oracle, key
             r, c = ind // n, ind % n
             want_p = want[r, :, c] if vec else want[r, c]
             assert np.max(np.abs(picked - want_p)) / scale < 1e-9, (i, j)
+
+
+def _fields_from_matrices(og, ks, k0, sig, inout):
+    """phi, phip, a, ap, e, h of region inout from the oracle's region-pair matrices
+    (compgreen_ret.py:911-1034): both source regions contribute."""
+    r = inout - 1
+    H = "H1" if inout == 1 else "H2"
+    Hp = "H1p" if inout == 1 else "H2p"
+    srcs = ((np.asarray(sig.sig1), np.asarray(sig.h1)), (np.asarray(sig.sig2), np.asarray(sig.h2)))
+    out = dict(phi=0, phip=0, a=0, ap=0, e=0, h=0)
+    for src, (s, h) in enumerate(srcs):
+        G = og.eval(r, src, "G", ks)
+        if np.isscalar(G):
+            continue
+        Hm, P = og.eval(r, src, H, ks), og.eval(r, src, Hp, ks)
+        out["phi"] = out["phi"] + G @ s
+        out["phip"] = out["phip"] + Hm @ s
+        out["a"] = out["a"] + G @ h
+        out["ap"] = out["ap"] + Hm @ h
+        out["e"] = out["e"] + 1j * k0 * (G @ h) - np.einsum("icj,j->ic", P, s)
+        # h_i = sum_j Hp_ij x h_j
+        out["h"] = out["h"] + np.stack([P[:, 1] @ h[:, 2] - P[:, 2] @ h[:, 1],
+                                        P[:, 2] @ h[:, 0] - P[:, 0] @ h[:, 2],
+                                        P[:, 0] @ h[:, 1] - P[:, 1] @ h[:, 0]], axis=1)
+    return out
+
+
+@pytest.mark.parametrize("inout", [1, 2])
+def test_nested_surface_potential_and_field_match_oracle(coreshell, coreshell_oracle, inout):
+    """CompGreenRet.potential / .field on the faces of the core-shell particle from its BEMRet
+    solution: the sources of both regions contribute (the shell medium joins the shell's inside
+    and the core's outside), against the oracle's region-pair matrices."""
+    cp, s, fp, closed = coreshell
+    lam = 530.0
+    banner("nested surface fields inout={}".format(inout))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    sig, bem = M.BEMRet(cp).solve(exc.potential(cp, lam))
+    g = M.CompGreenRet(cp, cp)
+    pot = g.potential(sig, inout)
+    fld = g.field(sig, inout)
+    ks = [e(lam)[1] for e in cp.eps]
+    want = _fields_from_matrices(coreshell_oracle, ks, 2 * np.pi / lam, sig, inout)
+    t = str(inout)
+    got = dict(phi=pot["phi" + t], phip=pot["phi" + t + "p"], a=pot["a" + t], ap=pot["a" + t + "p"], e=fld.e,
+               h=fld.h)
+    for key in want:
+        w = np.asarray(want[key])
+        assert np.max(np.abs(np.asarray(got[key]).reshape(w.shape) - w)) / np.max(np.abs(w)) < 1e-9, key

@@ -85,3 +85,47 @@ def test_close_particles_match_oracle(key):
     # the refined pairs are there: near the gap the entries differ from the point formula
     rows, cols, kind = M.nearfield.refine_pairs(comb.pos, comb.pc.bradius(), comb.pos, comb.pc.bradius())
     assert np.any((rows < n1) & (cols >= n1))
+
+
+@pytest.mark.parametrize("inout", [1, 2])
+def test_two_particle_potential_and_field_match_oracle(inout):
+    """Potentials and fields on p1's faces from surface charges and currents on p2 (a BEMRet
+    solution of p2 alone), region inout of p1: G(r, s) and Hp(r, s) of the p1-row / p2-column
+    blocks (no 2 pi self terms) against the oracle's matrices of the composite surface."""
+    eps, p1, p2 = _pair(1.0)
+    lam = 540.0
+    banner("two particles potential/field inout={}".format(inout))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    sig, _ = M.BEMRet(p2).solve(exc.potential(p2, lam))
+    g = M.CompGreenRet(p1, p2)
+    pot, fld = g.potential(sig, inout), g.field(sig, inout)
+    comb = M.ComParticle(eps, [p1.p[0], p2.p[0]], [[2, 1], [2, 1]])
+    pc = comb.pc
+    og = O.Green(O.Surface(pc.verts, pc.faces, pc.verts2, pc.faces2), comb.face_particle, [[2, 1], [2, 1]], [])
+    ks = [e(lam)[1] for e in eps]
+    n1 = p1.nfaces
+    r = inout - 1
+    Hk, Pk = ("H1", "H1p") if inout == 1 else ("H2", "H2p")
+    k0 = 2 * np.pi / lam
+    want = dict(phi=0, phip=0, a=0, ap=0, e=0, h=0)
+    for src, (s, h) in enumerate(((sig.sig1, sig.h1), (sig.sig2, sig.h2))):
+        G = og.eval(r, src, "G", ks)
+        if np.isscalar(G):
+            continue
+        G = G[:n1, n1:]
+        Hm = og.eval(r, src, Hk, ks)[:n1, n1:]
+        P = og.eval(r, src, Pk, ks)[:n1, :, n1:]
+        want["phi"] = want["phi"] + G @ s
+        want["phip"] = want["phip"] + Hm @ s
+        want["a"] = want["a"] + G @ h
+        want["ap"] = want["ap"] + Hm @ h
+        want["e"] = want["e"] + 1j * k0 * (G @ h) - np.einsum("icj,j->ic", P, s)
+        want["h"] = want["h"] + np.stack([P[:, 1] @ h[:, 2] - P[:, 2] @ h[:, 1], P[:, 2] @ h[:, 0] - P[:, 0] @ h[:, 2],
+                                          P[:, 0] @ h[:, 1] - P[:, 1] @ h[:, 0]], axis=1)
+    t = str(inout)
+    got = dict(phi=pot["phi" + t], phip=pot["phi" + t + "p"], a=pot["a" + t], ap=pot["a" + t + "p"], e=fld.e,
+               h=fld.h)
+    for key, w in want.items():
+        w = np.asarray(w)
+        assert np.asarray(got[key]).shape == w.shape, key
+        assert np.max(np.abs(got[key] - w)) / np.max(np.abs(w)) < 1e-9, key
