@@ -102,9 +102,9 @@ class PointGreen:
     """Device data of the Green function between a ComPoint and a ComParticle."""
 
     def __init__(self, pt, p, dev, rel_cutoff=3.0, abs_cutoff=0.0, order=ORDER):
-        io = np.asarray(p.inout)
-        if np.any(io[:, 1][:, None] == io[:, 0][None, :]):
-            raise NotImplementedError("fields at points of nested particles are not provided")
+        # a point in medium m sees every face that touches m, through sig1, h1 where m is the
+        # face's inside and sig2, h2 where it is its outside; for nested particles a point in the
+        # shell medium so sees the shell's inside and the core's outside (compgreen_ret.py:435-505)
         self.pt, self.p, self.dev, self.order = pt, p, dev, order
         rows, cols = point_refine_pairs(pt, p, rel_cutoff, abs_cutoff)
         g, f = point_coefficients(pt, p, rows, cols, order)

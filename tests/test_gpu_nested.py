@@ -201,3 +201,28 @@ def test_nested_surface_potential_and_field_match_oracle(coreshell, coreshell_or
     for key in want:
         w = np.asarray(want[key])
         assert np.max(np.abs(np.asarray(got[key]).reshape(w.shape) - w)) / np.max(np.abs(w)) < 1e-9, key
+
+
+def test_nested_point_fields_match_oracle(coreshell):
+    """Fields and potentials at points in the core, the shell and outside (ComPoint,
+    CompGreenRet(pt, p)): a point in the shell medium sees the shell faces through sig1, h1 and
+    the core faces through sig2, h2; against the oracle's point fields."""
+    cp, s, fp, closed = coreshell
+    lam = 560.0
+    rng = np.random.default_rng(9)
+    d = rng.standard_normal((36, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rr = np.concatenate([rng.uniform(1.0, 4.5, 12), rng.uniform(6.8, 9.0, 12), rng.uniform(11.0, 30.0, 12)])
+    pts = d * rr[:, None]
+    banner("nested point fields npt={}".format(len(pts)))
+    exc = M.PlaneWaveRet([1.0, 0, 0], [0, 0, 1.0])
+    sig, _ = M.BEMRet(cp).solve(exc.potential(cp, lam))
+    pt = M.ComPoint(cp, pts)
+    assert set(np.unique(pt.medium)) == {1, 2, 3}
+    assert np.array_equal(pt.medium, O.point_media(s, fp, INOUT, pt.pos))
+    g = M.CompGreenRet(pt, cp)
+    fld, pot = g.field(sig), g.potential(sig)
+    ks = [e(lam)[1] for e in cp.eps]
+    ref = O.point_potential_field(s, fp, INOUT, pt.pos, ks, lam, sig.sig1, sig.h1, sig.sig2, sig.h2)
+    for got, want in ((fld.e, ref["e"]), (fld.h, ref["h"]), (pot.phi1, ref["phi"]), (pot.a1, ref["a"])):
+        assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
