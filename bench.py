@@ -77,7 +77,9 @@ def _oracle_sweep(cp, lams, pol, direc, eps_out):
     from oracle import bem_oracle as O
     p = cp.pc
     s = O.Surface(p.verts, p.faces, p.verts2, p.faces2)
-    green = O.Green(s, np.zeros(s.n, int), [[2, 1]], [(np.arange(s.n), np.arange(s.n), 1.0)])
+    fp = np.asarray(cp.face_particle)
+    closed = [(np.nonzero(fp == a)[0], np.nonzero(fp == a)[0], 1.0) for a in range(len(cp.p))]
+    green = O.Green(s, fp, np.asarray(cp.inout), closed)
     pol, direc = np.asarray(pol, float), np.asarray(direc, float)
     t0 = time.perf_counter()
     for lam in lams:
@@ -113,6 +115,56 @@ def cpu_baseline(cp, nsample, nsample1=1):
                 sample="{} of the 201 wavelengths at n={} with {} BLAS threads on {} host CPUs ({:.1f} s), and {} "
                        "on one core ({:.1f} s): assembly + LU chain + solve + extinction per wavelength, geometry "
                        "set-up excluded".format(len(lams), cp.nfaces, blas, affinity, dt, nsample1, dt1))
+
+
+def _fit_power(ns, ts):
+    """t = c n^a through the samples (least squares in log-log)."""
+    a, lc = np.polyfit(np.log(np.asarray(ns, float)), np.log(np.asarray(ts, float)), 1)
+    return float(a), float(np.exp(lc))
+
+
+def cpu_fitted_configs():
+    """CPU baselines of the 64-particle set and of the 24.5k-face rod dimer, from bounded samples
+    of the oracle (numpy + the host's LAPACK, all host threads) scaled by a fitted power law of the
+    face count (stated as a fit: the full runs take hours on the host). Each sample is the
+    per-wavelength loop of one plane-wave solve at 600 nm (assembly + LU chain + solve +
+    extinction), after the wavelength-independent set-up.
+      batched64: the particles at the 0th, 20th and 40th percentile of the face counts, each
+        timed once; total = sum over the 64 particles of 32 x t(n_p); solves/s = 2048 / total.
+      gmres25k: the same rod dimer geometry with fewer faces (two sizes), dense direct solve,
+        extrapolated to the dimer's face count (the reference's H-matrix GMRES at 25k faces has
+        no published time; it timed out after > 5 min).
+    Small samples keep the run short; their exponent is below the LU's 3 (the O(n^2) assembly
+    weighs more at small n), so the fits understate the CPU time of the large cases."""
+    import mnpbem_amd as M
+    out = {}
+    parts = particles64()
+    order = np.argsort([cp.nfaces for _, cp in parts])
+    sample = [parts[i][1] for i in (order[0], order[len(order) // 5], order[2 * len(order) // 5])]
+    ns, ts = [], []
+    for cp in sample:
+        ns.append(cp.nfaces)
+        ts.append(_oracle_sweep(cp, [600.0], [1.0, 0, 0], [0, 0, 1.0], 1.33))
+    a, c = _fit_power(ns, ts)
+    total = sum(32 * c * cp.nfaces ** a for _, cp in parts)
+    out["batched64"] = dict(value=round(2048 / total, 4), unit="solves/s", kind="port", fitted=True,
+                            exponent=round(a, 3), sample_faces=ns, sample_s=[round(t, 3) for t in ts],
+                            total_s=round(total, 1))
+    _progress("cpu batched64 fit: {}".format(out["batched64"]))
+    ns, ts = [], []
+    for counts in ([10, 5, 12], [15, 6, 18]):
+        p1 = M.trirod(20, 80, counts, triangles=True).shift([0, 0, -41.0])
+        p2 = M.trirod(20, 80, counts, triangles=True).shift([0, 0, 41.0])
+        cp = M.ComParticle([M.EpsConst(1.0), M.EpsDrude(*DRUDE)], [p1, p2], [[2, 1], [2, 1]], 1, 2, interp="curv")
+        ns.append(cp.nfaces)
+        ts.append(_oracle_sweep(cp, [750.0], [0, 0, 1.0], [1.0, 0, 0], 1.0))
+    a, c = _fit_power(ns, ts)
+    n25 = 24544
+    out["gmres25k"] = dict(value=round(c * n25 ** a, 1), unit="s (dense direct solve, one wavelength)",
+                           kind="port", fitted=True, exponent=round(a, 3), faces=n25, sample_faces=ns,
+                           sample_s=[round(t, 3) for t in ts])
+    _progress("cpu gmres25k fit: {}".format(out["gmres25k"]))
+    return out
 
 
 def config1(skip_cpu=False):
@@ -564,6 +616,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(cp, args.cpu_sample)
         _progress("cpu baseline: {}".format(cpu))
+        if not (args.no_batched64 and args.no_gmres25k):
+            cpu["other_configs"] = cpu_fitted_configs()
 
     line = {
         "metric": METRIC,
