@@ -117,53 +117,32 @@ def cpu_baseline(cp, nsample, nsample1=1):
                        "set-up excluded".format(len(lams), cp.nfaces, blas, affinity, dt, nsample1, dt1))
 
 
-def _fit_power(ns, ts):
-    """t = c n^a through the samples (least squares in log-log)."""
-    a, lc = np.polyfit(np.log(np.asarray(ns, float)), np.log(np.asarray(ts, float)), 1)
-    return float(a), float(np.exp(lc))
-
-
-def cpu_fitted_configs():
-    """CPU baselines of the 64-particle set and of the 24.5k-face rod dimer, from bounded samples
-    of the oracle (numpy + the host's LAPACK, all host threads) scaled by a fitted power law of the
-    face count (stated as a fit: the full runs take hours on the host). Each sample is the
-    per-wavelength loop of one plane-wave solve at 600 nm (assembly + LU chain + solve +
-    extinction), after the wavelength-independent set-up.
-      batched64: the particles at the 0th, 20th and 40th percentile of the face counts, each
-        timed once; total = sum over the 64 particles of 32 x t(n_p); solves/s = 2048 / total.
-      gmres25k: the same rod dimer geometry with fewer faces (two sizes), dense direct solve,
-        extrapolated to the dimer's face count (the reference's H-matrix GMRES at 25k faces has
-        no published time; it timed out after > 5 min).
-    Small samples keep the run short; their exponent is below the LU's 3 (the O(n^2) assembly
-    weighs more at small n), so the fits understate the CPU time of the large cases."""
-    import mnpbem_amd as M
-    out = {}
+def cpu_other_configs():
+    """CPU references for the 64-particle set and the 24.5k-face rod dimer, whose full runs take
+    hours on the host, as a measured sample and a bound rather than an extrapolation:
+      batched64: the oracle (all host threads) on the particles at the 0th, 20th and 40th
+        percentile of the face counts, one wavelength (600 nm) each, per-wavelength loop after
+        the wavelength-independent set-up; solves/s over the sample is an upper bound for the
+        whole set (its other particles are larger).
+      gmres25k: a lower bound for a dense direct solve at 24,544 faces on the host: the LU
+        chain's 34.67 n^3 flops at the host's measured complex128 GEMM rate (numpy matmul of
+        2048 x 2048 matrices); the reference's H-matrix GMRES at 25k faces has no published time
+        (it timed out after > 5 min)."""
     parts = particles64()
     order = np.argsort([cp.nfaces for _, cp in parts])
     sample = [parts[i][1] for i in (order[0], order[len(order) // 5], order[2 * len(order) // 5])]
-    ns, ts = [], []
-    for cp in sample:
-        ns.append(cp.nfaces)
-        ts.append(_oracle_sweep(cp, [600.0], [1.0, 0, 0], [0, 0, 1.0], 1.33))
-    a, c = _fit_power(ns, ts)
-    total = sum(32 * c * cp.nfaces ** a for _, cp in parts)
-    out["batched64"] = dict(value=round(2048 / total, 4), unit="solves/s", kind="port", fitted=True,
-                            exponent=round(a, 3), sample_faces=ns, sample_s=[round(t, 3) for t in ts],
-                            total_s=round(total, 1))
-    _progress("cpu batched64 fit: {}".format(out["batched64"]))
-    ns, ts = [], []
-    for counts in ([10, 5, 12], [15, 6, 18]):
-        p1 = M.trirod(20, 80, counts, triangles=True).shift([0, 0, -41.0])
-        p2 = M.trirod(20, 80, counts, triangles=True).shift([0, 0, 41.0])
-        cp = M.ComParticle([M.EpsConst(1.0), M.EpsDrude(*DRUDE)], [p1, p2], [[2, 1], [2, 1]], 1, 2, interp="curv")
-        ns.append(cp.nfaces)
-        ts.append(_oracle_sweep(cp, [750.0], [0, 0, 1.0], [1.0, 0, 0], 1.0))
-    a, c = _fit_power(ns, ts)
-    n25 = 24544
-    out["gmres25k"] = dict(value=round(c * n25 ** a, 1), unit="s (dense direct solve, one wavelength)",
-                           kind="port", fitted=True, exponent=round(a, 3), faces=n25, sample_faces=ns,
-                           sample_s=[round(t, 3) for t in ts])
-    _progress("cpu gmres25k fit: {}".format(out["gmres25k"]))
+    ts = [_oracle_sweep(cp, [600.0], [1.0, 0, 0], [0, 0, 1.0], 1.33) for cp in sample]
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal((2048, 2048)) + 1j * rng.standard_normal((2048, 2048))
+    A @ A
+    t0 = time.perf_counter()
+    A @ A
+    rate = 8.0 * 2048 ** 3 / (time.perf_counter() - t0)
+    out = dict(batched64=dict(value=round(len(ts) / sum(ts), 4), unit="solves/s", kind="port", bound="upper",
+                              sample_faces=[cp.nfaces for cp in sample], sample_s=[round(t, 3) for t in ts]),
+               gmres25k=dict(value=round(34.67 * 24544.0 ** 3 / rate, 1), unit="s (dense direct solve)", kind="port",
+                             bound="lower", host_zgemm_gflops=round(rate / 1e9, 1)))
+    _progress("cpu other configs: {}".format(out))
     return out
 
 
@@ -617,7 +596,7 @@ def main():
         cpu = cpu_baseline(cp, args.cpu_sample)
         _progress("cpu baseline: {}".format(cpu))
         if not (args.no_batched64 and args.no_gmres25k):
-            cpu["other_configs"] = cpu_fitted_configs()
+            cpu["other_configs"] = cpu_other_configs()
 
     line = {
         "metric": METRIC,
