@@ -19,8 +19,9 @@
 // wavelength at stride `stride` elements. One launch assembles the four matrices for every
 // wavelength of the batch: a workgroup of 256 lanes owns R rows x (256 / R) columns of entries
 // (lanes along the rows: each store instruction writes R consecutive elements of one column)
-// and a chunk of LB wavelengths (grid z); the geometry terms of an entry are computed once and
-// reused for every wavelength of its chunk. R and LB are tuning knobs of the handle (small
+// and a chunk of LB wavelengths (grid z); the tile's observer and source face data are staged in
+// LDS, and the geometry terms of an entry are computed once and reused for every wavelength of
+// its chunk. R and LB are tuning knobs of the handle (small
 // matrices fill the GPU only with short wavelength chunks); the autotuner times them per shape.
 #include "common.hpp"
 #include "args.hpp"
@@ -34,17 +35,40 @@ constexpr double TWO_PI = 6.283185307179586476925286766559;
 template <typename OT, int R>
 __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, int lb, cx<OT>* G1, cx<OT>* G2, cx<OT>* H1,
                                                      cx<OT>* H2) {
-    int i = blockIdx.x * R + (threadIdx.x % R);
-    int j = blockIdx.y * (256 / R) + (threadIdx.x / R);
+    constexpr int C = 256 / R;
+    // the workgroup's R observer faces (centroid, normal, particle) and C source faces
+    // (centroid, area, particle), staged in LDS once and read by every entry of the tile
+    __shared__ double ro[R][6];
+    __shared__ double cs[C][4];
+    __shared__ int rpart[R], cpart[C];
+    const int t = threadIdx.x;
+    const int i0 = blockIdx.x * R, j0 = blockIdx.y * C;
+    if (t < R && i0 + t < a.n) {
+        const int i = i0 + t;
+        for (int c = 0; c < 3; ++c) {
+            ro[t][c] = a.pos[3 * i + c];
+            ro[t][3 + c] = a.nvec[3 * i + c];
+        }
+        rpart[t] = a.fpart[i];
+    }
+    if (t < C && j0 + t < a.n) {
+        const int j = j0 + t;
+        for (int c = 0; c < 3; ++c) cs[t][c] = a.pos[3 * j + c];
+        cs[t][3] = a.area[j];
+        cpart[t] = a.fpart[j];
+    }
+    __syncthreads();
+    const int li = t % R, lj = t / R;
+    const int i = i0 + li, j = j0 + lj;
     if (i >= a.n || j >= a.n) return;
     const int l0 = blockIdx.z * lb, l1 = min(a.nlam, l0 + lb);
-    double rx = a.pos[3 * i] - a.pos[3 * j];
-    double ry = a.pos[3 * i + 1] - a.pos[3 * j + 1];
-    double rz = a.pos[3 * i + 2] - a.pos[3 * j + 2];
+    double rx = ro[li][0] - cs[lj][0];
+    double ry = ro[li][1] - cs[lj][1];
+    double rz = ro[li][2] - cs[lj][2];
     double d = sqrt(rx * rx + ry * ry + rz * rz);
-    double nr = a.nvec[3 * i] * rx + a.nvec[3 * i + 1] * ry + a.nvec[3 * i + 2] * rz;
-    double aj = a.area[j];
-    int pa = a.fpart[i], pb = a.fpart[j];
+    double nr = ro[li][3] * rx + ro[li][4] * ry + ro[li][5] * rz;
+    double aj = cs[lj][3];
+    int pa = rpart[li], pb = cpart[lj];
     int c1 = a.con_in[pa * a.npart + pb];
     int c2 = a.con_out[pa * a.npart + pb];
     bool diag = (i == j);
