@@ -93,6 +93,16 @@ int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const do
                        const double* ref_f, int prec, void* G1, void* G2, void* H1, void* H2, long ld, long stride,
                        void* stream);
 
+/* The same entries as mnp_green_assemble once in complex double (G1, G2, H1, H2 at ld, stride)
+ * and once in complex float (G1s, G2s, H1s, H2s at lds, strides) from one evaluation of each
+ * exponential: the exact operator and the factor matrices of a mixed-precision BEMRet sweep.
+ * Replaces the same CompGreenRet.eval calls (compgreen_ret.py:739-858, bem_ret.py:407-433). */
+int mnp_green_assemble_pair(int n, int npart, int nmat, int nlam, int order, const double* pos, const double* nvec,
+                            const double* area, const int* fpart, const int* con_in, const int* con_out,
+                            const double* kmat, int nref, const int* ref_row, const int* ref_col, const double* ref_g,
+                            const double* ref_f, void* G1, void* G2, void* H1, void* H2, long ld, long stride,
+                            void* G1s, void* G2s, void* H1s, void* H2s, long lds, long strides, void* stream);
+
 /* C -= A * B (m x k times k x n), batched. Matrix-core kernel behind the LU and solves.
  * Replaces the cupy/numpy GEMMs of bem_ret.py:472-495 / :844-886. */
 int mnp_gemm_sub_batched(int prec, int m, int n, int k, const void* A, long lda, long sA, const void* B, long ldb,

@@ -27,6 +27,8 @@ _SIGS = {
     "mnp_range_push": [C.c_char_p],
     "mnp_range_pop": [],
     "mnp_green_assemble": [I, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, I, P, P, P, P, L, L, P],
+    "mnp_green_assemble_pair": [I, I, I, I, I, P, P, P, P, P, P, P, I, P, P, P, P, P, P, P, P, L, L, P, P, P, P, L, L,
+                                P],
     "mnp_gemm_sub_batched": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
     "mnp_gemm_sub_batched_noalloc": [I, I, I, I, P, L, L, P, L, L, P, L, L, I, P],
     "mnp_noalloc_scratch_bytes": [P],

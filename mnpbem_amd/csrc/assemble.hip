@@ -32,9 +32,20 @@ namespace mnp {
 constexpr double TWO_PI = 6.283185307179586476925286766559;
 
 
-template <typename OT, int R>
+// Second output set of the pair assembly: complex-float copies of the same entries (the factor
+// matrices of the mixed chain next to the complex128 ones of its exact operator), at their own
+// leading dimension and stride; one evaluation of each e^{ikd} serves both.
+struct AsmOut2 {
+    cx<float>* G1;
+    cx<float>* G2;
+    cx<float>* H1;
+    cx<float>* H2;
+    long ld, stride;
+};
+
+template <typename OT, int R, bool DUAL = false>
 __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, int lb, cx<OT>* G1, cx<OT>* G2, cx<OT>* H1,
-                                                     cx<OT>* H2) {
+                                                     cx<OT>* H2, AsmOut2 o2 = AsmOut2{}) {
     constexpr int C = 256 / R;
     // the workgroup's R observer faces (centroid, normal, particle) and C source faces
     // (centroid, area, particle), staged in LDS once and read by every entry of the tile
@@ -74,34 +85,41 @@ __global__ __launch_bounds__(256) void k_assemble_far(AsmArgs a, int lb, cx<OT>*
     bool diag = (i == j);
     double inv_d = diag ? 0.0 : 1.0 / d;
     long off = (long)i + (long)j * a.ld;
+    const long off2 = (long)i + (long)j * o2.ld;
     const cx<double>* kk = reinterpret_cast<const cx<double>*>(a.kmat);
     for (int l = l0; l < l1; ++l) {
         long o = off + (long)l * a.stride;
+        const long q = off2 + (long)l * o2.stride;
         for (int side = 0; side < 2; ++side) {
             const int c = side ? c2 : c1;
             const int m = c < 0 ? -c : c;
             const double sgn = c < 0 ? -1.0 : 1.0;
-            cx<OT> g = mk<OT>(0, 0), f = mk<OT>(0, 0);
+            cx<double> gg = mk(0.0, 0.0), ff = mk(0.0, 0.0);
             if (m > 0 && !diag) {
                 cx<double> k = kk[l * a.nmat + m - 1];
                 cx<double> ph = expikd(k, d);
-                cx<double> gg = (sgn * aj * inv_d) * ph;
+                gg = (sgn * aj * inv_d) * ph;
                 // (i k - 1/d) / d^2 * nr * a_j
                 cx<double> fac = mk(-k.im - inv_d, k.re);
-                cx<double> ff = (sgn * nr * aj * inv_d * inv_d) * (fac * ph);
-                g = conv<OT>(gg);
-                f = conv<OT>(ff);
+                ff = (sgn * nr * aj * inv_d * inv_d) * (fac * ph);
             }
+            const cx<OT> g = conv<OT>(gg), f = conv<OT>(ff);
             if (side == 0) { G1[o] = g; H1[o] = f; }
             else { G2[o] = g; H2[o] = f; }
+            if constexpr (DUAL) {
+                const cx<float> gs = conv<float>(gg), fs = conv<float>(ff);
+                if (side == 0) { o2.G1[q] = gs; o2.H1[q] = fs; }
+                else { o2.G2[q] = gs; o2.H2[q] = fs; }
+            }
         }
     }
 }
 
-template <typename OT>
+template <typename OT, bool DUAL = false>
 __global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int lb, int nref, const int* rrow,
                                                          const int* rcol, const double* rg, const double* rf,
-                                                         cx<OT>* G1, cx<OT>* G2, cx<OT>* H1, cx<OT>* H2) {
+                                                         cx<OT>* G1, cx<OT>* G2, cx<OT>* H1, cx<OT>* H2,
+                                                         AsmOut2 o2 = AsmOut2{}) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nref) return;
     const int l0 = blockIdx.y * lb, l1 = min(a.nlam, l0 + lb);
@@ -145,13 +163,19 @@ __global__ __launch_bounds__(256) void k_assemble_refined(AsmArgs a, int lb, int
             if (diag) f.re += side ? -TWO_PI : TWO_PI;
             if (side == 0) { G1[o] = conv<OT>(g); H1[o] = conv<OT>(f); }
             else { G2[o] = conv<OT>(g); H2[o] = conv<OT>(f); }
+            if constexpr (DUAL) {
+                const long q = (long)i + (long)j * o2.ld + (long)l * o2.stride;
+                if (side == 0) { o2.G1[q] = conv<float>(g); o2.H1[q] = conv<float>(f); }
+                else { o2.G2[q] = conv<float>(g); o2.H2[q] = conv<float>(f); }
+            }
         }
     }
 }
 
-template <typename OT>
-void launch_assemble(const AsmArgs& a, int nref, const int* rrow, const int* rcol, const double* rg,
-                     const double* rf, void* G1, void* G2, void* H1, void* H2, hipStream_t s) {
+template <typename OT, bool DUAL>
+static void launch_assemble_impl(const AsmArgs& a, int nref, const int* rrow, const int* rcol, const double* rg,
+                                 const double* rf, void* G1, void* G2, void* H1, void* H2, const AsmOut2& o2,
+                                 hipStream_t s) {
     const Ctx& h = ctx();
     const int R = h.asm_rows;
     const int lb = (h.asm_lchunk > 0 && h.asm_lchunk < a.nlam) ? h.asm_lchunk : a.nlam;
@@ -162,14 +186,33 @@ void launch_assemble(const AsmArgs& a, int nref, const int* rrow, const int* rco
     cx<OT>* h1 = (cx<OT>*)H1;
     cx<OT>* h2 = (cx<OT>*)H2;
     switch (R) {
-        case 128: hipLaunchKernelGGL((k_assemble_far<OT, 128>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2); break;
-        case 256: hipLaunchKernelGGL((k_assemble_far<OT, 256>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2); break;
-        default: hipLaunchKernelGGL((k_assemble_far<OT, 64>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2);
+        case 128:
+            hipLaunchKernelGGL((k_assemble_far<OT, 128, DUAL>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2, o2);
+            break;
+        case 256:
+            hipLaunchKernelGGL((k_assemble_far<OT, 256, DUAL>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2, o2);
+            break;
+        default:
+            hipLaunchKernelGGL((k_assemble_far<OT, 64, DUAL>), grid, dim3(256), 0, s, a, lb, g1, g2, h1, h2, o2);
     }
     if (nref > 0) {
-        hipLaunchKernelGGL(k_assemble_refined<OT>, dim3((nref + 255) / 256, nz), dim3(256), 0, s, a, lb, nref, rrow,
-                           rcol, rg, rf, g1, g2, h1, h2);
+        hipLaunchKernelGGL((k_assemble_refined<OT, DUAL>), dim3((nref + 255) / 256, nz), dim3(256), 0, s, a, lb,
+                           nref, rrow, rcol, rg, rf, g1, g2, h1, h2, o2);
     }
+}
+
+template <typename OT>
+void launch_assemble(const AsmArgs& a, int nref, const int* rrow, const int* rcol, const double* rg,
+                     const double* rf, void* G1, void* G2, void* H1, void* H2, hipStream_t s) {
+    launch_assemble_impl<OT, false>(a, nref, rrow, rcol, rg, rf, G1, G2, H1, H2, AsmOut2{}, s);
+}
+
+// complex128 matrices and their complex-float copies from one pass
+void launch_assemble_pair(const AsmArgs& a, int nref, const int* rrow, const int* rcol, const double* rg,
+                          const double* rf, void* G1, void* G2, void* H1, void* H2, void* G1s, void* G2s, void* H1s,
+                          void* H2s, long lds, long strides, hipStream_t s) {
+    const AsmOut2 o2{(cx<float>*)G1s, (cx<float>*)G2s, (cx<float>*)H1s, (cx<float>*)H2s, lds, strides};
+    launch_assemble_impl<double, true>(a, nref, rrow, rcol, rg, rf, G1, G2, H1, H2, o2, s);
 }
 
 // Assembly knobs of the calling thread's handle: rows per workgroup (64, 128, 256) and

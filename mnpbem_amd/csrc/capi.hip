@@ -19,6 +19,8 @@ namespace mnp {
 template <typename OT>
 void launch_assemble(const AsmArgs&, int, const int*, const int*, const double*, const double*, void*, void*, void*,
                      void*, hipStream_t);
+void launch_assemble_pair(const AsmArgs&, int, const int*, const int*, const double*, const double*, void*, void*,
+                          void*, void*, void*, void*, void*, void*, long, long, hipStream_t);
 template <typename T>
 void gemm_sub(int, int, int, const void*, long, long, const void*, long, long, void*, long, long, int, hipStream_t);
 template <typename T>
@@ -179,6 +181,32 @@ int mnp_green_assemble(int n, int npart, int nmat, int nlam, int order, const do
         launch_assemble<double>(a, nref, ref_row, ref_col, ref_g, ref_f, G1, G2, H1, H2, s);
     else
         launch_assemble<float>(a, nref, ref_row, ref_col, ref_g, ref_f, G1, G2, H1, H2, s);
+    return launch_status();
+}
+
+int mnp_green_assemble_pair(int n, int npart, int nmat, int nlam, int order, const double* pos, const double* nvec,
+                            const double* area, const int* fpart, const int* con_in, const int* con_out,
+                            const double* kmat, int nref, const int* ref_row, const int* ref_col, const double* ref_g,
+                            const double* ref_f, void* G1, void* G2, void* H1, void* H2, long ld, long stride,
+                            void* G1s, void* G2s, void* H1s, void* H2s, long lds, long strides, void* stream) {
+    StageRange range_("mnp:assemble");
+    REQUIRE(n > 0, 1);
+    REQUIRE(npart > 0, 2);
+    REQUIRE(nmat > 0, 3);
+    REQUIRE(nlam > 0, 4);
+    REQUIRE(order >= 0 && order <= 7, 5);
+    REQUIRE(pos && nvec && area && fpart && con_in && con_out && kmat, 6);
+    REQUIRE(nref >= 0, 13);
+    REQUIRE(nref == 0 || (ref_row && ref_col && ref_g && ref_f), 14);
+    REQUIRE(G1 && G2 && H1 && H2, 18);
+    REQUIRE(ld >= n, 22);
+    REQUIRE(stride >= ld * (long)n || nlam == 1, 23);
+    REQUIRE(G1s && G2s && H1s && H2s, 24);
+    REQUIRE(lds >= n, 28);
+    REQUIRE(strides >= lds * (long)n || nlam == 1, 29);
+    AsmArgs a{n, npart, nmat, nlam, order, pos, nvec, area, fpart, con_in, con_out, kmat, ld, stride};
+    launch_assemble_pair(a, nref, ref_row, ref_col, ref_g, ref_f, G1, G2, H1, H2, G1s, G2s, H1s, H2s, lds, strides,
+                         (hipStream_t)stream);
     return launch_status();
 }
 

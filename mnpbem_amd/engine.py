@@ -162,6 +162,14 @@ class GreenBatch:
              _p(g.ref_g), _p(g.ref_f), self.prec, _p(self.G1), _p(self.G2), _p(self.H1), _p(self.H2),
              self.ld, self.stride, stream_handle())
 
+    def assemble_pair(self, other, kmat, nlam):
+        """This (complex128) batch and its complex-float copy ``other`` from one pass."""
+        g = self.geo
+        call("mnp_green_assemble_pair", g.n, g.npart, g.nmat, nlam, g.order, _p(g.pos), _p(g.nvec), _p(g.area),
+             _p(g.fpart), _p(g.con_in), _p(g.con_out), _p(kmat), g.nref, _p(g.ref_row), _p(g.ref_col),
+             _p(g.ref_g), _p(g.ref_f), _p(self.G1), _p(self.G2), _p(self.H1), _p(self.H2), self.ld, self.stride,
+             _p(other.G1), _p(other.G2), _p(other.H1), _p(other.H2), other.ld, other.stride, stream_handle())
+
     @staticmethod
     def to_numpy(t, b):
         """Column-major device matrix b -> (n, n) numpy array."""
@@ -211,8 +219,17 @@ class RetSolverBatch:
         self.kmat, self.L1, self.L2, self.kvac, self.kmat_np, self.eps_np = g.media(enei_list)
         self.enei = enei_list
         self.nlam = nl
-        for gb in ((self.exact, self.fac) if self.exact is not None else (self.fac,)):
-            self._assemble_tuned(gb, nl)
+        if self.exact is not None:
+            # mixed precision: the exact complex128 operator and the complex-float factor
+            # matrices from one pass (each exponential evaluated once)
+            from .autotune import knobs, tune_assembly
+            key = str(C128)
+            if self.tune_assembly and key not in self.asm_cfg:
+                self.asm_cfg[key] = tune_assembly(self.exact, self.kmat, nl)
+            with knobs(self.asm_cfg.get(key)):
+                self.exact.assemble_pair(self.fac, self.kmat, nl)
+        else:
+            self._assemble_tuned(self.fac, nl)
         return self._factor_assembled()
 
     def _assemble_tuned(self, gb, nl):

@@ -350,3 +350,28 @@ def test_quadrature_options_match_oracle(rule, refine):
     ref = out["fields"][0]["sol"][2]
     assert np.max(np.abs(sig.sig2 - ref)) / np.max(np.abs(ref)) < 1e-10
     assert abs(ext - ext0) / abs(ext0) > 1e-9
+
+
+def test_pair_assembly_equals_separate_passes():
+    """mnp_green_assemble_pair writes the complex128 matrices and their complex-float copies
+    from one pass: both equal the separate assemblies bit for bit (dimer, refined pairs,
+    several wavelengths)."""
+    from mnpbem_amd.engine import GreenBatch
+    a = M.trisphere(60, 10)
+    b = M.trisphere(60, 10).shift([0, 0, 11.0])
+    cp = M.ComParticle([M.EpsConst(1.33), M.EpsDrude(10.0, 9.065, 0.0708)], [a, b], [[2, 1], [2, 1]], 1, 2,
+                       interp="curv")
+    banner("pair assembly n={}".format(cp.nfaces))
+    g = M.CompGreenRet(cp, cp)
+    geo = g.geo
+    lams = np.array([480.0, 560.0, 700.0])
+    kmat = geo.media(lams)[0]
+    d, f = GreenBatch(geo, 3, torch.complex128), GreenBatch(geo, 3, torch.complex64)
+    d.assemble(kmat, 3)
+    f.assemble(kmat, 3)
+    d2, f2 = GreenBatch(geo, 3, torch.complex128), GreenBatch(geo, 3, torch.complex64)
+    d2.assemble_pair(f2, kmat, 3)
+    torch.cuda.synchronize()
+    for x, y in ((d, d2), (f, f2)):
+        for name in ("G1", "G2", "H1", "H2"):
+            assert torch.equal(getattr(x, name), getattr(y, name)), name
